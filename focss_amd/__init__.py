@@ -1,0 +1,163 @@
+"""focss_amd -- MI355X (gfx950) implementation of focss's split-step Fourier
+hot path behind a C-ABI (include/focss.h).
+
+This module is a thin ctypes view of the in-tree HIP library
+focss_amd/libfocss_hip.so.  It raises at import if the library is missing:
+there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfocss_hip.so")
+
+F64, F32 = 0, 1
+UNIFORM, LOGARITHMIC, ADAPTIVE = 0, 1, 2
+FORWARD, BACKWARD = 1.0, -1.0
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"focss_amd: {LIB_PATH} is not built; run `python -m focss_amd.build`")
+
+_lib = C.CDLL(LIB_PATH)
+
+
+class FocssError(RuntimeError):
+    pass
+
+
+class Fiber(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta2", C.c_double), ("gamma", C.c_double),
+                ("length", C.c_double), ("wavelength", C.c_double), ("kappa", C.c_double)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("grid", C.c_int), ("total_steps", C.c_int),
+                ("max_phase_shift", C.c_double), ("max_step_size", C.c_double)]
+
+
+class Link(C.Structure):
+    _fields_ = [("fiber", Fiber), ("grid", Grid), ("direction", C.c_double),
+                ("n_spans", C.c_int), ("modes", C.c_int), ("samples", C.c_int),
+                ("pad", C.c_int), ("dt", C.c_double), ("center_frequency", C.c_double),
+                ("span_pre_scale", C.c_double), ("span_post_scale", C.c_double),
+                ("noise_sigma", C.c_double), ("noise_seed", C.c_uint64)]
+
+
+_P = C.c_void_p
+_sig = {
+    "focss_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "focss_ctx_destroy": (C.c_int, [_P]),
+    "focss_last_error": (C.c_char_p, []),
+    "focss_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "focss_device_free": (C.c_int, [_P, _P]),
+    "focss_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "focss_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "focss_synchronize": (C.c_int, [_P]),
+    "focss_link_run": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, _P]),
+    "focss_link_run_host": (C.c_int, [_P, C.c_int, C.POINTER(Link), _P, _P]),
+    "focss_fft": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "focss_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "focss_tile_variants": (C.c_int, [C.c_int, C.c_int]),
+    "focss_build_info": (C.c_char_p, []),
+}
+for _name, (_res, _args) in _sig.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = tuple(_sig)
+
+
+def _check(rc):
+    if rc != 0:
+        raise FocssError(f"focss error {rc}: {_lib.focss_last_error().decode()}")
+
+
+def build_info():
+    return _lib.focss_build_info().decode()
+
+
+def make_link(fiber, grid, direction, n_spans, samples, dt, modes=1, center_frequency=0.0,
+              pre_scale=1.0, post_scale=1.0, noise_sigma=0.0, noise_seed=0):
+    return Link(fiber, grid, direction, n_spans, modes, samples, 0, dt, center_frequency,
+                pre_scale, post_scale, noise_sigma, noise_seed)
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = _P()
+        _check(_lib.focss_device_alloc(ctx.handle, nbytes, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(_lib.focss_memcpy_h2d(self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, arr):
+        assert arr.flags.c_contiguous and arr.nbytes <= self.nbytes
+        _check(_lib.focss_memcpy_d2h(self.ctx.handle, arr.ctypes.data, self.ptr, arr.nbytes))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            _check(_lib.focss_device_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One device + one stream + cached graphs (focss_ctx)."""
+
+    def __init__(self, device=0):
+        h = _P()
+        _check(_lib.focss_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _lib.focss_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self):
+        _check(_lib.focss_synchronize(self.handle))
+
+    def set_tile(self, precision, log2_samples, variant):
+        _check(_lib.focss_set_tile(self.handle, precision, log2_samples, variant))
+
+    def link_run_host(self, link, field, precision=F64):
+        """Run one link program on a host complex128 array (copied in/out)."""
+        y = np.ascontiguousarray(field, dtype=np.complex128).copy()
+        steps = C.c_int(0)
+        _check(_lib.focss_link_run_host(self.handle, precision, C.byref(link), y.ctypes.data,
+                                        C.byref(steps)))
+        return y, steps.value
+
+    def link_run(self, links, batch, offsets, precision=F64):
+        """Run link programs on waveforms resident in a device buffer."""
+        n = len(links)
+        arr = (Link * n)(*links)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        steps = np.zeros(n, dtype=np.int32)
+        _check(_lib.focss_link_run(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
+                                   steps.ctypes.data))
+        return steps
+
+    def fft(self, buf, modes, samples, sign, precision=F64):
+        _check(_lib.focss_fft(self.handle, precision, sign, modes, samples, buf.ptr))
+
+
+def tile_variants(precision, log2_samples):
+    return _lib.focss_tile_variants(precision, log2_samples)

@@ -1,0 +1,138 @@
+// Stockham auto-sort FFT held in LDS (replaces the FFTW plans of
+// src/focss/field.cpp:530-541).  A tile holds S independent sequences of
+// length L interleaved as buf[i * S + s], so consecutive lanes always touch
+// consecutive LDS words in every stage.  Each stage is the radix-R step of
+// the Stockham formulation in Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors" (SC'08): read R values strided
+// by L/R, twiddle by W_{Ns R}^{r k}, radix-R DFT, write to expand(j) + r Ns.
+// Radix 8 stages first, then one radix-4 or radix-2 stage for the remainder.
+#pragma once
+#include "cx.h"
+
+namespace focss_hip {
+
+// Twiddle banks: tw[off(p) + m] = exp(-2 pi i m / 2^p) for m < 2^p, p = 1..TW_MAX_LOG.
+constexpr int TW_MAX_LOG = 14;
+__host__ __device__ constexpr int tw_offset(int log_len) { return (1 << log_len) - 2; }
+constexpr int TW_BANK_SIZE = (1 << (TW_MAX_LOG + 1)) - 2;
+
+template <int L>
+struct ilog2 {
+    static constexpr int value = 1 + ilog2<L / 2>::value;
+};
+template <>
+struct ilog2<1> {
+    static constexpr int value = 0;
+};
+
+template <typename R, int SIGN>
+__device__ __forceinline__ cx<R> twiddle(const cx<R>* __restrict__ tw, int m) {
+    cx<R> w = tw[m];
+    return SIGN < 0 ? w : conj(w);
+}
+
+// radix-2/4/8 DFT in registers, exponent sign SIGN (-1 forward, +1 inverse)
+template <typename R, int RADIX, int SIGN>
+struct butterfly;
+
+template <typename R, int SIGN>
+struct butterfly<R, 2, SIGN> {
+    static __device__ __forceinline__ void run(cx<R>* v) {
+        cx<R> a = v[0] + v[1];
+        cx<R> b = v[0] - v[1];
+        v[0] = a;
+        v[1] = b;
+    }
+};
+
+template <typename R, int SIGN>
+struct butterfly<R, 4, SIGN> {
+    static __device__ __forceinline__ void run(cx<R>* v) {
+        cx<R> a0 = v[0] + v[2];
+        cx<R> a1 = v[0] - v[2];
+        cx<R> b0 = v[1] + v[3];
+        cx<R> b1 = mul_si<R, SIGN>(v[1] - v[3]);
+        v[0] = a0 + b0;
+        v[2] = a0 - b0;
+        v[1] = a1 + b1;
+        v[3] = a1 - b1;
+    }
+};
+
+template <typename R, int SIGN>
+struct butterfly<R, 8, SIGN> {
+    static __device__ __forceinline__ void run(cx<R>* v) {
+        const R h = (R)0.70710678118654752440084436210484904;
+        cx<R> a[4], b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            a[k] = v[k] + v[k + 4];
+            b[k] = v[k] - v[k + 4];
+        }
+        // b_k *= W8^k, W8 = exp(SIGN 2 pi i / 8)
+        b[1] = mk<R>((b[1].x - SIGN * b[1].y) * h, (b[1].y + SIGN * b[1].x) * h);
+        b[2] = mul_si<R, SIGN>(b[2]);
+        b[3] = mk<R>((-b[3].x - SIGN * b[3].y) * h, (-b[3].y + SIGN * b[3].x) * h);
+        butterfly<R, 4, SIGN>::run(a);
+        butterfly<R, 4, SIGN>::run(b);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[2 * k] = a[k];
+            v[2 * k + 1] = b[k];
+        }
+    }
+};
+
+// One Stockham stage over S interleaved sequences of length L with NT
+// threads, then the remaining stages.  tw points at the length-L bank.
+template <typename R, int L, int S, int NT, int SIGN, int NS>
+__device__ __forceinline__ void fft_stages(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw) {
+    constexpr int RADIX = (L / NS >= 8) ? 8 : (L / NS);
+    constexpr int NB = L / RADIX;
+    constexpr int TOTAL = NB * S;
+    constexpr int PER = (TOTAL + NT - 1) / NT;
+    constexpr int TWS = L / (NS * RADIX);
+    const int tid = threadIdx.x;
+    cx<R> v[PER][RADIX];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = tid + b * NT;
+        if (TOTAL % NT == 0 || id < TOTAL) {
+            const int s = id % S;
+            const int j = id / S;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) v[b][r] = buf[(j + r * NB) * S + s];
+            if (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < RADIX; ++r) v[b][r] = v[b][r] * twiddle<R, SIGN>(tw, r * k * TWS);
+            }
+            butterfly<R, RADIX, SIGN>::run(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = tid + b * NT;
+        if (TOTAL % NT == 0 || id < TOTAL) {
+            const int s = id % S;
+            const int j = id / S;
+            const int k = j % NS;
+            const int base = (j / NS) * NS * RADIX + k;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) buf[(base + r * NS) * S + s] = v[b][r];
+        }
+    }
+    __syncthreads();
+    if constexpr (NS * RADIX < L) fft_stages<R, L, S, NT, SIGN, NS * RADIX>(buf, tw);
+}
+
+// Unscaled DFT of S interleaved length-L sequences in LDS (natural order in
+// and out).  Caller must __syncthreads() before (data written) -- the stage
+// loop ends with a barrier.
+template <typename R, int L, int S, int NT, int SIGN>
+__device__ __forceinline__ void fft_lds(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw_bank) {
+    if constexpr (L > 1) fft_stages<R, L, S, NT, SIGN, 1>(buf, tw_bank + tw_offset(ilog2<L>::value));
+}
+
+}  // namespace focss_hip

@@ -1,0 +1,604 @@
+// C-ABI of the focss MI355X library (declarations in include/focss.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/focss.h"
+#include "ssfm_kernels.h"
+
+using namespace focss_hip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(FOCSS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------
+// kernel registry
+// ------------------------------------------------------------------
+template <typename R>
+using KernelFn = void (*)(PassArgs, Tables<R>);
+
+constexpr int kTileVariants = 3;
+constexpr int kTileElems[kTileVariants] = {4096, 2048, 1024};
+
+template <typename R>
+struct Registry {
+    // resident[log_n][modes-1]
+    KernelFn<R> resident[21][2] = {};
+    // time[log_n][modes-1][variant], spec[log_n][variant]
+    KernelFn<R> time_k[21][2][kTileVariants] = {};
+    KernelFn<R> spec_k[21][kTileVariants] = {};
+    int time_cols[21][2][kTileVariants] = {};
+    int spec_rows[21][kTileVariants] = {};
+};
+
+template <typename R, int N, int M>
+void reg_resident(Registry<R>& r) {
+    r.resident[ilog2<N>::value][M - 1] = &k_resident<R, N, M>;
+}
+
+template <typename R, int LOGN, int M, int V>
+void reg_time(Registry<R>& r) {
+    constexpr int N1 = 1 << (LOGN / 2);
+    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int C0 = kTileElems[V] / (N1 * M);
+    constexpr int C = C0 < 1 ? 0 : (C0 > N2 ? N2 : C0);
+    if constexpr (C >= 1) {
+        r.time_k[LOGN][M - 1][V] = &k_time_pass<R, N1, C, M>;
+        r.time_cols[LOGN][M - 1][V] = C;
+    }
+}
+
+template <typename R, int LOGN, int V>
+void reg_spec(Registry<R>& r) {
+    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int S0 = kTileElems[V] / N2;
+    constexpr int S = S0 < 1 ? 0 : S0;
+    if constexpr (S >= 1) {
+        r.spec_k[LOGN][V] = &k_spec_pass<R, N2, S>;
+        r.spec_rows[LOGN][V] = S;
+    }
+}
+
+template <typename R, int LOGN>
+void reg_two_pass(Registry<R>& r) {
+    reg_time<R, LOGN, 1, 0>(r);
+    reg_time<R, LOGN, 1, 1>(r);
+    reg_time<R, LOGN, 1, 2>(r);
+    reg_time<R, LOGN, 2, 0>(r);
+    reg_time<R, LOGN, 2, 1>(r);
+    reg_time<R, LOGN, 2, 2>(r);
+    reg_spec<R, LOGN, 0>(r);
+    reg_spec<R, LOGN, 1>(r);
+    reg_spec<R, LOGN, 2>(r);
+}
+
+template <typename R>
+Registry<R> make_registry() {
+    Registry<R> r;
+    reg_resident<R, 64, 1>(r);
+    reg_resident<R, 128, 1>(r);
+    reg_resident<R, 256, 1>(r);
+    reg_resident<R, 512, 1>(r);
+    reg_resident<R, 1024, 1>(r);
+    reg_resident<R, 2048, 1>(r);
+    reg_resident<R, 4096, 1>(r);
+    reg_resident<R, 64, 2>(r);
+    reg_resident<R, 128, 2>(r);
+    reg_resident<R, 256, 2>(r);
+    reg_resident<R, 512, 2>(r);
+    reg_resident<R, 1024, 2>(r);
+    reg_resident<R, 2048, 2>(r);
+    reg_resident<R, 8192, 1>(r);
+    reg_resident<R, 4096, 2>(r);
+    reg_two_pass<R, 12>(r);
+    reg_two_pass<R, 13>(r);
+    reg_two_pass<R, 14>(r);
+    reg_two_pass<R, 15>(r);
+    reg_two_pass<R, 16>(r);
+    reg_two_pass<R, 17>(r);
+    reg_two_pass<R, 18>(r);
+    reg_two_pass<R, 19>(r);
+    reg_two_pass<R, 20>(r);
+    return r;
+}
+
+const Registry<double>& reg64() {
+    static Registry<double> r = make_registry<double>();
+    return r;
+}
+const Registry<float>& reg32() {
+    static Registry<float> r = make_registry<float>();
+    return r;
+}
+
+template <typename R>
+const Registry<R>& reg();
+template <>
+const Registry<double>& reg<double>() {
+    return reg64();
+}
+template <>
+const Registry<float>& reg<float>() {
+    return reg32();
+}
+
+int ilog2_int(int n) {
+    int p = 0;
+    while ((1 << p) < n) ++p;
+    return ((1 << p) == n) ? p : -1;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<unsigned char> key;
+};
+
+struct focss_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void* bank[2] = {nullptr, nullptr};  // [F64, F32]
+    void* fine[2] = {nullptr, nullptr};
+    // per-call device scratch (grown on demand)
+    WaveStatic* d_statics = nullptr;
+    WaveState* d_states = nullptr;
+    Reduction* d_red = nullptr;
+    int* d_waves = nullptr;
+    int* d_done = nullptr;
+    int* h_done = nullptr;  // pinned
+    size_t cap_waves = 0;
+    int tile_variant[2][21] = {};
+    GraphCache graph;
+    int graph_pairs = 24;  // multiple of 6 (state parity 2, reduction slots 3)
+    long long max_split_steps = 20000000;
+};
+
+namespace {
+
+template <typename R>
+void host_tables(std::vector<cx<R>>& bank, std::vector<cx<R>>& fine) {
+    bank.assign(TW_BANK_SIZE, mk<R>(0, 0));
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int p = 1; p <= TW_MAX_LOG; ++p) {
+        int n = 1 << p;
+        for (int m = 0; m < n; ++m) {
+            long double a = two_pi * (long double)m / (long double)n;
+            bank[tw_offset(p) + m] = mk<R>((R)cosl(a), (R)(-sinl(a)));
+        }
+    }
+    fine.assign(1024, mk<R>(0, 0));
+    for (int m = 0; m < 1024; ++m) {
+        long double a = two_pi * (long double)m / (long double)(1 << 20);
+        fine[m] = mk<R>((R)cosl(a), (R)(-sinl(a)));
+    }
+}
+
+int ensure_capacity(focss_ctx* c, size_t n_waves) {
+    if (n_waves <= c->cap_waves) return 0;
+    if (c->d_statics) {
+        (void)hipFree(c->d_statics);
+        (void)hipFree(c->d_states);
+        (void)hipFree(c->d_red);
+        (void)hipFree(c->d_waves);
+    }
+    size_t cap = n_waves < 64 ? 64 : n_waves;
+    HIP_TRY(hipMalloc(&c->d_statics, cap * sizeof(WaveStatic)));
+    HIP_TRY(hipMalloc(&c->d_states, 2 * cap * sizeof(WaveState)));
+    HIP_TRY(hipMalloc(&c->d_red, 3 * cap * sizeof(Reduction)));
+    HIP_TRY(hipMalloc(&c->d_waves, cap * sizeof(int)));
+    c->cap_waves = cap;
+    return 0;
+}
+
+void drop_graph(focss_ctx* c) {
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
+    c->graph.exec = nullptr;
+    c->graph.graph = nullptr;
+    c->graph.key.clear();
+}
+
+bool resident_ok(int precision, int log_n, int modes) {
+    if (precision == FOCSS_F64) return reg64().resident[log_n][modes - 1] != nullptr;
+    return reg32().resident[log_n][modes - 1] != nullptr;
+}
+
+bool two_pass_ok(int precision, int log_n, int modes, int variant) {
+    if (log_n < 0 || log_n > 20) return false;
+    if (precision == FOCSS_F64)
+        return reg64().time_k[log_n][modes - 1][variant] && reg64().spec_k[log_n][variant];
+    return reg32().time_k[log_n][modes - 1][variant] && reg32().spec_k[log_n][variant];
+}
+
+struct LengthClass {
+    int log_n, modes, resident, variant;
+    int first;  // index into the waves array
+    int count;
+};
+
+template <typename R>
+void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
+    const Registry<R>& r = reg<R>();
+    int cols = r.time_cols[lc.log_n][lc.modes - 1][lc.variant];
+    int n2 = 1 << (lc.log_n - lc.log_n / 2);
+    a.tiles_per_wave = n2 / cols;
+    a.waves += lc.first;
+    a.log_n = lc.log_n;
+    hipLaunchKernelGGL(r.time_k[lc.log_n][lc.modes - 1][lc.variant], dim3(lc.count * a.tiles_per_wave),
+                       dim3(NT), 0, c->stream, a, tb);
+}
+
+template <typename R>
+void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
+    const Registry<R>& r = reg<R>();
+    int rows = r.spec_rows[lc.log_n][lc.variant];
+    int n1 = 1 << (lc.log_n / 2);
+    a.tiles_per_wave = lc.modes * n1 / rows;
+    a.waves += lc.first;
+    a.log_n = lc.log_n;
+    hipLaunchKernelGGL(r.spec_k[lc.log_n][lc.variant], dim3(lc.count * a.tiles_per_wave), dim3(NT), 0,
+                       c->stream, a, tb);
+}
+
+template <typename R>
+void launch_resident(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
+    const Registry<R>& r = reg<R>();
+    a.waves += lc.first;
+    a.log_n = lc.log_n;
+    a.tiles_per_wave = 1;
+    hipLaunchKernelGGL(r.resident[lc.log_n][lc.modes - 1], dim3(lc.count), dim3(NT), 0, c->stream, a, tb);
+}
+
+int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
+    if (L.modes < 1 || L.modes > 2) return fail(FOCSS_E_ARG, "modes must be 1 or 2");
+    int log_n = ilog2_int(L.samples);
+    if (log_n < 6 || log_n > 20) return fail(FOCSS_E_ARG, "samples must be a power of two in [2^6, 2^20]");
+    if (!(L.dt > 0)) return fail(FOCSS_E_ARG, "dt must be > 0");
+    if (L.n_spans < 1) return fail(FOCSS_E_ARG, "n_spans must be >= 1");
+    if (!(L.fiber.length > 0)) return fail(FOCSS_E_ARG, "fibre length must be > 0");
+    if (L.direction != 1.0 && L.direction != -1.0) return fail(FOCSS_E_ARG, "direction must be +1 or -1");
+    const int g = L.grid.grid;
+    if (g == FOCSS_UNIFORM) {
+        if (L.grid.total_steps < 1) return fail(FOCSS_E_ARG, "UNIFORM grid needs total_steps >= 1");
+    } else if (g == FOCSS_LOGARITHMIC) {
+        if (L.fiber.alpha == 0) return fail(FOCSS_E_ARG, "LOGARITHMIC grid needs alpha != 0 (ssfm.cpp:76)");
+        if (!(L.grid.max_step_size > 0)) return fail(FOCSS_E_ARG, "LOGARITHMIC grid needs max_step_size > 0");
+        if (!(L.grid.max_phase_shift > 0)) return fail(FOCSS_E_ARG, "LOGARITHMIC grid needs max_phase_shift > 0");
+        if (L.fiber.gamma == 0 || L.fiber.kappa == 0)
+            return fail(FOCSS_E_ARG, "LOGARITHMIC grid needs gamma, kappa != 0");
+    } else if (g == FOCSS_ADAPTIVE) {
+        if (!(L.grid.max_step_size > 0))
+            return fail(FOCSS_E_ARG, "ADAPTIVE grid needs max_step_size > 0 (h = 0 never terminates, ssfm.cpp:99)");
+        if (!(L.grid.max_phase_shift > 0)) return fail(FOCSS_E_ARG, "ADAPTIVE grid needs max_phase_shift > 0");
+    } else {
+        return fail(FOCSS_E_ARG, "unknown grid");
+    }
+    w.alpha = L.fiber.alpha;
+    w.beta2 = L.fiber.beta2;
+    w.gamma = L.fiber.gamma;
+    w.kappa = L.fiber.kappa;
+    w.length = L.fiber.length;
+    w.kgamma = L.fiber.kappa * L.fiber.gamma;
+    w.abs_kgamma = std::fabs(L.fiber.kappa * L.fiber.gamma);
+    w.dir = L.direction;
+    w.phi_max = L.grid.max_phase_shift;
+    w.h_max = L.grid.max_step_size;
+    w.pre_scale = L.span_pre_scale;
+    w.post_scale = L.span_post_scale;
+    w.noise_sigma = L.noise_sigma;
+    w.dw = 2.0 * 3.14159265358979323846264338 / (L.dt * L.samples);
+    w.seed = L.noise_seed;
+    w.offset = offset;
+    w.grid = g;
+    w.steps = L.grid.total_steps;
+    w.n_spans = L.n_spans;
+    w.modes = L.modes;
+    w.samples = L.samples;
+    w.linear_only = (g == FOCSS_UNIFORM && w.kgamma == 0) ? 1 : 0;
+    return 0;
+}
+
+template <typename R>
+int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void* batch,
+              const int64_t* offsets, int* total_steps) {
+    std::vector<WaveStatic> st(n);
+    for (int i = 0; i < n; ++i) {
+        int rc = fill_static(links[i], offsets[i], st[i]);
+        if (rc) return rc;
+    }
+    if (int rc = ensure_capacity(c, n)) return rc;
+    // group by length class
+    std::map<std::pair<int, int>, std::vector<int>> groups;
+    for (int i = 0; i < n; ++i) groups[{ilog2_int(links[i].samples), links[i].modes}].push_back(i);
+    std::vector<int> order;
+    std::vector<LengthClass> resident, two_pass;
+    for (auto& kv : groups) {
+        LengthClass lc;
+        lc.log_n = kv.first.first;
+        lc.modes = kv.first.second;
+        lc.variant = c->tile_variant[precision][lc.log_n];
+        lc.first = (int)order.size();
+        lc.count = (int)kv.second.size();
+        for (int i : kv.second) order.push_back(i);
+        if (resident_ok(precision, lc.log_n, lc.modes)) {
+            lc.resident = 1;
+            resident.push_back(lc);
+        } else if (two_pass_ok(precision, lc.log_n, lc.modes, lc.variant)) {
+            lc.resident = 0;
+            two_pass.push_back(lc);
+        } else {
+            return fail(FOCSS_E_UNSUPPORTED, "no kernel for samples=2^" + std::to_string(lc.log_n) +
+                                                 " modes=" + std::to_string(lc.modes));
+        }
+    }
+    std::vector<WaveState> init(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) init[i] = initial_state(st[i]);
+    std::vector<Reduction> red0(3 * (size_t)n);
+    memset(red0.data(), 0, red0.size() * sizeof(Reduction));
+    Tables<R> tb;
+    tb.bank = (const cx<R>*)c->bank[precision];
+    tb.fine = (const cx<R>*)c->fine[precision];
+    PassArgs a;
+    a.field = batch;
+    a.statics = c->d_statics;
+    a.states = c->d_states;
+    a.red = c->d_red;
+    a.done_count = c->d_done;
+    a.waves = c->d_waves;
+    a.n_waves = n;
+    a.counter = 0;
+    if (!two_pass.empty()) {
+        // graph of [spectral pass r, time pass r+1] x K, captured before any
+        // work is queued; its key is everything baked into the kernel args
+        std::vector<unsigned char> key;
+        auto put = [&key](const void* p, size_t s) {
+            const unsigned char* b = (const unsigned char*)p;
+            key.insert(key.end(), b, b + s);
+        };
+        put(&precision, sizeof(int));
+        put(&n, sizeof(int));
+        put(&batch, sizeof(void*));
+        put(&c->d_statics, sizeof(void*));
+        put(&c->d_states, sizeof(void*));
+        put(&c->d_red, sizeof(void*));
+        put(&c->d_waves, sizeof(void*));
+        for (const LengthClass& lc : two_pass) put(&lc, sizeof(lc));
+        if (c->graph.key != key || !c->graph.exec) {
+            drop_graph(c);
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < c->graph_pairs; ++k) {
+                PassArgs ak = a;
+                ak.counter = k;
+                for (const LengthClass& lc : two_pass) launch_spec<R>(c, lc, ak, tb);
+                ak.counter = k + 1;
+                for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
+            }
+            HIP_TRY(hipStreamEndCapture(c->stream, &c->graph.graph));
+            HIP_TRY(hipGraphInstantiate(&c->graph.exec, c->graph.graph, nullptr, nullptr, 0));
+            c->graph.key = key;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_statics, st.data(), n * sizeof(WaveStatic), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_states, init.data(), 2 * n * sizeof(WaveState), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_red, red0.data(), 3 * n * sizeof(Reduction), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_waves, order.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    *c->h_done = 0;
+    HIP_TRY(hipMemcpyAsync(c->d_done, c->h_done, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    for (const LengthClass& lc : resident) launch_resident<R>(c, lc, a, tb);
+    HIP_TRY(hipGetLastError());
+    if (!two_pass.empty()) {
+        // entry time pass (counter 0), then replays of the graph
+        for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
+        HIP_TRY(hipGetLastError());
+        const int target = n;  // resident waves also count themselves done
+        for (long long rep = 0;; ++rep) {
+            HIP_TRY(hipGraphLaunch(c->graph.exec, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->h_done, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (*c->h_done >= target) break;
+            if (rep > c->max_split_steps / c->graph_pairs)
+                return fail(FOCSS_E_NUMERIC, "link program did not terminate within the split-step limit");
+        }
+    }
+    if (total_steps) {
+        std::vector<WaveState> fin(2 * (size_t)n);
+        HIP_TRY(hipMemcpyAsync(fin.data(), c->d_states, 2 * n * sizeof(WaveState), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < n; ++i) {
+            // the larger count of the two buffers is the final one
+            int a0 = fin[i].total_steps, a1 = fin[n + i].total_steps;
+            total_steps[i] = a0 > a1 ? a0 : a1;
+        }
+    } else {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* focss_last_error(void) { return g_err.c_str(); }
+
+const char* focss_build_info(void) {
+    return "focss-mi355x: gfx950 HIP kernels; resident LDS SSFM (N*modes <= 8192 c128 / 16384 c64), "
+           "two-pass four-step SSFM (2^12..2^20); FP64 and FP32 (FP64 phase generation)";
+}
+
+int focss_ctx_create(int device, focss_ctx** out) {
+    if (!out) return fail(FOCSS_E_ARG, "null out");
+    focss_ctx* c = new focss_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(FOCSS_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    std::vector<cx<double>> b64, f64;
+    std::vector<cx<float>> b32, f32;
+    host_tables<double>(b64, f64);
+    host_tables<float>(b32, f32);
+    HIP_TRY(hipMalloc(&c->bank[0], b64.size() * sizeof(cx<double>)));
+    HIP_TRY(hipMalloc(&c->fine[0], f64.size() * sizeof(cx<double>)));
+    HIP_TRY(hipMalloc(&c->bank[1], b32.size() * sizeof(cx<float>)));
+    HIP_TRY(hipMalloc(&c->fine[1], f32.size() * sizeof(cx<float>)));
+    HIP_TRY(hipMemcpy(c->bank[0], b64.data(), b64.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fine[0], f64.data(), f64.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->bank[1], b32.data(), b32.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fine[1], f32.data(), f32.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_done, sizeof(int)));
+    HIP_TRY(hipHostMalloc(&c->h_done, sizeof(int), hipHostMallocDefault));
+    *out = c;
+    return 0;
+}
+
+int focss_ctx_destroy(focss_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (int i = 0; i < 2; ++i) {
+        if (c->bank[i]) (void)hipFree(c->bank[i]);
+        if (c->fine[i]) (void)hipFree(c->fine[i]);
+    }
+    if (c->d_statics) {
+        (void)hipFree(c->d_statics);
+        (void)hipFree(c->d_states);
+        (void)hipFree(c->d_red);
+        (void)hipFree(c->d_waves);
+    }
+    if (c->d_done) (void)hipFree(c->d_done);
+    if (c->h_done) (void)hipHostFree(c->h_done);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int focss_device_alloc(focss_ctx* c, size_t bytes, void** dptr) {
+    if (!c || !dptr) return fail(FOCSS_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMalloc(dptr, bytes));
+    return 0;
+}
+
+int focss_device_free(focss_ctx* c, void* dptr) {
+    if (!c) return fail(FOCSS_E_ARG, "null ctx");
+    HIP_TRY(hipFree(dptr));
+    return 0;
+}
+
+int focss_memcpy_h2d(focss_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return fail(FOCSS_E_ARG, "null ctx");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int focss_memcpy_d2h(focss_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return fail(FOCSS_E_ARG, "null ctx");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int focss_synchronize(focss_ctx* c) {
+    if (!c) return fail(FOCSS_E_ARG, "null ctx");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int focss_link_run(focss_ctx* c, int precision, int n_waves, const focss_link* links, void* batch,
+                   const int64_t* offsets, int* total_steps) {
+    if (!c || !links || !batch || !offsets || n_waves < 1) return fail(FOCSS_E_ARG, "null argument");
+    if (precision != FOCSS_F64 && precision != FOCSS_F32) return fail(FOCSS_E_ARG, "bad precision");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == FOCSS_F64) return run_links<double>(c, precision, n_waves, links, batch, offsets, total_steps);
+    return run_links<float>(c, precision, n_waves, links, batch, offsets, total_steps);
+}
+
+int focss_link_run_host(focss_ctx* c, int precision, const focss_link* link, double* field, int* total_steps) {
+    if (!c || !link || !field) return fail(FOCSS_E_ARG, "null argument");
+    size_t n = (size_t)link->modes * (size_t)link->samples;
+    size_t esz = precision == FOCSS_F64 ? 16 : 8;
+    void* d = nullptr;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMalloc(&d, n * esz));
+    std::vector<float> tmp;
+    if (precision == FOCSS_F64) {
+        HIP_TRY(hipMemcpyAsync(d, field, n * esz, hipMemcpyHostToDevice, c->stream));
+    } else {
+        tmp.resize(2 * n);
+        for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)field[i];
+        HIP_TRY(hipMemcpyAsync(d, tmp.data(), n * esz, hipMemcpyHostToDevice, c->stream));
+    }
+    int64_t off = 0;
+    int rc = focss_link_run(c, precision, 1, link, d, &off, total_steps);
+    if (rc == 0) {
+        if (precision == FOCSS_F64) {
+            HIP_TRY(hipMemcpyAsync(field, d, n * esz, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        } else {
+            HIP_TRY(hipMemcpyAsync(tmp.data(), d, n * esz, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            for (size_t i = 0; i < 2 * n; ++i) field[i] = tmp[i];
+        }
+    }
+    drop_graph(c);  // the graph captured d, which is freed now
+    (void)hipFree(d);
+    return rc;
+}
+
+int focss_fft(focss_ctx* c, int precision, int sign, int modes, int samples, void* data) {
+    (void)c;
+    (void)precision;
+    (void)sign;
+    (void)modes;
+    (void)samples;
+    (void)data;
+    return fail(FOCSS_E_UNSUPPORTED, "focss_fft: not built yet");
+}
+
+int focss_set_tile(focss_ctx* c, int precision, int log2_samples, int variant) {
+    if (!c || precision < 0 || precision > 1 || log2_samples < 0 || log2_samples > 20 || variant < 0 ||
+        variant >= kTileVariants)
+        return fail(FOCSS_E_ARG, "bad tile selection");
+    c->tile_variant[precision][log2_samples] = variant;
+    drop_graph(c);
+    return 0;
+}
+
+int focss_tile_variants(int precision, int log2_samples) {
+    int n = 0;
+    for (int v = 0; v < kTileVariants; ++v)
+        if (two_pass_ok(precision, log2_samples, 1, v)) ++n;
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+/*
+ * focss C-ABI -- the drop-in boundary between the focss C++ host API
+ * (Field / Fiber / Amplifier / SSFM / link drivers) and the MI355X (gfx950)
+ * kernels.  Plain C types only: pointers, sizes, PODs.  Every entry point
+ * returns 0 on success or a negative FOCSS_E_* code; focss_last_error()
+ * returns the message of the calling thread's last failure.
+ *
+ * Field layout everywhere: mode-major interleaved complex, element
+ * (mode, sample) at data[sample + mode * samples]   (src/focss/field.cpp:140-155).
+ * precision FOCSS_F64 = complex<double> (16 B), FOCSS_F32 = complex<float> (8 B).
+ * samples must be a power of two, 2^6 .. 2^20.
+ *
+ * A context owns one HIP stream on one device plus cached device buffers and
+ * captured graphs.  A context is used by one host thread at a time; distinct
+ * contexts may run concurrently.
+ */
+#ifndef FOCSS_H_
+#define FOCSS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOCSS_OK 0
+#define FOCSS_E_ARG -1      /* invalid argument (sizes, grid, null pointer) */
+#define FOCSS_E_HIP -2      /* HIP runtime error */
+#define FOCSS_E_UNSUPPORTED -3
+#define FOCSS_E_NUMERIC -4  /* non-finite result detected */
+
+#define FOCSS_F64 0
+#define FOCSS_F32 1
+
+#define FOCSS_UNIFORM 0     /* SSFM::UNIFORM     src/focss/solver/ssfm.cpp:9 */
+#define FOCSS_LOGARITHMIC 1 /* SSFM::LOGARITHMIC src/focss/solver/ssfm.cpp:10 */
+#define FOCSS_ADAPTIVE 2    /* SSFM::ADAPTIVE    src/focss/solver/ssfm.cpp:11 */
+
+typedef struct focss_ctx focss_ctx;
+
+/* focss::Fiber POD, SI units (src/focss/module/fiber.h:5-21) */
+typedef struct {
+    double alpha;      /* power attenuation [1/m] */
+    double beta2;      /* [s^2/m] */
+    double gamma;      /* [1/(W m)] */
+    double length;     /* [m] */
+    double wavelength; /* [m] */
+    double kappa;      /* polarisation coupling */
+} focss_fiber;
+
+/* Grid parameters of focss::SSFM (src/focss/solver/ssfm.h:25-29) */
+typedef struct {
+    int grid;                /* FOCSS_UNIFORM / FOCSS_LOGARITHMIC / FOCSS_ADAPTIVE */
+    int total_steps;         /* UNIFORM steps per span (set_total_steps) */
+    double max_phase_shift;  /* set_maximum_phase_shift */
+    double max_step_size;    /* set_maximum_step_size [m] */
+} focss_grid;
+
+/* One waveform's link program:
+ *   for span in 0..n_spans-1:
+ *       field *= span_pre_scale                (Amplifier::drop_power, amplifier.cpp:37-42)
+ *       SSFM(fiber, grid).run(field, direction) (ssfm.cpp:39-60)
+ *       field *= span_post_scale               (Amplifier::amplify gain, amplifier.cpp:15-18)
+ *       if noise_sigma: field += noise_sigma * (z0 + i z1), z ~ N(0,1) from
+ *                       Philox4x32-10(seed, stream = span, index = flat element)
+ * This is forward_propagation / backward_propagation / cd_compensation
+ * (src/tasks/systems.cpp:26-91) and SSFM::run itself (n_spans = 1). */
+typedef struct {
+    focss_fiber fiber;
+    focss_grid grid;
+    double direction;        /* +1 FORWARD, -1 BACKWARD (ssfm.cpp:13-14) */
+    int n_spans;
+    int modes;               /* 1 or 2 (Manakov) */
+    int samples;             /* power of two */
+    int pad;
+    double dt;               /* sample spacing [s] (Domain::time_step) */
+    double center_frequency; /* [Hz] (Domain::center_frequency) */
+    double span_pre_scale;
+    double span_post_scale;
+    double noise_sigma;      /* per-quadrature sigma, 0 = no ASE */
+    uint64_t noise_seed;
+} focss_link;
+
+/* ---- context / memory ---- */
+int focss_ctx_create(int device, focss_ctx** out);
+int focss_ctx_destroy(focss_ctx* ctx);
+const char* focss_last_error(void);
+int focss_device_alloc(focss_ctx* ctx, size_t bytes, void** dptr);
+int focss_device_free(focss_ctx* ctx, void* dptr);
+int focss_memcpy_h2d(focss_ctx* ctx, void* dst, const void* src, size_t bytes);
+int focss_memcpy_d2h(focss_ctx* ctx, void* dst, const void* src, size_t bytes);
+int focss_synchronize(focss_ctx* ctx);
+
+/* ---- the hot path ----
+ * Runs the link programs of n_waves waveforms resident in device memory.
+ * Waveform i occupies modes*samples complex elements at
+ * (complex*)batch + offsets[i] (element offsets).  Waveforms may differ in
+ * samples, modes, fibre, grid and span count; all run in one stream, the
+ * small ones LDS-resident in one launch, the large ones as a hipGraph of
+ * alternating time/spectral passes replayed until every waveform is done
+ * (device-side step control, no host sync per step).
+ * total_steps (optional, n_waves ints) receives the split steps performed.
+ * Replaces SSFM::run (src/focss/solver/ssfm.cpp:39-60) and the span loops of
+ * forward_propagation / backward_propagation / cd_compensation
+ * (src/tasks/systems.cpp:26-91, src/tasks/wdm_run.cpp:26-47). */
+int focss_link_run(focss_ctx* ctx, int precision, int n_waves, const focss_link* links,
+                   void* batch, const int64_t* offsets, int* total_steps);
+
+/* Host-buffer convenience for one waveform: complex<double> in/out, computed
+ * at `precision` on the device.  Used by focss::SSFM::run. */
+int focss_link_run_host(focss_ctx* ctx, int precision, const focss_link* link,
+                        double* field_c128, int* total_steps);
+
+/* Batched in-place DFT of `modes` rows of `samples` (device memory).
+ * sign -1: forward e^{-2 pi i jk/N} then x 1/N (Field::fft_inplace,
+ * src/focss/field.cpp:343-347); sign +1: inverse, unscaled
+ * (Field::ifft_inplace, src/focss/field.cpp:349-352). */
+int focss_fft(focss_ctx* ctx, int precision, int sign, int modes, int samples, void* data);
+
+/* Tuning: choose the tile shape of the two-pass kernels for one length
+ * (0 = default).  Exposed for the autotuner. */
+int focss_set_tile(focss_ctx* ctx, int precision, int log2_samples, int variant);
+int focss_tile_variants(int precision, int log2_samples);
+
+/* Version / capability string of the build. */
+const char* focss_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FOCSS_H_ */
