@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark: forward propagation of a 5-channel 32 GBd 16QAM WDM field
+(2^18 samples at 8 samples/symbol, 50 GHz grid, 0 dBm per channel) over
+20 x 80 km SSMF with the ADAPTIVE grid (max nonlinear phase 1e-3 rad,
+max step 1 km), EDFA gain 16 dB and seeded ASE at NF 5 dB, on one MI355X.
+
+One bench step = the whole 20-span link for the batch (default 16 independent
+realisations).
+Metric: GSample-steps/s = sum over waveforms of samples x split steps / s.
+Prints one JSON line.  FP64 (complex128) is the headline; the FP32 path
+(complex64 arithmetic, FP64 phase generation) is reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LIGHT_SPEED = 299792458.0
+PLANCK = 6.62607004081818181818181e-34
+WAVELENGTH = 1.55e-6
+F_C = LIGHT_SPEED / WAVELENGTH
+HBM_PEAK_GBS = 8000.0
+
+N_SAMPLES = 1 << 18
+N_CH, BAUD, SPACING, OSF = 5, 32e9, 50e9, 8
+SPANS, SPAN_LEN = 20, 80e3
+ALPHA_DB_KM, BETA2_PS2_KM, GAMMA = 0.2, -21.7, 1.3e-3
+PHI_MAX, H_MAX, NF_DB = 1e-3, 1e3, 5.0
+POWER_DBM = 0.0
+SEED = 0xF0C55002
+
+
+def fiber_dict():
+    return dict(alpha=ALPHA_DB_KM * 1e-3 * np.log(10) / 10, beta2=BETA2_PS2_KM * 1e-27,
+                gamma=GAMMA, length=SPAN_LEN, wavelength=WAVELENGTH, kappa=1.0)
+
+
+def make_links(F, dt, batch):
+    gain = 10 ** (ALPHA_DB_KM * SPAN_LEN * 1e-3 / 10)
+    nf = 10 ** (NF_DB / 10)
+    variance = PLANCK * F_C * nf * (gain - 1) / 2 / dt  # amplifier.cpp:21-23
+    sigma = np.sqrt(variance / 2)
+    links = []
+    for b in range(batch):
+        links.append(F.make_link(F.Fiber(**fiber_dict()), F.Grid(F.ADAPTIVE, 0, PHI_MAX, H_MAX), F.FORWARD,
+                                 SPANS, N_SAMPLES, dt, modes=1, center_frequency=F_C,
+                                 post_scale=np.sqrt(gain), noise_sigma=sigma, noise_seed=SEED + b))
+    return links
+
+
+def load_traffic(precision):
+    path = os.path.join(ROOT, "profiles", "traffic_wdm5_2p18.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(precision)
+
+
+def run_gpu(F, ctx, fields, dt, precision, steps, warmup, batch):
+    from focss_amd.autotune import tune
+    esz = 16 if precision == F.F64 else 8
+    host = np.concatenate(fields[:batch])
+    if precision == F.F32:
+        host = host.astype(np.complex64)
+    nbytes = host.nbytes
+    offsets = np.arange(batch, dtype=np.int64) * N_SAMPLES
+    links = make_links(F, dt, batch)
+    tune_buf = ctx.alloc(nbytes)
+    tuned = tune(ctx, links, tune_buf, offsets, host, precision=precision,
+                 log=lambda m: print(m, file=sys.stderr, flush=True))
+    tune_buf.free()
+    bufs = [ctx.alloc(nbytes) for _ in range(warmup + steps)]
+    for b in bufs:
+        b.upload(host)
+    ctx.synchronize()
+    total_steps = None
+    for w in range(warmup):
+        total_steps = ctx.link_run(links, bufs[w], offsets, precision=precision)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s = ctx.link_run(links, bufs[warmup + k], offsets, precision=precision)
+        total_steps = s if total_steps is None else total_steps
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    split_steps = int(np.sum(total_steps))
+    sample_steps = float(N_SAMPLES) * split_steps * steps
+    value = sample_steps / elapsed / 1e9
+    # kernel durations from HIP events on the library's stream
+    prof_buf = ctx.alloc(nbytes)
+    prof_buf.upload(host)
+    prof = ctx.link_profile(links, prof_buf, offsets, 48, precision=precision)
+    prof_buf.free()
+    out = bufs[-1].download(np.empty_like(host))
+    for b in bufs:
+        b.free()
+    t_time, t_spec = prof[0], prof[1]
+    dominant = "k_time_pass" if t_time >= t_spec else "k_spec_pass"
+    t_dom = max(t_time, t_spec)
+    bytes_per_launch = 2.0 * esz * N_SAMPLES * batch  # one read + one write of the field
+    achieved = bytes_per_launch / (t_dom * 1e-3) / 1e9
+    return dict(value=value, ms_per_step=elapsed / steps * 1e3, split_steps=split_steps, tuned=tuned,
+                kernel=dominant, kernel_ms=t_dom, time_pass_ms=t_time, spec_pass_ms=t_spec,
+                achieved_gbs=achieved, bytes_per_launch=bytes_per_launch,
+                finite=bool(np.all(np.isfinite(out))), output_power=float(np.mean(np.abs(out) ** 2)))
+
+
+def cpu_baseline(field, dt, km):
+    """The CPU FP64 oracle (a restatement of the reference path, single
+    thread) on the first `km` km of span 1 of the same field."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    fd = fiber_dict()
+    fd["length"] = km * 1e3
+    fib = O.fiber(**fd)
+    grd = O.grid(O.ADAPTIVE, 0, PHI_MAX, H_MAX)
+    t0 = time.perf_counter()
+    _, st = O.ssfm_run(fib, grd, 1.0, field, dt)
+    t = time.perf_counter() - t0
+    return dict(value=N_SAMPLES * st / t / 1e9, unit="GSample-steps/s", cores=1, kind="port",
+                sample=f"FP64 oracle (own Stockham FFT, 1 thread), first {km:g} km of span 1 "
+                       f"({st} adaptive steps, 2^18 samples), {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--precision", choices=["f64", "f32", "both"], default="both")
+    ap.add_argument("--cpu-km", type=float, default=5.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the link batch would shard "
+                                   "waveform-wise across GPUs (see DESIGN.md), --gpus > 1 is not run"}))
+        return 2
+
+    import focss_amd as F
+    from focss_amd.transmitter import wdm_signal
+
+    fields = []
+    for b in range(args.batch):
+        f, dt, _ = wdm_signal(SEED + 7919 * b, N_CH, N_SAMPLES // OSF, OSF, BAUD, SPACING, POWER_DBM)
+        fields.append(f)
+    field = fields[0]
+    res = {}
+    with F.Context(0) as ctx:
+        if args.precision in ("f64", "both"):
+            res["f64"] = run_gpu(F, ctx, fields, dt, F.F64, args.steps, args.warmup, args.batch)
+        if args.precision in ("f32", "both"):
+            res["f32"] = run_gpu(F, ctx, fields, dt, F.F32, args.steps, args.warmup, args.batch)
+    head = res.get("f64") or res.get("f32")
+    head_key = "f64" if "f64" in res else "f32"
+    traffic = load_traffic(head_key)
+    line = {
+        "metric": "GSample-steps/s SSFM forward link (adaptive, ASE), 1 GPU",
+        "value": round(head["value"], 4),
+        "unit": "GSample-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head_key,
+        "data": "synthetic (seeded PCG64 16QAM, RRC 0.1, WDM mux; Philox ASE)",
+        "config": {"workload": f"{args.batch} independent realisations (symbols + ASE seeds) of: "
+                               "5x32 GBd 16QAM WDM, 50 GHz grid, 2^18 samples at 8 sps, "
+                               "20x80 km SSMF, ADAPTIVE phi<=1e-3 rad h<=1 km, EDFA 16 dB + ASE NF 5 dB",
+                   "samples": N_SAMPLES, "batch": args.batch, "launch_dbm_per_channel": POWER_DBM,
+                   "split_steps_per_waveform": head["split_steps"] // max(args.batch, 1),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": head["kernel"],
+                     "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "bytes_per_launch": head["bytes_per_launch"],
+                     "kernel_ms": round(head["kernel_ms"], 5),
+                     "time_pass_ms": round(head["time_pass_ms"], 5),
+                     "spec_pass_ms": round(head["spec_pass_ms"], 5)},
+    }
+    if "f32" in res and head_key != "f32":
+        r = res["f32"]
+        line["f32"] = {"value": round(r["value"], 4), "ms_per_step": round(r["ms_per_step"], 3),
+                       "kernel": r["kernel"], "achieved_gbs": round(r["achieved_gbs"], 1),
+                       "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                       "time_pass_ms": round(r["time_pass_ms"], 5),
+                       "spec_pass_ms": round(r["spec_pass_ms"], 5)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(field, dt, args.cpu_km)
+    line["finite"] = all(r["finite"] for r in res.values())
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -18,7 +18,7 @@ UNIFORM, LOGARITHMIC, ADAPTIVE = 0, 1, 2
 FORWARD, BACKWARD = 1.0, -1.0
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"focss_amd: {LIB_PATH} is not built; run `python -m focss_amd.build`")
+    raise ImportError(f"focss_amd: {LIB_PATH} is not built; run `python focss_amd/build.py`")
 
 _lib = C.CDLL(LIB_PATH)
 
@@ -61,6 +61,11 @@ _sig = {
     "focss_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "focss_tile_variants": (C.c_int, [C.c_int, C.c_int]),
     "focss_build_info": (C.c_char_p, []),
+    "focss_link_profile": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, C.c_int, _P]),
+    "focss_device_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "focss_ctx_stream": (_P, [_P]),
+    "focss_link_validate": (C.c_int, [C.POINTER(Link)]),
+    "focss_add_noise": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint32, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -155,8 +160,33 @@ class Context:
                                    steps.ctypes.data))
         return steps
 
+    def link_profile(self, links, batch, offsets, n_pairs, precision=F64):
+        """Mean durations (ms) of the time pass, spectral pass and resident
+        launch over the first n_pairs split steps, from HIP events."""
+        n = len(links)
+        arr = (Link * n)(*links)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        out = np.zeros(3, dtype=np.float64)
+        _check(_lib.focss_link_profile(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
+                                       n_pairs, out.ctypes.data))
+        return out
+
+    def device_copy(self, dst, src, nbytes):
+        _check(_lib.focss_device_copy(self.handle, dst.ptr, src.ptr, nbytes))
+
+    def stream(self):
+        return _lib.focss_ctx_stream(self.handle)
+
     def fft(self, buf, modes, samples, sign, precision=F64):
         _check(_lib.focss_fft(self.handle, precision, sign, modes, samples, buf.ptr))
+
+    def add_noise(self, buf, modes, samples, sigma, seed, stream, precision=F64):
+        _check(_lib.focss_add_noise(self.handle, precision, modes, samples, sigma, seed, stream, buf.ptr))
+
+
+def validate(link):
+    """Host-side argument check; raises FocssError with the reason."""
+    _check(_lib.focss_link_validate(C.byref(link)))
 
 
 def tile_variants(precision, log2_samples):

@@ -1,6 +1,7 @@
 """Builds the in-tree HIP library focss_amd/libfocss_hip.so for gfx950 and
 the C++ host library focss_amd/libfocss.so (focss::Field / SSFM / link
-drivers over the C-ABI).  Run as `python -m focss_amd.build`."""
+drivers over the C-ABI).  Run as `python focss_amd/build.py` (by path: importing the package needs the
+library already built)."""
 import os
 import subprocess
 import sys

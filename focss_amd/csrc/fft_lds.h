@@ -1,17 +1,20 @@
 // Stockham auto-sort FFT held in LDS (replaces the FFTW plans of
 // src/focss/field.cpp:530-541).  A tile holds S independent sequences of
-// length L interleaved as buf[i * S + s], so consecutive lanes always touch
-// consecutive LDS words in every stage.  Each stage is the radix-R step of
-// the Stockham formulation in Govindaraju et al., "High performance discrete
-// Fourier transforms on graphics processors" (SC'08): read R values strided
-// by L/R, twiddle by W_{Ns R}^{r k}, radix-R DFT, write to expand(j) + r Ns.
-// Radix 8 stages first, then one radix-4 or radix-2 stage for the remainder.
+// length L interleaved as element (i, s) at lds[pad(i * S + s)], so
+// consecutive lanes touch consecutive LDS words in every stage; pad() inserts
+// one element every 8 so that strided transposing loads spread over banks.
+// Each stage is the radix-R step of the Stockham formulation in Govindaraju
+// et al., "High performance discrete Fourier transforms on graphics
+// processors" (SC'08): read R values strided by L/R, twiddle by
+// W_{Ns R}^{r k}, radix-R DFT, write to expand(j) + r Ns.  Radix-8 stages
+// first, then one radix-4 or radix-2 stage for the remainder.
 #pragma once
 #include "cx.h"
 
 namespace focss_hip {
 
-// Twiddle banks: tw[off(p) + m] = exp(-2 pi i m / 2^p) for m < 2^p, p = 1..TW_MAX_LOG.
+// Twiddle banks in global memory: tw[off(p) + m] = exp(-2 pi i m / 2^p),
+// m < 2^p, p = 1..TW_MAX_LOG.  Rounded once from long double on the host.
 constexpr int TW_MAX_LOG = 14;
 __host__ __device__ constexpr int tw_offset(int log_len) { return (1 << log_len) - 2; }
 constexpr int TW_BANK_SIZE = (1 << (TW_MAX_LOG + 1)) - 2;
@@ -25,10 +28,35 @@ struct ilog2<1> {
     static constexpr int value = 0;
 };
 
-template <typename R, int SIGN>
-__device__ __forceinline__ cx<R> twiddle(const cx<R>* __restrict__ tw, int m) {
-    cx<R> w = tw[m];
-    return SIGN < 0 ? w : conj(w);
+__host__ __device__ constexpr int pad(int p) { return p + (p >> 3); }
+__host__ __device__ constexpr int padded(int n) { return n + (n >> 3) + 1; }
+
+// W_L^m from a full length-L table in LDS
+template <typename R>
+struct TwFull {
+    const cx<R>* t;
+    __device__ __forceinline__ cx<R> get(int m) const { return t[m]; }
+};
+
+// W_L^m = hi[m >> LB] * lo[m & (2^LB - 1)] from two short LDS tables
+template <typename R, int LB>
+struct TwSplit {
+    const cx<R>* lo;
+    const cx<R>* hi;
+    __device__ __forceinline__ cx<R> get(int m) const { return hi[m >> LB] * lo[m & ((1 << LB) - 1)]; }
+};
+
+// Fill the LDS twiddle tables for length L from the global bank.
+template <typename R, int L, int NT>
+__device__ __forceinline__ void load_tw_full(cx<R>* t, const cx<R>* __restrict__ bank) {
+    const cx<R>* src = bank + tw_offset(ilog2<L>::value);
+    for (int i = threadIdx.x; i < L; i += NT) t[i] = src[i];
+}
+template <typename R, int L, int LB, int NT>
+__device__ __forceinline__ void load_tw_split(cx<R>* lo, cx<R>* hi, const cx<R>* __restrict__ bank) {
+    const cx<R>* src = bank + tw_offset(ilog2<L>::value);
+    for (int i = threadIdx.x; i < (1 << LB); i += NT) lo[i] = src[i];
+    for (int i = threadIdx.x; i < L / (1 << LB); i += NT) hi[i] = src[i << LB];
 }
 
 // radix-2/4/8 DFT in registers, exponent sign SIGN (-1 forward, +1 inverse)
@@ -84,9 +112,9 @@ struct butterfly<R, 8, SIGN> {
 };
 
 // One Stockham stage over S interleaved sequences of length L with NT
-// threads, then the remaining stages.  tw points at the length-L bank.
-template <typename R, int L, int S, int NT, int SIGN, int NS>
-__device__ __forceinline__ void fft_stages(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw) {
+// threads, then the remaining stages.
+template <typename R, int L, int S, int NT, int SIGN, int NS, typename TW>
+__device__ __forceinline__ void fft_stages(cx<R>* __restrict__ buf, const TW& tw) {
     constexpr int RADIX = (L / NS >= 8) ? 8 : (L / NS);
     constexpr int NB = L / RADIX;
     constexpr int TOTAL = NB * S;
@@ -101,11 +129,14 @@ __device__ __forceinline__ void fft_stages(cx<R>* __restrict__ buf, const cx<R>*
             const int s = id % S;
             const int j = id / S;
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) v[b][r] = buf[(j + r * NB) * S + s];
+            for (int r = 0; r < RADIX; ++r) v[b][r] = buf[pad((j + r * NB) * S + s)];
             if (NS > 1) {
                 const int k = j % NS;
 #pragma unroll
-                for (int r = 1; r < RADIX; ++r) v[b][r] = v[b][r] * twiddle<R, SIGN>(tw, r * k * TWS);
+                for (int r = 1; r < RADIX; ++r) {
+                    cx<R> w = tw.get(r * k * TWS);
+                    v[b][r] = v[b][r] * (SIGN < 0 ? w : conj(w));
+                }
             }
             butterfly<R, RADIX, SIGN>::run(v[b]);
         }
@@ -120,19 +151,19 @@ __device__ __forceinline__ void fft_stages(cx<R>* __restrict__ buf, const cx<R>*
             const int k = j % NS;
             const int base = (j / NS) * NS * RADIX + k;
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) buf[(base + r * NS) * S + s] = v[b][r];
+            for (int r = 0; r < RADIX; ++r) buf[pad((base + r * NS) * S + s)] = v[b][r];
         }
     }
     __syncthreads();
-    if constexpr (NS * RADIX < L) fft_stages<R, L, S, NT, SIGN, NS * RADIX>(buf, tw);
+    if constexpr (NS * RADIX < L) fft_stages<R, L, S, NT, SIGN, NS * RADIX, TW>(buf, tw);
 }
 
 // Unscaled DFT of S interleaved length-L sequences in LDS (natural order in
-// and out).  Caller must __syncthreads() before (data written) -- the stage
-// loop ends with a barrier.
-template <typename R, int L, int S, int NT, int SIGN>
-__device__ __forceinline__ void fft_lds(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw_bank) {
-    if constexpr (L > 1) fft_stages<R, L, S, NT, SIGN, 1>(buf, tw_bank + tw_offset(ilog2<L>::value));
+// and out).  The caller's writes must be visible (barrier) before the call;
+// the call ends with a barrier.
+template <typename R, int L, int S, int NT, int SIGN, typename TW>
+__device__ __forceinline__ void fft_lds(cx<R>* __restrict__ buf, const TW& tw) {
+    if constexpr (L > 1) fft_stages<R, L, S, NT, SIGN, 1, TW>(buf, tw);
 }
 
 }  // namespace focss_hip

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -11,8 +12,15 @@
 
 #include "../../include/focss.h"
 #include "ssfm_kernels.h"
+#include "fft_kernels.h"
 
 using namespace focss_hip;
+
+// plain 16-byte copy: the calibration kernel for HBM byte counters
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 namespace {
 
@@ -141,6 +149,68 @@ const Registry<float>& reg<float>() {
     return reg32();
 }
 
+// standalone DFT kernels: [log_n][sign 0 = forward, 1 = inverse]
+template <typename R>
+struct FftRegistry {
+    void (*whole[21][2])(cx<R>*, double, Tables<R>) = {};
+    void (*cols[21][2])(cx<R>*, int, int, Tables<R>) = {};
+    void (*rows[21][2])(const cx<R>*, cx<R>*, int, int, double, Tables<R>) = {};
+    int ncols[21] = {}, nrows[21] = {};
+};
+
+template <typename R, int LOGN>
+void reg_fft_whole(FftRegistry<R>& r) {
+    r.whole[LOGN][0] = &k_fft_whole<R, (1 << LOGN), -1>;
+    r.whole[LOGN][1] = &k_fft_whole<R, (1 << LOGN), +1>;
+}
+
+template <typename R, int LOGN>
+void reg_fft_two(FftRegistry<R>& r) {
+    constexpr int N1 = 1 << (LOGN / 2);
+    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int TILE = sizeof(R) == 8 ? 2048 : 4096;
+    constexpr int C = TILE / N1 > N2 ? N2 : TILE / N1;
+    constexpr int S = TILE / N2;
+    r.cols[LOGN][0] = &k_fft_cols<R, N1, C, -1>;
+    r.cols[LOGN][1] = &k_fft_cols<R, N1, C, +1>;
+    r.rows[LOGN][0] = &k_fft_rows<R, N2, S, -1>;
+    r.rows[LOGN][1] = &k_fft_rows<R, N2, S, +1>;
+    r.ncols[LOGN] = C;
+    r.nrows[LOGN] = S;
+}
+
+template <typename R>
+FftRegistry<R> make_fft_registry() {
+    FftRegistry<R> r;
+    reg_fft_whole<R, 1>(r);
+    reg_fft_whole<R, 2>(r);
+    reg_fft_whole<R, 3>(r);
+    reg_fft_whole<R, 4>(r);
+    reg_fft_whole<R, 5>(r);
+    reg_fft_whole<R, 6>(r);
+    reg_fft_whole<R, 7>(r);
+    reg_fft_whole<R, 8>(r);
+    reg_fft_whole<R, 9>(r);
+    reg_fft_whole<R, 10>(r);
+    reg_fft_whole<R, 11>(r);
+    reg_fft_whole<R, 12>(r);
+    reg_fft_whole<R, 13>(r);
+    reg_fft_two<R, 14>(r);
+    reg_fft_two<R, 15>(r);
+    reg_fft_two<R, 16>(r);
+    reg_fft_two<R, 17>(r);
+    reg_fft_two<R, 18>(r);
+    reg_fft_two<R, 19>(r);
+    reg_fft_two<R, 20>(r);
+    return r;
+}
+
+template <typename R>
+const FftRegistry<R>& fft_reg() {
+    static FftRegistry<R> r = make_fft_registry<R>();
+    return r;
+}
+
 int ilog2_int(int n) {
     int p = 0;
     while ((1 << p) < n) ++p;
@@ -169,12 +239,16 @@ struct focss_ctx {
     Reduction* d_red = nullptr;
     int* d_waves = nullptr;
     int* d_done = nullptr;
-    int* h_done = nullptr;  // pinned
+    int* h_done = nullptr;  // pinned, [2]
+    hipEvent_t ev_done[2] = {nullptr, nullptr};
     size_t cap_waves = 0;
     int tile_variant[2][21] = {};
     GraphCache graph;
-    int graph_pairs = 24;  // multiple of 6 (state parity 2, reduction slots 3)
+    void* scratch = nullptr;  // out-of-place buffer of the two-pass DFT
+    size_t scratch_bytes = 0;
+    int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
     long long max_split_steps = 20000000;
+    bool use_graph = true;  // FOCSS_NO_GRAPH=1: plain launches (profilers)
 };
 
 namespace {
@@ -324,7 +398,7 @@ int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
 
 template <typename R>
 int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void* batch,
-              const int64_t* offsets, int* total_steps) {
+              const int64_t* offsets, int* total_steps, int profile_pairs = 0, double* prof_ms = nullptr) {
     std::vector<WaveStatic> st(n);
     for (int i = 0; i < n; ++i) {
         int rc = fill_static(links[i], offsets[i], st[i]);
@@ -371,6 +445,52 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     a.waves = c->d_waves;
     a.n_waves = n;
     a.counter = 0;
+    if (profile_pairs > 0) {
+        // measurement mode: the first profile_pairs split steps launched one
+        // kernel at a time, each bracketed by events on the context stream
+        std::vector<hipEvent_t> ev(2 * (size_t)profile_pairs + 3);
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipMemcpyAsync(c->d_statics, st.data(), n * sizeof(WaveStatic), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_states, init.data(), 2 * n * sizeof(WaveState), hipMemcpyHostToDevice,
+                               c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_red, red0.data(), 3 * n * sizeof(Reduction), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_waves, order.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipEventRecord(ev[0], c->stream));
+        for (const LengthClass& lc : resident) launch_resident<R>(c, lc, a, tb);
+        HIP_TRY(hipEventRecord(ev[1], c->stream));
+        for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
+        HIP_TRY(hipEventRecord(ev[2], c->stream));
+        for (int k = 0; k < profile_pairs && !two_pass.empty(); ++k) {
+            PassArgs ak = a;
+            ak.counter = k;
+            for (const LengthClass& lc : two_pass) launch_spec<R>(c, lc, ak, tb);
+            HIP_TRY(hipEventRecord(ev[3 + 2 * k], c->stream));
+            ak.counter = k + 1;
+            for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
+            HIP_TRY(hipEventRecord(ev[4 + 2 * k], c->stream));
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        float t = 0;
+        double t_res = 0, t_time = 0, t_spec = 0;
+        HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+        t_res = t;
+        if (!two_pass.empty()) {
+            for (int k = 0; k < profile_pairs; ++k) {
+                HIP_TRY(hipEventElapsedTime(&t, ev[2 + 2 * k], ev[3 + 2 * k]));
+                t_spec += t;
+                HIP_TRY(hipEventElapsedTime(&t, ev[3 + 2 * k], ev[4 + 2 * k]));
+                t_time += t;
+            }
+            t_spec /= profile_pairs;
+            t_time /= profile_pairs;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        prof_ms[0] = t_time;
+        prof_ms[1] = t_spec;
+        prof_ms[2] = t_res;
+        return 0;
+    }
     if (!two_pass.empty()) {
         // graph of [spectral pass r, time pass r+1] x K, captured before any
         // work is queued; its key is everything baked into the kernel args
@@ -387,7 +507,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         put(&c->d_red, sizeof(void*));
         put(&c->d_waves, sizeof(void*));
         for (const LengthClass& lc : two_pass) put(&lc, sizeof(lc));
-        if (c->graph.key != key || !c->graph.exec) {
+        if (c->use_graph && (c->graph.key != key || !c->graph.exec)) {
             drop_graph(c);
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < c->graph_pairs; ++k) {
@@ -414,12 +534,30 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         // entry time pass (counter 0), then replays of the graph
         for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
         HIP_TRY(hipGetLastError());
+        // two replays in flight: replay k+1 is queued before the done
+        // counter read after replay k is inspected, so the GPU never waits
+        // on the host; surplus passes after completion exit at once
         const int target = n;  // resident waves also count themselves done
         for (long long rep = 0;; ++rep) {
-            HIP_TRY(hipGraphLaunch(c->graph.exec, c->stream));
-            HIP_TRY(hipMemcpyAsync(c->h_done, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            if (*c->h_done >= target) break;
+            if (c->use_graph) {
+                HIP_TRY(hipGraphLaunch(c->graph.exec, c->stream));
+            } else {
+                for (int k = 0; k < c->graph_pairs; ++k) {
+                    PassArgs ak = a;
+                    ak.counter = k;
+                    for (const LengthClass& lc : two_pass) launch_spec<R>(c, lc, ak, tb);
+                    ak.counter = k + 1;
+                    for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
+                }
+                HIP_TRY(hipGetLastError());
+            }
+            HIP_TRY(hipMemcpyAsync(&c->h_done[rep & 1], c->d_done, sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream));
+            HIP_TRY(hipEventRecord(c->ev_done[rep & 1], c->stream));
+            if (rep >= 1) {
+                HIP_TRY(hipEventSynchronize(c->ev_done[(rep - 1) & 1]));
+                if (c->h_done[(rep - 1) & 1] >= target) break;
+            }
             if (rep > c->max_split_steps / c->graph_pairs)
                 return fail(FOCSS_E_NUMERIC, "link program did not terminate within the split-step limit");
         }
@@ -442,6 +580,47 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
 
 }  // namespace
 
+namespace {
+template <typename R>
+int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
+    int log_n = ilog2_int(samples);
+    if (log_n < 1 || log_n > 20) return fail(FOCSS_E_ARG, "fft: samples must be a power of two in [2, 2^20]");
+    if (modes < 1) return fail(FOCSS_E_ARG, "fft: modes must be >= 1");
+    const int si = sign < 0 ? 0 : 1;
+    const double scale = sign < 0 ? 1.0 / samples : 1.0;
+    Tables<R> tb;
+    const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
+    tb.bank = (const cx<R>*)c->bank[prec];
+    tb.fine = (const cx<R>*)c->fine[prec];
+    const FftRegistry<R>& r = fft_reg<R>();
+    cx<R>* d = (cx<R>*)data;
+    if (r.whole[log_n][si]) {
+        hipLaunchKernelGGL(r.whole[log_n][si], dim3(modes), dim3(NT), 0, c->stream, d, scale, tb);
+    } else {
+        size_t bytes = (size_t)modes * samples * sizeof(cx<R>);
+        if (c->scratch_bytes < bytes) {
+            if (c->scratch) HIP_TRY(hipFree(c->scratch));
+            c->scratch = nullptr;
+            c->scratch_bytes = 0;
+            HIP_TRY(hipMalloc(&c->scratch, bytes));
+            c->scratch_bytes = bytes;
+        }
+        int n1 = 1 << (log_n / 2);
+        int n2 = samples / n1;
+        int tc = n2 / r.ncols[log_n];
+        int tr = n1 / r.nrows[log_n];
+        hipLaunchKernelGGL(r.cols[log_n][si], dim3(modes * tc), dim3(NT), 0, c->stream, d, log_n, tc, tb);
+        hipLaunchKernelGGL(r.rows[log_n][si], dim3(modes * tr), dim3(NT), 0, c->stream, (const cx<R>*)d,
+                           (cx<R>*)c->scratch, log_n, tr, scale, tb);
+        HIP_TRY(hipMemcpyAsync(d, c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* focss_last_error(void) { return g_err.c_str(); }
@@ -455,6 +634,12 @@ int focss_ctx_create(int device, focss_ctx** out) {
     if (!out) return fail(FOCSS_E_ARG, "null out");
     focss_ctx* c = new focss_ctx();
     c->device = device;
+    const char* ng = getenv("FOCSS_NO_GRAPH");
+    c->use_graph = !(ng && ng[0] == '1');
+    for (int p = 0; p <= 20; ++p) {
+        c->tile_variant[FOCSS_F64][p] = 1;  // 2048-element tiles for complex128
+        c->tile_variant[FOCSS_F32][p] = 0;  // 4096-element tiles for complex64
+    }
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
         delete c;
@@ -474,7 +659,9 @@ int focss_ctx_create(int device, focss_ctx** out) {
     HIP_TRY(hipMemcpy(c->bank[1], b32.data(), b32.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->fine[1], f32.data(), f32.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_done, sizeof(int)));
-    HIP_TRY(hipHostMalloc(&c->h_done, sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&c->h_done, 2 * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_done[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_done[1], hipEventDisableTiming));
     *out = c;
     return 0;
 }
@@ -495,7 +682,10 @@ int focss_ctx_destroy(focss_ctx* c) {
         (void)hipFree(c->d_waves);
     }
     if (c->d_done) (void)hipFree(c->d_done);
+    if (c->scratch) (void)hipFree(c->scratch);
     if (c->h_done) (void)hipHostFree(c->h_done);
+    for (int i = 0; i < 2; ++i)
+        if (c->ev_done[i]) (void)hipEventDestroy(c->ev_done[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -575,14 +765,64 @@ int focss_link_run_host(focss_ctx* c, int precision, const focss_link* link, dou
     return rc;
 }
 
+int focss_link_profile(focss_ctx* c, int precision, int n_waves, const focss_link* links, void* batch,
+                       const int64_t* offsets, int n_pairs, double* avg_ms) {
+    if (!c || !links || !batch || !offsets || !avg_ms || n_waves < 1 || n_pairs < 1)
+        return fail(FOCSS_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == FOCSS_F64)
+        return run_links<double>(c, precision, n_waves, links, batch, offsets, nullptr, n_pairs, avg_ms);
+    if (precision == FOCSS_F32)
+        return run_links<float>(c, precision, n_waves, links, batch, offsets, nullptr, n_pairs, avg_ms);
+    return fail(FOCSS_E_ARG, "bad precision");
+}
+
+int focss_device_copy(focss_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src || (bytes % 16) != 0) return fail(FOCSS_E_ARG, "copy needs 16-byte multiples");
+    size_t n = bytes / 16;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, c->stream, (const uint4*)src, (uint4*)dst, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void* focss_ctx_stream(focss_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int focss_link_validate(const focss_link* link) {
+    if (!link) return fail(FOCSS_E_ARG, "null link");
+    WaveStatic w;
+    return fill_static(*link, 0, w);
+}
+
 int focss_fft(focss_ctx* c, int precision, int sign, int modes, int samples, void* data) {
-    (void)c;
-    (void)precision;
-    (void)sign;
-    (void)modes;
-    (void)samples;
-    (void)data;
-    return fail(FOCSS_E_UNSUPPORTED, "focss_fft: not built yet");
+    if (!c || !data) return fail(FOCSS_E_ARG, "null argument");
+    if (sign != -1 && sign != 1) return fail(FOCSS_E_ARG, "fft: sign must be -1 or +1");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == FOCSS_F64) return fft_impl<double>(c, sign, modes, samples, data);
+    if (precision == FOCSS_F32) return fft_impl<float>(c, sign, modes, samples, data);
+    return fail(FOCSS_E_ARG, "bad precision");
+}
+
+int focss_add_noise(focss_ctx* c, int precision, int modes, int samples, double sigma, uint64_t seed,
+                    uint32_t stream, void* data) {
+    if (!c || !data || modes < 1 || samples < 1) return fail(FOCSS_E_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    long long n = (long long)modes * samples;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    if (precision == FOCSS_F64)
+        hipLaunchKernelGGL(k_add_noise<double>, dim3(blocks), dim3(256), 0, c->stream, (cx<double>*)data, n, sigma,
+                           (unsigned long long)seed, stream);
+    else if (precision == FOCSS_F32)
+        hipLaunchKernelGGL(k_add_noise<float>, dim3(blocks), dim3(256), 0, c->stream, (cx<float>*)data, n, sigma,
+                           (unsigned long long)seed, stream);
+    else
+        return fail(FOCSS_E_ARG, "bad precision");
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int focss_set_tile(focss_ctx* c, int precision, int log2_samples, int variant) {

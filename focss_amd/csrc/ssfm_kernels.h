@@ -104,14 +104,19 @@ __device__ __forceinline__ cx<R> spectral_factor(const WaveStatic& w, const Spec
 // ------------------------------------------------------------------
 template <typename R, int N, int M>
 __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
-    __shared__ cx<R> buf[N * M];
+    constexpr int LB = (ilog2<N>::value + 1) / 2;
+    __shared__ cx<R> buf[padded(N * M)];
+    __shared__ cx<R> tw_lo[1 << LB];
+    __shared__ cx<R> tw_hi[N >> LB];
     __shared__ double redbuf[2 * (NT / 64)];
     const int w_id = a.waves[blockIdx.x];
     const WaveStatic w = a.statics[w_id];
     cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset;
+    load_tw_split<R, N, LB, NT>(tw_lo, tw_hi, tb.bank);
+    const TwSplit<R, LB> tw{tw_lo, tw_hi};
     for (int i = threadIdx.x; i < N * M; i += NT) {
         int m = i / N, n = i % N;
-        buf[n * M + m] = g[i];
+        buf[pad(n * M + m)] = g[i];
     }
     __syncthreads();
     WaveState st = initial_state(w);
@@ -120,13 +125,13 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
     for (int n = threadIdx.x; n < N; n += NT) {
         cx<R> v[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = buf[n * M + m];
+        for (int m = 0; m < M; ++m) v[m] = buf[pad(n * M + m)];
         time_op<R, M>(v, w, st, n, pk, sm);
 #pragma unroll
-        for (int m = 0; m < M; ++m) buf[n * M + m] = v[m];
+        for (int m = 0; m < M; ++m) buf[pad(n * M + m)] = v[m];
     }
     block_max_sum<NT>(pk, sm, redbuf);
-    fft_lds<R, N, M, NT, -1>(buf, tb.bank);
+    fft_lds<R, N, M, NT, -1>(buf, tw);
     for (;;) {
         SpectralOp op = plan_spectral(w, st, pk, sm);
         if (!op.active) break;
@@ -135,27 +140,27 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
         for (int k = threadIdx.x; k < N; k += NT) {
             cx<R> f = spectral_factor<R>(w, op, k, arg, mag);
 #pragma unroll
-            for (int m = 0; m < M; ++m) buf[k * M + m] = buf[k * M + m] * f;
+            for (int m = 0; m < M; ++m) buf[pad(k * M + m)] = buf[pad(k * M + m)] * f;
         }
         __syncthreads();
-        fft_lds<R, N, M, NT, +1>(buf, tb.bank);
+        fft_lds<R, N, M, NT, +1>(buf, tw);
         pk = 0;
         sm = 0;
         for (int n = threadIdx.x; n < N; n += NT) {
             cx<R> v[M];
 #pragma unroll
-            for (int m = 0; m < M; ++m) v[m] = buf[n * M + m];
+            for (int m = 0; m < M; ++m) v[m] = buf[pad(n * M + m)];
             time_op<R, M>(v, w, st, n, pk, sm);
 #pragma unroll
-            for (int m = 0; m < M; ++m) buf[n * M + m] = v[m];
+            for (int m = 0; m < M; ++m) buf[pad(n * M + m)] = v[m];
         }
         block_max_sum<NT>(pk, sm, redbuf);
         if (st.top == TOP_EXIT) break;
-        fft_lds<R, N, M, NT, -1>(buf, tb.bank);
+        fft_lds<R, N, M, NT, -1>(buf, tw);
     }
     for (int i = threadIdx.x; i < N * M; i += NT) {
         int m = i / N, n = i % N;
-        g[i] = buf[n * M + m];
+        g[i] = buf[pad(n * M + m)];
     }
     if (threadIdx.x == 0) {
         a.states[w_id] = st;
@@ -170,7 +175,8 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
 template <typename R, int N1, int C, int M>
 __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
     constexpr int S = C * M;
-    __shared__ cx<R> buf[N1 * S];
+    __shared__ cx<R> buf[padded(N1 * S)];
+    __shared__ cx<R> tw_t[N1];
     __shared__ double redbuf[2 * (NT / 64)];
     const int w_id = a.waves[blockIdx.x / a.tiles_per_wave];
     const int tile = blockIdx.x % a.tiles_per_wave;
@@ -183,24 +189,26 @@ __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
     cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset;
     const bool inverse = st.top != TOP_ENTRY;
     const bool forward = st.top != TOP_EXIT;
+    load_tw_full<R, N1, NT>(tw_t, tb.bank);
+    const TwFull<R> tw{tw_t};
     for (int i = threadIdx.x; i < N1 * S; i += NT) {
         int n1 = i / S, s = i % S;
         int m = s / C, c = s % C;
         cx<R> v = g[(long long)m * N + (long long)n1 * N2 + n2_0 + c];
         if (inverse) v = v * conj(tw_big<R>(tb, (unsigned)((n2_0 + c) * n1) & (unsigned)(N - 1), a.log_n));
-        buf[n1 * S + s] = v;
+        buf[pad(n1 * S + s)] = v;
     }
     __syncthreads();
-    if (inverse) fft_lds<R, N1, S, NT, +1>(buf, tb.bank);
+    if (inverse) fft_lds<R, N1, S, NT, +1>(buf, tw);
     double pk = 0, sm = 0;
     for (int i = threadIdx.x; i < N1 * C; i += NT) {
         int n1 = i / C, c = i % C;
         cx<R> v[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = buf[n1 * S + m * C + c];
+        for (int m = 0; m < M; ++m) v[m] = buf[pad(n1 * S + m * C + c)];
         time_op<R, M>(v, w, st, (long long)n1 * N2 + n2_0 + c, pk, sm);
 #pragma unroll
-        for (int m = 0; m < M; ++m) buf[n1 * S + m * C + c] = v[m];
+        for (int m = 0; m < M; ++m) buf[pad(n1 * S + m * C + c)] = v[m];
     }
     block_max_sum<NT>(pk, sm, redbuf);
     if (threadIdx.x == 0) {
@@ -208,11 +216,11 @@ __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
         atomicMax(&r->peak_bits, double_to_bits(pk));
         atomicAdd(&r->sum, sm);
     }
-    if (forward) fft_lds<R, N1, S, NT, -1>(buf, tb.bank);
+    if (forward) fft_lds<R, N1, S, NT, -1>(buf, tw);
     for (int i = threadIdx.x; i < N1 * S; i += NT) {
         int n1 = i / S, s = i % S;
         int m = s / C, c = s % C;
-        cx<R> v = buf[n1 * S + s];
+        cx<R> v = buf[pad(n1 * S + s)];
         if (forward) v = v * tw_big<R>(tb, (unsigned)((n2_0 + c) * n1) & (unsigned)(N - 1), a.log_n);
         g[(long long)m * N + (long long)n1 * N2 + n2_0 + c] = v;
     }
@@ -221,7 +229,8 @@ __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
 // spectral pass: S rows of length N2; row q = m * N1 + k1 is contiguous
 template <typename R, int N2, int S>
 __global__ __launch_bounds__(NT) void k_spec_pass(PassArgs a, Tables<R> tb) {
-    __shared__ cx<R> buf[N2 * S];
+    __shared__ cx<R> buf[padded(N2 * S)];
+    __shared__ cx<R> tw_t[N2];
     const int w_id = a.waves[blockIdx.x / a.tiles_per_wave];
     const int tile = blockIdx.x % a.tiles_per_wave;
     WaveState st = a.states[(a.counter & 1) * a.n_waves + w_id];
@@ -241,25 +250,27 @@ __global__ __launch_bounds__(NT) void k_spec_pass(PassArgs a, Tables<R> tb) {
     const int N = w.samples;
     const int N1 = N / N2;
     cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset + (long long)tile * S * N2;
+    load_tw_full<R, N2, NT>(tw_t, tb.bank);
+    const TwFull<R> tw{tw_t};
     for (int i = threadIdx.x; i < N2 * S; i += NT) {
         int s = i / N2, n2 = i % N2;
-        buf[n2 * S + s] = g[i];
+        buf[pad(n2 * S + s)] = g[i];
     }
     __syncthreads();
-    fft_lds<R, N2, S, NT, -1>(buf, tb.bank);
+    fft_lds<R, N2, S, NT, -1>(buf, tw);
     const double arg = 0.5 * w.beta2 * op.s;
     const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
     for (int i = threadIdx.x; i < N2 * S; i += NT) {
         int k2 = i / S, s = i % S;
         int q = tile * S + s;
         int k1 = q % N1;
-        buf[i] = buf[i] * spectral_factor<R>(w, op, k1 + N1 * k2, arg, mag);
+        buf[pad(i)] = buf[pad(i)] * spectral_factor<R>(w, op, k1 + N1 * k2, arg, mag);
     }
     __syncthreads();
-    fft_lds<R, N2, S, NT, +1>(buf, tb.bank);
+    fft_lds<R, N2, S, NT, +1>(buf, tw);
     for (int i = threadIdx.x; i < N2 * S; i += NT) {
         int s = i / N2, n2 = i % N2;
-        g[i] = buf[n2 * S + s];
+        g[i] = buf[pad(n2 * S + s)];
     }
 }
 

@@ -8,7 +8,7 @@
  * Field layout everywhere: mode-major interleaved complex, element
  * (mode, sample) at data[sample + mode * samples]   (src/focss/field.cpp:140-155).
  * precision FOCSS_F64 = complex<double> (16 B), FOCSS_F32 = complex<float> (8 B).
- * samples must be a power of two, 2^6 .. 2^20.
+ * samples must be a power of two, 2^6 .. 2^20 (2 .. 2^20 for focss_fft).
  *
  * A context owns one HIP stream on one device plus cached device buffers and
  * captured graphs.  A context is used by one host thread at a time; distinct
@@ -107,6 +107,26 @@ int focss_synchronize(focss_ctx* ctx);
 int focss_link_run(focss_ctx* ctx, int precision, int n_waves, const focss_link* links,
                    void* batch, const int64_t* offsets, int* total_steps);
 
+/* Measurement: runs only the first n_pairs split steps of the programs, one
+ * kernel launch at a time, each bracketed by HIP events on the context's
+ * stream.  avg_ms[0] = mean time-pass duration, avg_ms[1] = mean
+ * spectral-pass duration (two-pass waveforms), avg_ms[2] = duration of the
+ * LDS-resident launch (whole programs).  The field is left mid-program. */
+int focss_link_profile(focss_ctx* ctx, int precision, int n_waves, const focss_link* links,
+                       void* batch, const int64_t* offsets, int n_pairs, double* avg_ms);
+
+/* Host-only argument check of one link program (no device needed): the
+ * guards the reference leaves to assert or to an endless loop -- h_max = 0
+ * on the ADAPTIVE grid (ssfm.cpp:99), alpha = 0 on the LOGARITHMIC grid
+ * (ssfm.cpp:76), non-power-of-two or out-of-range sizes, bad direction. */
+int focss_link_validate(const focss_link* link);
+
+/* Device-to-device copy by a plain streaming kernel (HBM counter calibration). */
+int focss_device_copy(focss_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* The context's hipStream_t, for callers that record their own events. */
+void* focss_ctx_stream(focss_ctx* ctx);
+
 /* Host-buffer convenience for one waveform: complex<double> in/out, computed
  * at `precision` on the device.  Used by focss::SSFM::run. */
 int focss_link_run_host(focss_ctx* ctx, int precision, const focss_link* link,
@@ -117,6 +137,12 @@ int focss_link_run_host(focss_ctx* ctx, int precision, const focss_link* link,
  * src/focss/field.cpp:343-347); sign +1: inverse, unscaled
  * (Field::ifft_inplace, src/focss/field.cpp:349-352). */
 int focss_fft(focss_ctx* ctx, int precision, int sign, int modes, int samples, void* data);
+
+/* Amplifier::add_noise (src/focss/module/amplifier.cpp:20-28) on a device
+ * field: element i (flat, mode-major) += sigma * (z0 + i z1), (z0, z1) the
+ * Box-Muller pair of Philox4x32-10 block (i, stream) under key seed. */
+int focss_add_noise(focss_ctx* ctx, int precision, int modes, int samples, double sigma,
+                    uint64_t seed, uint32_t stream, void* data);
 
 /* Tuning: choose the tile shape of the two-pass kernels for one length
  * (0 = default).  Exposed for the autotuner. */

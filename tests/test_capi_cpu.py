@@ -1,0 +1,82 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol
+include/*.h declares, and its host-side argument guards reject the inputs
+the reference would assert on or loop forever on."""
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(focss_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "focss_link_run" in names and "focss_ctx_create" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    import focss_amd as F
+    lib = ctypes.CDLL(F.LIB_PATH)
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(F.EXPORTED) <= set(declared_symbols())
+
+
+def _link(F, **kw):
+    fib = F.Fiber(4.6e-5, -2.17e-26, 1.3e-3, 80e3, 1.55e-6, 1.0)
+    d = dict(grid=F.Grid(F.ADAPTIVE, 0, 1e-3, 1e3), n=1 << 12, dt=1e-11, spans=1, direction=1.0, modes=1)
+    d.update(kw)
+    if "fiber" in kw:
+        fib = kw["fiber"]
+    return F.make_link(fib, d["grid"], d["direction"], d["spans"], d["n"], d["dt"], modes=d["modes"])
+
+
+def test_validate_accepts_good_link():
+    import focss_amd as F
+    F.validate(_link(F))
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("hmax0", "max_step_size"),
+    ("log_alpha0", "alpha"),
+    ("npot", "power of two"),
+    ("small", "power of two"),
+    ("modes3", "modes"),
+    ("dir", "direction"),
+    ("steps0", "total_steps"),
+    ("dt0", "dt"),
+])
+def test_validate_rejects(case, msg):
+    import focss_amd as F
+    if case == "hmax0":
+        link = _link(F, grid=F.Grid(F.ADAPTIVE, 0, 1e-3, 0.0))
+    elif case == "log_alpha0":
+        link = _link(F, grid=F.Grid(F.LOGARITHMIC, 0, 1e-3, 1e3),
+                     fiber=F.Fiber(0.0, -2.17e-26, 1.3e-3, 80e3, 1.55e-6, 1.0))
+    elif case == "npot":
+        link = _link(F, n=3000)
+    elif case == "small":
+        link = _link(F, n=16)
+    elif case == "modes3":
+        link = _link(F, modes=3)
+    elif case == "dir":
+        link = _link(F, direction=0.5)
+    elif case == "steps0":
+        link = _link(F, grid=F.Grid(F.UNIFORM, 0, 0.0, 0.0))
+    else:
+        link = _link(F, dt=0.0)
+    with pytest.raises(F.FocssError, match=msg):
+        F.validate(link)

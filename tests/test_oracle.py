@@ -1,0 +1,409 @@
+"""Pins the CPU FP64 oracle with checks independent of its own code: brute
+force, numpy.fft, closed-form solutions of the NLSE pieces, solitons,
+invariants and published known-answer vectors.  The reference ships no
+tests or golden vectors, so these are the parity anchors."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def crandn(r, n):
+    return r.standard_normal(n) + 1j * r.standard_normal(n)
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 64])
+def test_fft_vs_bruteforce_long_double(n):
+    x = crandn(rng(n), n)
+    ref_f = O.dft_naive(x, -1)
+    ref_i = O.dft_naive(x, +1)
+    assert np.max(np.abs(O.fft(x) - ref_f / n)) < 1e-14 * np.max(np.abs(ref_f))
+    assert np.max(np.abs(O.ifft(x) - ref_i)) < 1e-14 * np.max(np.abs(ref_i))
+
+
+@pytest.mark.parametrize("n", [2 ** k for k in range(1, 15)])
+def test_fft_vs_numpy(n):
+    x = crandn(rng(1), 2 * n)
+    y = O.fft(x, modes=2)
+    ref = np.concatenate([np.fft.fft(x[:n]), np.fft.fft(x[n:])]) / n
+    assert np.linalg.norm(y - ref) <= 1e-14 * np.linalg.norm(ref)
+    z = O.ifft(x, modes=2)
+    refi = np.concatenate([np.fft.ifft(x[:n]), np.fft.ifft(x[n:])]) * n
+    assert np.linalg.norm(z - refi) <= 1e-14 * np.linalg.norm(refi)
+
+
+def test_fft_roundtrip_and_parseval():
+    x = crandn(rng(2), 4096)
+    X = O.fft(x)
+    # reference normalisation: sum |x|^2 = N sum |X|^2
+    assert abs(np.sum(abs(x) ** 2) - 4096 * np.sum(abs(X) ** 2)) < 1e-10 * np.sum(abs(x) ** 2)
+    assert np.max(np.abs(O.ifft(X) - x)) < 1e-13
+
+
+# ---------------------------------------------------------------- reductions
+def test_power_reductions():
+    x = crandn(rng(3), 2 * 64)
+    p = np.abs(x[:64]) ** 2 + np.abs(x[64:]) ** 2
+    assert O.power(x, 2, 5) == pytest.approx(p[5], rel=1e-15)
+    assert O.peak_power(x, 2) == pytest.approx(p.max(), rel=1e-15)
+    assert O.average_power(x, 2) == pytest.approx(np.sum(np.abs(x) ** 2) / 64, rel=1e-14)
+
+
+# ---------------------------------------------------------------- SSFM closed forms
+def gaussian_case(n=4096, T=400e-12, t0=10e-12):
+    dt = T / n
+    t = (np.arange(n) - n // 2) * dt
+    return t, dt, np.exp(-t ** 2 / (2 * t0 ** 2)).astype(np.complex128), t0
+
+
+@pytest.mark.parametrize("steps", [1, 7])
+def test_pure_dispersion_gaussian_closed_form(steps):
+    """dA/dz = -i beta2/2 A_tt: a Gaussian broadens as
+    T0/sqrt(T0^2 - i beta2 z) exp(-t^2 / (2 (T0^2 - i beta2 z))).  Pins the
+    +i sign of the spectral phase (ssfm.cpp:146-148) and the w grid."""
+    t, dt, x, t0 = gaussian_case()
+    beta2, L = -21.7e-27, 10e3
+    f = O.fiber(0.0, beta2, 0.0, L)
+    y, _ = O.ssfm_run(f, O.grid(O.UNIFORM, steps), O.FORWARD, x, dt)
+    q = t0 ** 2 - 1j * beta2 * L
+    ref = t0 / np.sqrt(q) * np.exp(-t ** 2 / (2 * q))
+    assert np.max(np.abs(y - ref)) < 1e-10
+
+
+def test_pure_kerr_closed_form():
+    x = crandn(rng(4), 256) * 0.05
+    f = O.fiber(0.0, 0.0, 1.3e-3, 80e3, kappa=8 / 9)
+    y, _ = O.ssfm_run(f, O.grid(O.UNIFORM, 13), O.FORWARD, x, 1e-12)
+    ref = x * np.exp(1j * (8 / 9) * 1.3e-3 * 80e3 * np.abs(x) ** 2)
+    assert np.max(np.abs(y - ref)) < 1e-13
+    assert np.allclose(np.abs(y), np.abs(x), rtol=1e-13)
+
+
+def test_pure_loss_closed_form():
+    x = crandn(rng(5), 128)
+    alpha = O.db_to_natural(0.2e-3)
+    f = O.fiber(alpha, 0.0, 0.0, 80e3)
+    for g in (O.grid(O.UNIFORM, 5), O.grid(O.ADAPTIVE, 0, 1e-3, 1e3)):
+        y, _ = O.ssfm_run(f, g, O.FORWARD, x, 1e-12)
+        assert np.max(np.abs(y - x * np.exp(-alpha * 80e3 / 2))) < 1e-13
+    # drop_power of a 16 dB span is the exact inverse
+    assert O.db_to_linear(16.0) == pytest.approx(np.exp(alpha * 80e3), rel=1e-12)
+
+
+def sech_case(n=1024, window=40.0):
+    dt = window / n
+    t = (np.arange(n) - n // 2) * dt
+    return t, dt
+
+
+def test_bright_soliton_second_order_convergence():
+    """sqrt(P) sech(sqrt(P) t) e^{i P z / 2} solves the equation for beta2 = -1,
+    gamma = 1 (the unused sech_pulse of verification.cpp:10-22); the symmetric
+    split step converges as h^2."""
+    t, dt = sech_case()
+    x = (1 / np.cosh(t)).astype(np.complex128)
+    f = O.fiber(0.0, -1.0, 1.0, 1.0)
+    ref = np.exp(0.5j) / np.cosh(t)
+    errs = []
+    for steps in (25, 50, 100):
+        y, _ = O.ssfm_run(f, O.grid(O.UNIFORM, steps), O.FORWARD, x, dt)
+        errs.append(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+    assert errs[-1] < 1e-5
+    for a, b in zip(errs, errs[1:]):
+        assert 3.6 < a / b < 4.4
+
+
+def test_dark_soliton_pair():
+    """Dark soliton sqrt(P) tanh(sqrt(P) t) e^{i P z} for beta2 = +1
+    (verification.cpp:24-52).  The reference's single tanh jumps at the
+    periodic edge; a well separated pair tanh(t + a) tanh(a - t) is
+    continuous and exact up to e^{-4a}.  Flipping the dispersion sign must
+    break it."""
+    n, window = 1024, 40.0
+    dt = window / n
+    t = (np.arange(n) - n // 2) * dt
+    a = window / 4
+    x = (np.tanh(t + a) * np.tanh(a - t)).astype(np.complex128)
+    ref = np.exp(1j) * x
+    good, _ = O.ssfm_run(O.fiber(0.0, 1.0, 1.0, 1.0), O.grid(O.UNIFORM, 400), O.FORWARD, x, dt)
+    bad, _ = O.ssfm_run(O.fiber(0.0, -1.0, 1.0, 1.0), O.grid(O.UNIFORM, 400), O.FORWARD, x, dt)
+    e_good = np.linalg.norm(good - ref) / np.linalg.norm(ref)
+    e_bad = np.linalg.norm(bad - ref) / np.linalg.norm(ref)
+    assert e_good < 1e-5
+    assert e_bad > 1e-2
+
+
+@pytest.mark.parametrize("steps", [1, 3, 10])
+def test_uniform_forward_backward_identity(steps):
+    """S(-h) = S(h)^-1 for the symmetric step, so BACKWARD after FORWARD on the
+    same fibre (all of alpha, beta2, gamma on) is the identity."""
+    x = crandn(rng(6), 1024) * 0.03
+    alpha = O.db_to_natural(0.2e-3)
+    f = O.fiber(alpha, -21.7e-27, 1.3e-3, 80e3)
+    y, _ = O.ssfm_run(f, O.grid(O.UNIFORM, steps), O.FORWARD, x, 1 / 64e9)
+    z, _ = O.ssfm_run(f, O.grid(O.UNIFORM, steps), O.BACKWARD, y, 1 / 64e9)
+    assert np.linalg.norm(z - x) / np.linalg.norm(x) < 1e-12
+
+
+def test_energy_conserved_without_loss():
+    x = crandn(rng(7), 1024) * 0.03
+    f = O.fiber(0.0, -21.7e-27, 1.3e-3, 80e3)
+    for g in (O.grid(O.UNIFORM, 9), O.grid(O.ADAPTIVE, 0, 1e-2, 1e3)):
+        y, _ = O.ssfm_run(f, g, O.FORWARD, x, 1 / 64e9)
+        assert np.sum(np.abs(y) ** 2) == pytest.approx(np.sum(np.abs(x) ** 2), rel=1e-12)
+
+
+def test_cd_compensation_inverts_dispersion():
+    x = crandn(rng(8), 2048) * 0.03
+    dt = 1 / 64e9
+    f = O.fiber(O.db_to_natural(0.2e-3), -21.7e-27, 1.3e-3, 80e3)
+    lin = O.fiber(0.0, -21.7e-27, 0.0, 80e3 * 5)
+    y, _ = O.ssfm_run(lin, O.grid(O.UNIFORM, 1), O.FORWARD, x, dt)
+    z = O.cd_compensation(f, 5, y, dt)
+    assert np.linalg.norm(z - x) / np.linalg.norm(x) < 1e-13
+
+
+def test_adaptive_step_count_cw():
+    """A CW field stays CW; with P(z) = P0 e^{-alpha z} the step rule
+    h = min(h_max, phi / (kappa gamma P)) (ssfm.cpp:136-140) is computed
+    here from the analytic power, independently of the field.  The
+    reference measures the peak after the Kerr op, i.e. half way through the
+    step (ssfm.cpp:99-103)."""
+    alpha = O.db_to_natural(0.2e-3)
+    P0, kg, phi, hmax, L = 0.02, 1.3e-3, 1e-3, 1e3, 80e3
+    f = O.fiber(alpha, -21.7e-27, kg, L)
+    x = np.full(64, np.sqrt(P0), dtype=np.complex128)
+    y, steps = O.ssfm_run(f, O.grid(O.ADAPTIVE, 0, phi, hmax), O.FORWARD, x, 1e-11)
+    d, h, n = 0.0, min(hmax, phi / (kg * P0)), 1
+    while d + h < L:
+        h_next = min(hmax, phi / (kg * P0 * np.exp(-alpha * (d + h / 2))))
+        d += h
+        h = h_next
+        n += 1
+    assert steps == n
+    assert np.allclose(np.abs(y), np.sqrt(P0) * np.exp(-alpha * L / 2), rtol=1e-12)
+
+
+def test_logarithmic_steps_and_loss():
+    alpha = O.db_to_natural(0.2e-3)
+    P0, g, phi, hmax, L = 0.01, 1.3e-3, 1e-3, 1e3, 80e3
+    f = O.fiber(alpha, 0.0, g, L)
+    x = np.full(64, np.sqrt(P0), dtype=np.complex128)
+    y, steps = O.ssfm_run(f, O.grid(O.LOGARITHMIC, 0, phi, hmax), O.FORWARD, x, 1e-11)
+    l_nl = 1 / g / P0
+    l_eff = (1 - np.exp(-alpha * L)) / alpha
+    l_nl_eff = (1 - np.exp(-alpha * phi * l_nl)) / alpha
+    assert steps == max(int(L / hmax), int(l_eff / l_nl_eff))
+    # total Kerr phase of a CW field = gamma * integral of P = gamma P0 L_eff
+    # exactly in the limit; with the logarithmic grid each step carries the
+    # same phase, so the discrete phase is close to it
+    ph = np.angle(y[0] / np.abs(y[0]))
+    assert ph == pytest.approx(g * P0 * l_eff, rel=2e-3)
+    assert np.allclose(np.abs(y), np.sqrt(P0) * np.exp(-alpha * L / 2), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- RNG
+def test_philox_known_answers():
+    """Random123 Philox4x32-10 known-answer vectors."""
+    assert [hex(v) for v in O.philox([0, 0, 0, 0], [0, 0])] == \
+        ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(v) for v in O.philox([0xffffffff] * 4, [0xffffffff] * 2)] == \
+        ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(v) for v in O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                                     [0xa4093822, 0x299f31d0])] == \
+        ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_gaussian_statistics():
+    z = np.array([O.gauss_pair(123, 4, i) for i in range(40000)]).ravel()
+    assert abs(z.mean()) < 0.02
+    assert abs(z.var() - 1) < 0.03
+    assert abs(np.mean(z ** 4) - 3) < 0.15
+
+
+def test_ase_variance_formula():
+    """amplifier.cpp:20-28: per-quadrature variance h f_c NF (G-1)/2 / dt / 2."""
+    n = 1 << 15
+    gain, nf, fc, dt = 10 ** 1.6, 10 ** 0.5, 193.4e12, 1 / 64e9
+    y = O.amplify(np.zeros(n, dtype=np.complex128), gain, nf, fc, dt, 9, 0)
+    var = O.PLANCK * fc * nf * (gain - 1) / 2 / dt / 2
+    assert np.var(y.real) == pytest.approx(var, rel=0.03)
+    assert np.var(y.imag) == pytest.approx(var, rel=0.03)
+
+
+# ---------------------------------------------------------------- modulation / scoring
+def test_gray_map_and_demodulation():
+    pts = [O.gray16(c) for c in range(16)]
+    assert np.mean(np.abs(pts) ** 2) == pytest.approx(1.0)
+    for c, p in enumerate(pts):
+        assert O.demodulate_16qam(p) == c
+        assert O.demodulate_16qam(p + 0.05 * (1 + 1j) / np.sqrt(10)) == c
+    # Gray: horizontally / vertically adjacent points differ in one bit
+    for a in range(16):
+        for b in range(16):
+            d = abs(pts[a] - pts[b]) * np.sqrt(10)
+            if abs(d - 2) < 1e-9:
+                assert bin(a ^ b).count("1") == 1
+    # exact tie between codes 5 (-1-1j) and 7 (1-1j): strict '>' keeps the lower code
+    assert O.demodulate_16qam(complex(0, -1) / np.sqrt(10)) == 5
+
+
+def test_rrc_shaping_sampling_recovers_symbols():
+    syms, _ = O.qam16_symbols(42, 512)
+    for osf in (2, 8):
+        x = O.rrc_shaping(syms, osf, 0.1)
+        r = O.rrc_sampling(x, osf, 0.1)
+        # RRC x RRC is Nyquist: symbol-spaced samples are ISI-free up to scale
+        scale = np.vdot(syms, r) / np.vdot(syms, syms)
+        assert np.linalg.norm(r / scale - syms) / np.linalg.norm(syms) < 1e-3
+
+
+def test_q2_factor_and_ber():
+    syms, codes = O.qam16_symbols(1, 1000)
+    assert O.q2_factor(syms, syms * (1 + 0.01)) == pytest.approx(40.0, abs=1e-9)
+    assert O.ber_16qam(syms, syms) == 0.0
+    rx = syms.copy()
+    rx[0] = O.gray16(codes[0] ^ 1)  # one bit wrong
+    rx[1] = O.gray16(codes[1] ^ 6)  # two bits wrong
+    assert O.ber_16qam(syms, rx) == pytest.approx(3 / 4000)
+    assert np.array_equal(O.hd_sequence_16qam(syms), codes)
+
+
+def test_pse_finds_rotation():
+    syms, _ = O.qam16_symbols(3, 400)
+    ang = 2 * np.pi * 137 / 3600
+    rx = syms * np.exp(-1j * ang)
+    est = O.pse_estimate(syms, rx, 3600)
+    assert est == pytest.approx(ang, abs=1e-12)
+
+
+def test_decimate_bandlimited():
+    n, f = 1024, 8
+    k = np.arange(n)
+    x = np.exp(2j * np.pi * 3 * k / n) + 0.5 * np.exp(-2j * np.pi * 20 * k / n)
+    y = O.decimate(x, f)
+    assert np.max(np.abs(y - x[::f])) < 1e-12
+
+
+def test_wdm_select_isolates_channel():
+    n, dt = 4096, 1 / 256e9
+    s = np.arange(n)
+    ch_a = np.exp(2j * np.pi * 5 * s / n)
+    f_b = 50e9
+    ch_b = 0.7 * np.exp(2j * np.pi * 3 * s / n) * np.exp(2j * np.pi * f_b * dt * s)
+    out = O.wdm_select(ch_a + ch_b, dt, 1, 50e9)
+    assert np.max(np.abs(out - 0.7 * np.exp(2j * np.pi * 3 * s / n))) < 1e-10
+
+
+# ---------------------------------------------------------------- triplet equalizer
+def test_sino_term_definition_small():
+    x = crandn(rng(9), 2 * 40)
+    f0, f1 = x[:40], x[40:]
+    k, m, n = 15, 2, -3
+    ref = f0[k + m] * (f0[k + n] * np.conj(f0[k + m + n]) + f1[k + n] * np.conj(f1[k + m + n]))
+    assert O.sino_term(x, 2, 0, k, m, n) == pytest.approx(ref, rel=1e-14)
+
+
+def test_sino_cubic_spm_only_weight():
+    """W = delta(0,0): cubic(k) = W00 A(k) sum_p |A_p(k)|^2 away from the edges."""
+    R = 3
+    x = crandn(rng(10), 2 * 50)
+    W = np.zeros((2 * R + 1, 2 * R + 1), dtype=np.complex128)
+    W[R, R] = 0.3 - 0.2j
+    y = O.sino_cubic(W.ravel(), R, x, modes=2)
+    p = np.abs(x[:50]) ** 2 + np.abs(x[50:]) ** 2
+    for mode in range(2):
+        a = x[mode * 50:(mode + 1) * 50]
+        ref = W[R, R] * a * p
+        assert np.allclose(y[mode * 50 + R:mode * 50 + 50 - R], ref[R:50 - R], rtol=1e-13)
+        assert np.all(y[mode * 50:mode * 50 + R] == 0)
+
+
+def test_complex_lstsq_vs_numpy():
+    r = rng(11)
+    a = crandn(r, 60 * 7).reshape(60, 7)
+    b = crandn(r, 60)
+    ref = np.linalg.lstsq(a, b, rcond=None)[0]
+    assert np.allclose(O.complex_lstsq(a, b), ref, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("modes", [1, 2])
+def test_sino_train_recovers_planted_model(modes):
+    """rx = w tx + cubic(tx; W) with W in the symmetric family the training
+    folds over (sino.cpp:122-148); training must return W and w exactly."""
+    R = 4
+    n = 300
+    tx = crandn(rng(12), modes * n) * 0.5
+    r = rng(13)
+    W = np.zeros((2 * R + 1, 2 * R + 1), dtype=np.complex128)
+    for m in range(-(R // 2), R // 2 + 1):
+        for nn in range(abs(m), R - abs(m) + 1):
+            v = complex(r.standard_normal(), r.standard_normal()) * 0.1
+            for (a, b) in {(m, nn), (nn, m), (-m, -nn), (-nn, -m)}:
+                W[R + a, R + b] = v
+    w_lin = 0.9 + 0.1j
+    rx = w_lin * tx + O.sino_cubic(W.ravel(), R, tx, modes=modes)
+    W_hat, ic, w_hat = O.sino_train(R, 0, tx, rx, modes=modes)
+    assert abs(w_hat - w_lin) < 1e-10
+    assert abs(ic) < 1e-10
+    assert np.max(np.abs(W_hat - W)) < 1e-9
+    assert np.allclose(W_hat, W_hat.T)  # symmetric in (m, n)
+
+
+def test_wdmppe_delta_weights_closed_form():
+    c = 2
+    n = 30
+    A = crandn(rng(14), 6 * n).reshape(6, n)
+    C = np.zeros((2 * c + 1, 2 * c + 1), dtype=np.complex128)
+    C[c, c] = 0.5
+    spm = O.wdmppe_spm(c, C, A)
+    D = np.zeros_like(C)
+    D[c, c] = -0.25j
+    xpm = O.wdmppe_xpm(c, D, A)
+    k = np.arange(c, n - c)
+    p23 = np.abs(A[2]) ** 2 + np.abs(A[3]) ** 2
+    for pol in range(2):
+        assert np.allclose(spm[pol, k], 0.5 * A[2 + pol, k] * p23[k], rtol=1e-13)
+    a0, a1, a2, a3, a4, a5 = A
+    ref0 = (2 * a2 * abs(a0) ** 2 + a2 * abs(a1) ** 2 + a3 * a0 * np.conj(a1)
+            + 2 * a2 * abs(a4) ** 2 + a2 * abs(a5) ** 2 + a3 * a4 * np.conj(a5))
+    ref1 = (2 * a3 * abs(a1) ** 2 + a3 * abs(a0) ** 2 + a2 * a1 * np.conj(a0)
+            + 2 * a3 * abs(a5) ** 2 + a3 * abs(a4) ** 2 + a2 * a5 * np.conj(a4))
+    assert np.allclose(xpm[0, k], -0.25j * ref0[k], rtol=1e-12)
+    assert np.allclose(xpm[1, k], -0.25j * ref1[k], rtol=1e-12)
+
+
+# ---------------------------------------------------------------- learned DBP
+def test_ldbp_identity_and_gradients_fd():
+    steps, taps, n = 3, 5, 64
+    r = rng(15)
+    x = crandn(r, n) * 0.5
+    target = crandn(r, n) * 0.5
+    fir = np.zeros((steps, taps), dtype=np.complex128)
+    fir[:, taps // 2] = 1.0
+    theta = np.zeros(steps)
+    out, _ = O.ldbp_forward(fir, theta, x)
+    assert np.allclose(out, x, rtol=0, atol=1e-15)
+    fir = fir + 0.1 * crandn(r, steps * taps).reshape(steps, taps)
+    theta = 0.3 * r.standard_normal(steps)
+    loss, gf, gt = O.ldbp_backward(fir, theta, x, target)
+    eps = 1e-6
+    for (j, k) in [(0, 0), (1, 2), (2, 4)]:
+        for part in (1.0, 1j):
+            fp, fm = fir.copy(), fir.copy()
+            fp[j, k] += eps * part
+            fm[j, k] -= eps * part
+            d = (O.ldbp_forward(fp, theta, x, target)[1] - O.ldbp_forward(fm, theta, x, target)[1]) / (2 * eps)
+            ana = 2 * (gf[j, k].real if part == 1.0 else gf[j, k].imag)
+            assert d == pytest.approx(ana, rel=1e-6, abs=1e-9)
+    for j in range(steps):
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += eps
+        tm[j] -= eps
+        d = (O.ldbp_forward(fir, tp, x, target)[1] - O.ldbp_forward(fir, tm, x, target)[1]) / (2 * eps)
+        assert d == pytest.approx(gt[j], rel=1e-6, abs=1e-9)
