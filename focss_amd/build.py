@@ -30,20 +30,27 @@ def build_hip():
 
 
 def build_host():
-    out = os.path.join(HERE, "libfocss.so")
+    """libfocss.so from every host/*.cpp except the driver, then the
+    focss_tasks driver executable."""
     host = os.path.join(HERE, "host")
-    srcs = sorted(os.path.join(host, f) for f in os.listdir(host) if f.endswith(".cpp"))
-    if not srcs:
-        return None
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", host, "-I",
-           os.path.join(ROOT, "include"), "-o", out] + srcs + [
-           "-L", HERE, "-lfocss_hip", "-Wl,-rpath,$ORIGIN"]
-    _run(cmd)
-    return out
+    srcs = []
+    for d, _, files in os.walk(host):
+        srcs += [os.path.join(d, f) for f in files if f.endswith(".cpp") and f != "main.cpp"]
+    srcs.sort()
+    inc = ["-I", host, "-I", os.path.join(ROOT, "include")]
+    lib = os.path.join(HERE, "libfocss.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc + ["-o", lib] + srcs +
+         ["-L", HERE, "-lfocss_hip", "-Wl,-rpath,$ORIGIN"])
+    exe = os.path.join(HERE, "focss_tasks")
+    _run(["g++", "-O2", "-std=c++17", "-Wall"] + inc + ["-o", exe, os.path.join(host, "tasks", "main.cpp"),
+          "-L", HERE, "-lfocss", "-lfocss_hip", "-Wl,-rpath,$ORIGIN"])
+    return lib
 
 
 def main():
-    build_hip()
+    only_host = "--host" in sys.argv
+    if not only_host:
+        build_hip()
     build_host()
 
 

@@ -1,0 +1,89 @@
+// focss_tasks: command-line driver over the focss C++ API.
+//   focss_tasks verification <samples> <steps>
+//   focss_tasks dbp     <in> <out> <samples> <modes> <dt> <spans> <steps> <alpha> <beta2> <gamma> <length> <kappa>
+//   focss_tasks cdc     <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
+//   focss_tasks forward <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
+//                       <max_phase_shift> <max_step> <nf_db> <seed> <center_frequency>
+// <in>/<out> are raw little-endian complex<double> files in the field layout.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "focss/field.h"
+#include "focss/functions.h"
+#include "focss/runtime.h"
+#include "tasks/systems.h"
+
+double verification(const int& n_samples, const int& n_steps);
+
+namespace {
+focss::Field read_field(const char* path, int samples, int modes, double dt, double fc) {
+    focss::Field f(focss::Domain{modes, samples, dt, fc});
+    std::ifstream in(path, std::ios::binary);
+    in.read(reinterpret_cast<char*>(f.raw()), (std::streamsize)sizeof(focss::complex_t) * f.size());
+    if (!in) {
+        std::fprintf(stderr, "cannot read %s\n", path);
+        std::exit(2);
+    }
+    return f;
+}
+
+void write_field(const char* path, const focss::Field& f) {
+    std::ofstream out(path, std::ios::binary);
+    out.write(reinterpret_cast<const char*>(f.raw()), (std::streamsize)sizeof(focss::complex_t) * f.size());
+}
+
+focss::LinkParameters params(char** a) {
+    focss::LinkParameters p = focss::reference_link();
+    p.fiber.alpha = std::atof(a[0]);
+    p.fiber.beta2 = std::atof(a[1]);
+    p.fiber.gamma = std::atof(a[2]);
+    p.fiber.length = std::atof(a[3]);
+    p.fiber.kappa = std::atof(a[4]);
+    return p;
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: focss_tasks verification|dbp|cdc|forward ...\n");
+        return 2;
+    }
+    std::string cmd = argv[1];
+    if (cmd == "verification" && argc == 4) {
+        double e = verification(std::atoi(argv[2]), std::atoi(argv[3]));
+        return e < 1 ? 0 : 1;
+    }
+    if ((cmd == "dbp" && argc == 15) || (cmd == "cdc" && argc == 14) || (cmd == "forward" && argc == 19)) {
+        int samples = std::atoi(argv[4]), modes = std::atoi(argv[5]);
+        double dt = std::atof(argv[6]);
+        int spans = std::atoi(argv[7]);
+        double fc = cmd == "forward" ? std::atof(argv[18]) : 0.0;
+        focss::Field f = read_field(argv[2], samples, modes, dt, fc);
+        if (cmd == "dbp") {
+            focss::LinkParameters p = params(argv + 9);
+            p.spans = spans;
+            focss::backward_propagation(f, p, std::atoi(argv[8]));
+        } else if (cmd == "cdc") {
+            focss::LinkParameters p = params(argv + 8);
+            focss::cd_compensation(f, p, spans);
+        } else {
+            focss::LinkParameters p = params(argv + 8);
+            p.spans = spans;
+            p.max_phase_shift = std::atof(argv[13]);
+            p.max_step_size = std::atof(argv[14]);
+            p.noise_figure_db = std::atof(argv[15]);
+            focss::reseed_global_urng(std::strtoull(argv[16], nullptr, 0));
+            int steps = focss::forward_propagation(f, p);
+            std::cout << "split steps " << steps << std::endl;
+        }
+        write_field(argv[3], f);
+        return 0;
+    }
+    std::fprintf(stderr, "bad arguments\n");
+    return 2;
+}

@@ -80,3 +80,9 @@ def test_validate_rejects(case, msg):
         link = _link(F, dt=0.0)
     with pytest.raises(F.FocssError, match=msg):
         F.validate(link)
+
+
+def test_host_api_built():
+    """The C++ host library and its driver are built next to the HIP library."""
+    for name in ("libfocss.so", "focss_tasks"):
+        assert os.path.exists(os.path.join(ROOT, "focss_amd", name)), name

@@ -1,0 +1,82 @@
+"""The focss C++ API (focss::Field / SSFM / Amplifier, the systems.cpp
+drivers) over the C-ABI, driven through the focss_tasks executable and
+compared with the CPU oracle.  Covers the reference-side entry points
+backward_propagation, cd_compensation, forward_propagation and the
+verification() dark-soliton task."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from links import F_C, qam_waveform, rel_l2, ssmf
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "focss_amd", "focss_tasks")
+
+
+def run(args, tmp_path):
+    r = subprocess.run([EXE] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def fib_args(f):
+    return [repr(f["alpha"]), repr(f["beta2"]), repr(f["gamma"]), repr(f["length"]), repr(f["kappa"])]
+
+
+def test_verification_task(tmp_path):
+    out = run(["verification", 1 << 12, 200], tmp_path)
+    err = float(out.strip().splitlines()[-1])
+    # same recipe on the oracle
+    n, window = 1 << 12, 10.0
+    dt = window / n
+    t = (np.arange(n) - n // 2) * dt
+    y, _ = O.ssfm_run(O.fiber(0.0, 1.0, 1.0, 1.0, 0.0, 1.0), O.grid(O.UNIFORM, 200), O.FORWARD,
+                      np.tanh(t).astype(np.complex128), dt)
+    ref = np.exp(1j) * np.tanh(t)
+    e_ref = np.sqrt(np.mean(np.abs(y - ref) ** 2) / np.mean(np.abs(ref) ** 2))
+    assert err == pytest.approx(e_ref, rel=1e-9)
+
+
+@pytest.mark.parametrize("logn,modes,steps", [(12, 1, 3), (14, 1, 0), (11, 2, 2)])
+def test_backward_propagation(tmp_path, logn, modes, steps):
+    n = 1 << logn
+    f = ssmf(kappa=8 / 9 if modes == 2 else 1.0)
+    x, _, dt = qam_waveform(31 + logn, n, power_dbm=3.0, modes=modes)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.astype(np.complex128).tofile(src)
+    run(["dbp", src, dst, n, modes, repr(dt), 3, steps] + fib_args(f), tmp_path)
+    y = np.fromfile(dst, dtype=np.complex128)
+    gain = np.exp(f["alpha"] * f["length"])
+    ref = O.backward_propagation(O.fiber(**f), 3, steps, gain, x, dt, modes=modes)
+    assert rel_l2(y, ref) <= 1e-10
+
+
+def test_cd_compensation(tmp_path):
+    n = 1 << 13
+    f = ssmf()
+    x, _, dt = qam_waveform(5, n)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(src)
+    run(["cdc", src, dst, n, 1, repr(dt), 7] + fib_args(f), tmp_path)
+    y = np.fromfile(dst, dtype=np.complex128)
+    assert rel_l2(y, O.cd_compensation(O.fiber(**f), 7, x, dt)) <= 1e-10
+
+
+def test_forward_propagation_with_ase(tmp_path):
+    n = 1 << 12
+    f = ssmf()
+    x, _, dt = qam_waveform(9, n, power_dbm=2.0)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(src)
+    seed = 0x1234
+    out = run(["forward", src, dst, n, 1, repr(dt), 2] + fib_args(f) + [1e-2, 1e3, 5.0, seed, repr(F_C)], tmp_path)
+    y = np.fromfile(dst, dtype=np.complex128)
+    dev_seed = (seed ^ 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    gain = np.exp(f["alpha"] * f["length"])
+    ref, steps = O.forward_propagation(O.fiber(**f), 2, 1e-2, 1e3, gain, O.db_to_linear(5.0), F_C, dev_seed, x, dt)
+    assert f"split steps {int(steps.sum())}" in out
+    assert rel_l2(y, ref) <= 1e-10
