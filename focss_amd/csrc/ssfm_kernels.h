@@ -38,6 +38,15 @@ __device__ __forceinline__ cx<R> tw_big(const Tables<R>& t, unsigned e, int log_
     return t.bank[tw_offset(10) + (E >> 10)] * t.fine[E & 1023u];
 }
 
+// Logical block index grouping consecutive tiles on one XCD: the dispatcher
+// deals workgroups round-robin over the 8 XCDs, so block b runs on XCD b % 8;
+// tiles that share cache lines (adjacent columns) then share one L2.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    if (nb % 8) return b;
+    return (b % 8) * (nb / 8) + b / 8;
+}
+
 struct PassArgs {
     void* field;
     const WaveStatic* statics;
@@ -175,11 +184,16 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
 template <typename R, int N1, int C, int M>
 __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
     constexpr int S = C * M;
+    constexpr int TB = 32;  // inter-pass twiddle split k1 = hi * TB + lo
+    static_assert(N1 % TB == 0, "column length must be a multiple of 32");
     __shared__ cx<R> buf[padded(N1 * S)];
     __shared__ cx<R> tw_t[N1];
+    __shared__ cx<R> tw_lo[C][TB];       // W_N^{n2 lo}
+    __shared__ cx<R> tw_hi[C][N1 / TB];  // W_N^{n2 TB hi}
     __shared__ double redbuf[2 * (NT / 64)];
-    const int w_id = a.waves[blockIdx.x / a.tiles_per_wave];
-    const int tile = blockIdx.x % a.tiles_per_wave;
+    const int blk = xcd_block();
+    const int w_id = a.waves[blk / a.tiles_per_wave];
+    const int tile = blk % a.tiles_per_wave;
     const WaveState st = a.states[(a.counter & 1) * a.n_waves + w_id];
     if (st.top == TOP_NONE) return;
     const WaveStatic w = a.statics[w_id];
@@ -191,11 +205,20 @@ __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
     const bool forward = st.top != TOP_EXIT;
     load_tw_full<R, N1, NT>(tw_t, tb.bank);
     const TwFull<R> tw{tw_t};
+    for (int i = threadIdx.x; i < C * TB; i += NT) {
+        int c = i / TB, l = i % TB;
+        tw_lo[c][l] = tw_big<R>(tb, (unsigned)((n2_0 + c) * l) & (unsigned)(N - 1), a.log_n);
+    }
+    for (int i = threadIdx.x; i < C * (N1 / TB); i += NT) {
+        int c = i / (N1 / TB), h = i % (N1 / TB);
+        tw_hi[c][h] = tw_big<R>(tb, (unsigned)((n2_0 + c) * (TB * h)) & (unsigned)(N - 1), a.log_n);
+    }
+    __syncthreads();
     for (int i = threadIdx.x; i < N1 * S; i += NT) {
         int n1 = i / S, s = i % S;
         int m = s / C, c = s % C;
         cx<R> v = g[(long long)m * N + (long long)n1 * N2 + n2_0 + c];
-        if (inverse) v = v * conj(tw_big<R>(tb, (unsigned)((n2_0 + c) * n1) & (unsigned)(N - 1), a.log_n));
+        if (inverse) v = v * conj(tw_hi[c][n1 / TB] * tw_lo[c][n1 % TB]);
         buf[pad(n1 * S + s)] = v;
     }
     __syncthreads();
@@ -221,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
         int n1 = i / S, s = i % S;
         int m = s / C, c = s % C;
         cx<R> v = buf[pad(n1 * S + s)];
-        if (forward) v = v * tw_big<R>(tb, (unsigned)((n2_0 + c) * n1) & (unsigned)(N - 1), a.log_n);
+        if (forward) v = v * (tw_hi[c][n1 / TB] * tw_lo[c][n1 % TB]);
         g[(long long)m * N + (long long)n1 * N2 + n2_0 + c] = v;
     }
 }
@@ -231,8 +254,9 @@ template <typename R, int N2, int S>
 __global__ __launch_bounds__(NT) void k_spec_pass(PassArgs a, Tables<R> tb) {
     __shared__ cx<R> buf[padded(N2 * S)];
     __shared__ cx<R> tw_t[N2];
-    const int w_id = a.waves[blockIdx.x / a.tiles_per_wave];
-    const int tile = blockIdx.x % a.tiles_per_wave;
+    const int blk = xcd_block();
+    const int w_id = a.waves[blk / a.tiles_per_wave];
+    const int tile = blk % a.tiles_per_wave;
     WaveState st = a.states[(a.counter & 1) * a.n_waves + w_id];
     const int was_done = st.phase == PH_DONE;
     const WaveStatic w = a.statics[w_id];

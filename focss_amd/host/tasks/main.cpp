@@ -56,13 +56,15 @@ int main(int argc, char** argv) {
     std::string cmd = argv[1];
     if (cmd == "verification" && argc == 4) {
         double e = verification(std::atoi(argv[2]), std::atoi(argv[3]));
+        std::cout.precision(17);
+        std::cout << e << std::endl;
         return e < 1 ? 0 : 1;
     }
-    if ((cmd == "dbp" && argc == 15) || (cmd == "cdc" && argc == 14) || (cmd == "forward" && argc == 19)) {
+    if ((cmd == "dbp" && argc == 14) || (cmd == "cdc" && argc == 13) || (cmd == "forward" && argc == 18)) {
         int samples = std::atoi(argv[4]), modes = std::atoi(argv[5]);
         double dt = std::atof(argv[6]);
         int spans = std::atoi(argv[7]);
-        double fc = cmd == "forward" ? std::atof(argv[18]) : 0.0;
+        double fc = cmd == "forward" ? std::atof(argv[17]) : 0.0;
         focss::Field f = read_field(argv[2], samples, modes, dt, fc);
         if (cmd == "dbp") {
             focss::LinkParameters p = params(argv + 9);
