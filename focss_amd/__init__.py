@@ -66,6 +66,9 @@ _sig = {
     "focss_ctx_stream": (_P, [_P]),
     "focss_link_validate": (C.c_int, [C.POINTER(Link)]),
     "focss_add_noise": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint32, _P]),
+    "focss_sino_cubic": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P]),
+    "focss_sino_train": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "focss_wdm_ppe": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -179,6 +182,27 @@ class Context:
 
     def fft(self, buf, modes, samples, sign, precision=F64):
         _check(_lib.focss_fft(self.handle, precision, sign, modes, samples, buf.ptr))
+
+    def sino_cubic(self, weights, radius, inp, out, modes, samples, precision=F64):
+        w = np.ascontiguousarray(weights, dtype=np.complex128)
+        _check(_lib.focss_sino_cubic(self.handle, precision, radius, w.ctypes.data, modes, samples,
+                                     inp.ptr, out.ptr))
+
+    def sino_train(self, radius, type_, tx, rx, modes, samples, linear_weight=0j):
+        """tx, rx: complex128 device buffers.  Returns (W, intercept, w_lin)."""
+        lw = np.array([linear_weight], dtype=np.complex128)
+        w = np.zeros((2 * radius + 1, 2 * radius + 1), dtype=np.complex128)
+        ic = np.zeros(1, dtype=np.complex128)
+        wl = np.zeros(1, dtype=np.complex128)
+        _check(_lib.focss_sino_train(self.handle, radius, type_, lw.ctypes.data, modes, samples, tx.ptr,
+                                     rx.ptr, w.ctypes.data, ic.ctypes.data, wl.ctypes.data))
+        return w, ic[0], wl[0]
+
+    def wdm_ppe(self, c_radius, Cm, d_radius, Dm, a6, spm, xpm, samples, precision=F64):
+        cm = np.ascontiguousarray(Cm, dtype=np.complex128)
+        dm = np.ascontiguousarray(Dm, dtype=np.complex128)
+        _check(_lib.focss_wdm_ppe(self.handle, precision, c_radius, cm.ctypes.data, d_radius, dm.ctypes.data,
+                                  samples, a6.ptr, spm.ptr, xpm.ptr))
 
     def add_noise(self, buf, modes, samples, sigma, seed, stream, precision=F64):
         _check(_lib.focss_add_noise(self.handle, precision, modes, samples, sigma, seed, stream, buf.ptr))

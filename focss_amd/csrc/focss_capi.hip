@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -13,6 +14,7 @@
 #include "../../include/focss.h"
 #include "ssfm_kernels.h"
 #include "fft_kernels.h"
+#include "triplet_kernels.h"
 
 using namespace focss_hip;
 
@@ -621,6 +623,83 @@ int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
 
 }  // namespace
 
+namespace {
+
+// pack a (2R+1)^2 row-major weight matrix W[(R+m)(2R+1) + R+n] into diamond
+// order; m_outer selects (m outer, n inner) or (n outer, m inner)
+std::vector<cx<double>> diamond_pack(const double* w, int radius, bool m_outer) {
+    std::vector<cx<double>> d;
+    const int side = 2 * radius + 1;
+    for (int a = -radius; a <= radius; ++a) {
+        int b = radius - (a < 0 ? -a : a);
+        for (int c = -b; c <= b; ++c) {
+            int m = m_outer ? a : c, n = m_outer ? c : a;
+            const double* e = w + 2 * ((radius + m) * side + (radius + n));
+            d.push_back(mk<double>(e[0], e[1]));
+        }
+    }
+    return d;
+}
+
+template <typename R>
+int upload_weights(focss_ctx* c, const std::vector<cx<double>>& d, void** dev) {
+    std::vector<cx<R>> h(d.size());
+    for (size_t i = 0; i < d.size(); ++i) h[i] = mk<R>((R)d[i].x, (R)d[i].y);
+    HIP_TRY(hipMalloc(dev, std::max<size_t>(1, h.size()) * sizeof(cx<R>)));
+    HIP_TRY(hipMemcpyAsync(*dev, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+template <typename R>
+int sino_cubic_impl(focss_ctx* c, int radius, const double* weights, int modes, int samples, const void* in,
+                    void* out) {
+    std::vector<cx<double>> d = diamond_pack(weights, radius, true);
+    void* w = nullptr;
+    if (int rc = upload_weights<R>(c, d, &w)) return rc;
+    int blocks = (samples + TRIP_NT - 1) / TRIP_NT;
+    size_t lds = (size_t)modes * (TRIP_NT + 2 * radius) * sizeof(cx<R>);
+    if (lds > 64 * 1024) {
+        (void)hipFree(w);
+        return fail(FOCSS_E_UNSUPPORTED, "sino_cubic: radius too large for the LDS window");
+    }
+    hipLaunchKernelGGL(k_sino_cubic<R>, dim3(blocks), dim3(TRIP_NT), lds, c->stream, (const cx<R>*)in, (cx<R>*)out,
+                       (const cx<R>*)w, radius, modes, samples);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(w);
+    if (e != hipSuccess) return fail(FOCSS_E_HIP, std::string("sino_cubic: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// Hermitian positive-definite solve A x = b by Cholesky A = L L^H
+int cholesky_solve(std::vector<cx<double>>& A, int n, std::vector<cx<double>>& b) {
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j].x;
+        for (int k = 0; k < j; ++k) d -= norm2(A[(size_t)j * n + k]);
+        if (!(d > 0)) return fail(FOCSS_E_NUMERIC, "sino_train: normal matrix not positive definite");
+        double l = std::sqrt(d);
+        A[(size_t)j * n + j] = mk<double>(l, 0);
+        for (int i = j + 1; i < n; ++i) {
+            cx<double> v = A[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v = v - A[(size_t)i * n + k] * conj(A[(size_t)j * n + k]);
+            A[(size_t)i * n + j] = v * (1.0 / l);
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        cx<double> v = b[i];
+        for (int k = 0; k < i; ++k) v = v - A[(size_t)i * n + k] * b[k];
+        b[i] = v * (1.0 / A[(size_t)i * n + i].x);
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        cx<double> v = b[i];
+        for (int k = i + 1; k < n; ++k) v = v - conj(A[(size_t)k * n + i]) * b[k];
+        b[i] = v * (1.0 / A[(size_t)i * n + i].x);
+    }
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* focss_last_error(void) { return g_err.c_str(); }
@@ -822,6 +901,153 @@ int focss_add_noise(focss_ctx* c, int precision, int modes, int samples, double 
         return fail(FOCSS_E_ARG, "bad precision");
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int focss_sino_cubic(focss_ctx* c, int precision, int radius, const double* weights, int modes, int samples,
+                     const void* in, void* out) {
+    if (!c || !weights || !in || !out || radius < 0 || modes < 1 || samples <= 2 * radius)
+        return fail(FOCSS_E_ARG, "sino_cubic: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == FOCSS_F64) return sino_cubic_impl<double>(c, radius, weights, modes, samples, in, out);
+    if (precision == FOCSS_F32) return sino_cubic_impl<float>(c, radius, weights, modes, samples, in, out);
+    return fail(FOCSS_E_ARG, "bad precision");
+}
+
+int focss_sino_train(focss_ctx* c, int radius, int type, const double* linear_weight_in, int modes, int samples,
+                     const void* tx, const void* rx, double* weights, double* intercept, double* linear_weight) {
+    if (!c || !tx || !rx || !weights || !intercept || !linear_weight || radius < 0 || type < 0 || type > 2 ||
+        modes < 1 || samples <= 2 * radius)
+        return fail(FOCSS_E_ARG, "sino_train: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const int nf = radius * (radius + 2) / 2 + 1 + (type == 0 ? 1 : 0);
+    const long long rows = (long long)modes * (samples - 2 * radius);
+    if (rows < nf) return fail(FOCSS_E_ARG, "sino_train: fewer rows than features");
+    cx<double>* F = nullptr;
+    cx<double>* y = nullptr;
+    cx<double>* G = nullptr;
+    cx<double>* sums = nullptr;
+    cx<double>* fy = nullptr;
+    HIP_TRY(hipMalloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
+    HIP_TRY(hipMalloc(&y, (size_t)rows * sizeof(cx<double>)));
+    HIP_TRY(hipMalloc(&G, (size_t)nf * nf * sizeof(cx<double>)));
+    HIP_TRY(hipMalloc(&sums, (size_t)(nf + 1) * sizeof(cx<double>)));
+    HIP_TRY(hipMalloc(&fy, (size_t)nf * sizeof(cx<double>)));
+    cx<double> lw = mk<double>(linear_weight_in ? linear_weight_in[0] : 0.0, linear_weight_in ? linear_weight_in[1] : 0.0);
+    int fblocks = (int)std::min<long long>((rows + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sino_features, dim3(fblocks), dim3(256), 0, c->stream, (const cx<double>*)tx,
+                       (const cx<double>*)rx, F, y, radius, type, lw, modes, samples);
+    dim3 gg((nf + 31) / 32, (nf + 31) / 32);
+    hipLaunchKernelGGL(k_gram, gg, dim3(256), 0, c->stream, (const cx<double>*)F, G, rows, nf);
+    hipLaunchKernelGGL(k_col_sums, dim3(nf + 1), dim3(256), 0, c->stream, (const cx<double>*)F, (const cx<double>*)y,
+                       rows, nf, sums, fy);
+    HIP_TRY(hipGetLastError());
+    std::vector<cx<double>> hG((size_t)nf * nf), hs(nf + 1), hfy(nf);
+    HIP_TRY(hipMemcpyAsync(hG.data(), G, hG.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hs.data(), sums, hs.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hfy.data(), fy, hfy.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(F);
+    (void)hipFree(y);
+    (void)hipFree(G);
+    (void)hipFree(sums);
+    (void)hipFree(fy);
+    // z-score (std with 1/rows), centred responses, normal equations
+    // (sino.cpp:105-114 with the least squares as (F^H F) x = F^H y)
+    const double inv = 1.0 / (double)rows;
+    std::vector<cx<double>> mu(nf);
+    std::vector<double> sd(nf);
+    for (int j = 0; j < nf; ++j) {
+        mu[j] = hs[j] * inv;
+        double var = hG[(size_t)j * nf + j].x * inv - norm2(mu[j]);
+        sd[j] = std::sqrt(var > 0 ? var : 0.0);
+    }
+    const cx<double> ymean = hs[nf] * inv;
+    std::vector<cx<double>> A((size_t)nf * nf), b(nf);
+    for (int i = 0; i < nf; ++i) {
+        for (int j = 0; j < nf; ++j) {
+            cx<double> v = hG[(size_t)i * nf + j] - conj(mu[i]) * mu[j] * (double)rows;
+            A[(size_t)i * nf + j] = v * (1.0 / (sd[i] * sd[j]));
+        }
+        b[i] = (hfy[i] - conj(mu[i]) * ymean * (double)rows) * (1.0 / sd[i]);
+    }
+    if (int rc = cholesky_solve(A, nf, b)) return rc;
+    // de-standardise, intercept, unfold                  sino.cpp:122-153
+    cx<double> ic = ymean;
+    for (int j = 0; j < nf; ++j) {
+        b[j] = b[j] * (1.0 / sd[j]);
+        ic = ic - b[j] * mu[j];
+    }
+    intercept[0] = ic.x;
+    intercept[1] = ic.y;
+    const int side = 2 * radius + 1;
+    for (int i = 0; i < 2 * side * side; ++i) weights[i] = 0;
+    auto put = [&](int m, int n, cx<double> v) {
+        double* e = weights + 2 * ((size_t)(radius + m) * side + (radius + n));
+        e[0] = v.x;
+        e[1] = v.y;
+    };
+    int index = 0;
+    const int mb = radius / 2;
+    for (int m = -mb; m <= mb; ++m) {
+        const int am = m < 0 ? -m : m;
+        for (int n = am; n <= radius - am; ++n) {
+            cx<double> v = b[index++];
+            put(m, n, v);
+            if (m == 0 && n == 0) continue;
+            if (m == n) {
+                put(-m, -n, v);
+            } else if (m == -n) {
+                put(n, m, v);
+            } else {
+                put(n, m, v);
+                put(-m, -n, v);
+                put(-n, -m, v);
+            }
+        }
+    }
+    if (type == 0) {
+        linear_weight[0] = b[index].x;
+        linear_weight[1] = b[index].y;
+    } else if (type == 2) {
+        linear_weight[0] = 0;
+        linear_weight[1] = 0;
+    } else {
+        linear_weight[0] = lw.x;
+        linear_weight[1] = lw.y;
+    }
+    return 0;
+}
+
+int focss_wdm_ppe(focss_ctx* c, int precision, int c_radius, const double* C, int d_radius, const double* D,
+                  int samples, const void* a6, void* spm, void* xpm) {
+    if (!c || !a6 || !spm || !xpm || samples < 1 || (c_radius >= 0 && !C) || (d_radius >= 0 && !D))
+        return fail(FOCSS_E_ARG, "wdm_ppe: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<cx<double>> cd = c_radius >= 0 ? diamond_pack(C, c_radius, false) : std::vector<cx<double>>();
+    std::vector<cx<double>> dd = d_radius >= 0 ? diamond_pack(D, d_radius, true) : std::vector<cx<double>>();
+    void *cw = nullptr, *dw = nullptr;
+    int blocks = (samples + TRIP_NT - 1) / TRIP_NT;
+    if (precision == FOCSS_F64) {
+        if (int rc = upload_weights<double>(c, cd, &cw)) return rc;
+        if (int rc = upload_weights<double>(c, dd, &dw)) return rc;
+        hipLaunchKernelGGL(k_wdm_ppe<double>, dim3(blocks), dim3(TRIP_NT), 0, c->stream, (const cx<double>*)a6,
+                           (const cx<double>*)cw, c_radius, (const cx<double>*)dw, d_radius, samples, (cx<double>*)spm,
+                           (cx<double>*)xpm);
+    } else if (precision == FOCSS_F32) {
+        if (int rc = upload_weights<float>(c, cd, &cw)) return rc;
+        if (int rc = upload_weights<float>(c, dd, &dw)) return rc;
+        hipLaunchKernelGGL(k_wdm_ppe<float>, dim3(blocks), dim3(TRIP_NT), 0, c->stream, (const cx<float>*)a6,
+                           (const cx<float>*)cw, c_radius, (const cx<float>*)dw, d_radius, samples, (cx<float>*)spm,
+                           (cx<float>*)xpm);
+    } else {
+        return fail(FOCSS_E_ARG, "bad precision");
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(cw);
+    (void)hipFree(dw);
+    if (e != hipSuccess) return fail(FOCSS_E_HIP, std::string("wdm_ppe: ") + hipGetErrorString(e));
     return 0;
 }
 

@@ -144,6 +144,34 @@ int focss_fft(focss_ctx* ctx, int precision, int sign, int modes, int samples, v
 int focss_add_noise(focss_ctx* ctx, int precision, int modes, int samples, double sigma,
                     uint64_t seed, uint32_t stream, void* data);
 
+/* ---- perturbation-triplet equalizer ----
+ * Weights W are (2R+1)^2 complex<double> (interleaved re, im), row-major:
+ * W(m, n) at index (R + m) * (2R + 1) + (R + n).  Fields are device
+ * buffers in the mode-major layout. */
+
+/* SINO::cubic (src/focss/equalizer/sino.cpp:203-220): out(mode, k) =
+ * sum_{|m|+|n|<=R} W(m,n) A_mode(k+m) sum_p A_p(k+n) conj(A_p(k+m+n)) for
+ * R <= k < samples - R, 0 elsewhere. */
+int focss_sino_cubic(focss_ctx* ctx, int precision, int radius, const double* weights, int modes,
+                     int samples, const void* in, void* out);
+
+/* SINO::train (src/focss/equalizer/sino.cpp:42-159), type 0 ORDINARY,
+ * 1 FIXED (uses linear_weight_in), 2 HOMOGENEOUS.  tx, rx: complex<double>
+ * device fields.  Folded symmetric features and the Gram matrix F^H F are
+ * built on the device; the z-scored normal equations are solved by
+ * Cholesky on the host.  Outputs (host): weights (2R+1)^2, intercept (2
+ * doubles), linear weight (2 doubles). */
+int focss_sino_train(focss_ctx* ctx, int radius, int type, const double* linear_weight_in, int modes,
+                     int samples, const void* tx, const void* rx, double* weights, double* intercept,
+                     double* linear_weight);
+
+/* WdmPPE::spm and WdmPPE::xpm (src/focss/equalizer/wdm_sino.cpp:256-295) over
+ * six envelopes a6 = [6][samples] (row = channel * 2 + polarisation, rows
+ * 2..3 the channel of interest).  C is (2c+1)^2, D is (2d+1)^2; a negative
+ * radius skips that term.  spm, xpm: [2][samples] device outputs. */
+int focss_wdm_ppe(focss_ctx* ctx, int precision, int c_radius, const double* C, int d_radius,
+                  const double* D, int samples, const void* a6, void* spm, void* xpm);
+
 /* Tuning: choose the tile shape of the two-pass kernels for one length
  * (0 = default).  Exposed for the autotuner. */
 int focss_set_tile(focss_ctx* ctx, int precision, int log2_samples, int variant);
