@@ -2,6 +2,9 @@
 //   focss_tasks verification <samples> <steps>
 //   focss_tasks dbp     <in> <out> <samples> <modes> <dt> <spans> <steps> <alpha> <beta2> <gamma> <length> <kappa>
 //   focss_tasks cdc     <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
+//   focss_tasks sino    <tx> <rx> <out> <samples> <modes> <radius> <type> <lw_re> <lw_im>
+//                       (out: weights (2R+1)^2, intercept, linear weight, then
+//                        the equalized field (rx - cubic(tx)) / w, complex<double>)
 //   focss_tasks forward <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
 //                       <max_phase_shift> <max_step> <nf_db> <seed> <center_frequency>
 // <in>/<out> are raw little-endian complex<double> files in the field layout.
@@ -13,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "focss/equalizer/sino.h"
 #include "focss/field.h"
 #include "focss/functions.h"
 #include "focss/runtime.h"
@@ -84,6 +88,23 @@ int main(int argc, char** argv) {
             std::cout << "split steps " << steps << std::endl;
         }
         write_field(argv[3], f);
+        return 0;
+    }
+    if (cmd == "sino" && argc == 11) {
+        int samples = std::atoi(argv[5]), modes = std::atoi(argv[6]), radius = std::atoi(argv[7]);
+        focss::Field tx = read_field(argv[2], samples, modes, 1e-11, 0.0);
+        focss::Field rx = read_field(argv[3], samples, modes, 1e-11, 0.0);
+        focss::SINO sino(radius, 1.0, (focss::SinoType)std::atoi(argv[8]));
+        sino.set_linear_weight(focss::complex_t(std::atof(argv[9]), std::atof(argv[10])));
+        sino.train(tx, rx);
+        std::ofstream out(argv[4], std::ios::binary);
+        out.write(reinterpret_cast<const char*>(sino.weights().data()),
+                  (std::streamsize)(sino.weights().size() * sizeof(focss::complex_t)));
+        focss::complex_t ic = sino.get_intercept(), lw = sino.get_linear_weight();
+        out.write(reinterpret_cast<const char*>(&ic), sizeof(ic));
+        out.write(reinterpret_cast<const char*>(&lw), sizeof(lw));
+        focss::Field eq = sino.equalize(rx, tx);
+        out.write(reinterpret_cast<const char*>(eq.raw()), (std::streamsize)(eq.size() * sizeof(focss::complex_t)));
         return 0;
     }
     std::fprintf(stderr, "bad arguments\n");

@@ -80,3 +80,26 @@ def test_forward_propagation_with_ase(tmp_path):
     ref, steps = O.forward_propagation(O.fiber(**f), 2, 1e-2, 1e3, gain, O.db_to_linear(5.0), F_C, dev_seed, x, dt)
     assert f"split steps {int(steps.sum())}" in out
     assert rel_l2(y, ref) <= 1e-10
+
+
+def test_sino_class(tmp_path):
+    """focss::SINO train + equalize against the oracle's train + cubic."""
+    radius, n, modes = 5, 1200, 1
+    r = np.random.default_rng(41)
+    tx = (r.standard_normal(n) + 1j * r.standard_normal(n)) * 0.5
+    W = (r.standard_normal((2 * radius + 1) ** 2) + 1j * r.standard_normal((2 * radius + 1) ** 2)) * 0.05
+    W = W.reshape(2 * radius + 1, 2 * radius + 1)
+    W = (W + W.T) / 2
+    rx = 0.97 * tx + O.sino_cubic(W.ravel(), radius, tx) + 0.01 * (r.standard_normal(n) + 1j * r.standard_normal(n))
+    ftx, frx, fout = tmp_path / "tx.bin", tmp_path / "rx.bin", tmp_path / "w.bin"
+    tx.tofile(ftx)
+    rx.tofile(frx)
+    run(["sino", ftx, frx, fout, n, modes, radius, 0, 0.0, 0.0], tmp_path)
+    out = np.fromfile(fout, dtype=np.complex128)
+    side = 2 * radius + 1
+    Wg, icg, lwg, eq = out[:side * side].reshape(side, side), out[side * side], out[side * side + 1], out[side * side + 2:]
+    Wo, ico, lwo = O.sino_train(radius, 0, tx, rx)
+    assert np.max(np.abs(Wg - Wo)) < 1e-8
+    assert abs(lwg - lwo) < 1e-9 and abs(icg - ico) < 1e-9
+    eq_ref = (rx - O.sino_cubic(Wo.ravel(), radius, tx)) / lwo
+    assert np.linalg.norm(eq - eq_ref) / np.linalg.norm(eq_ref) < 1e-9
