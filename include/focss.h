@@ -172,6 +172,30 @@ int focss_sino_train(focss_ctx* ctx, int radius, int type, const double* linear_
 int focss_wdm_ppe(focss_ctx* ctx, int precision, int c_radius, const double* C, int d_radius,
                   const double* D, int samples, const void* a6, void* spm, void* xpm);
 
+/* ---- learned few-step DBP (no counterpart in the reference) ----
+ * Step j: y = complex FIR (taps, circular, centre taps/2) of x, then
+ * z = y exp(i theta_j |y|^2).  params (device, float): the FIR taps
+ * [steps][taps] as interleaved complex64, followed by theta[steps].
+ * Fields are complex64 device buffers [batch][samples], samples a power of
+ * two <= 16384, taps 5 or 15.  Loss = sum_b (1/(samples batch)) sum_t
+ * |z - target|^2.  Judged against plain DBP (focss_link_run). */
+size_t focss_ldbp_workspace_bytes(int steps, int samples, int batch);
+int focss_ldbp_forward(focss_ctx* ctx, int steps, int taps, int samples, int batch, const float* params,
+                       const void* in, void* out, const void* target, double* loss);
+/* Accumulates into grad (device, double [2 steps taps + steps], zeroed by the
+ * caller) the Wirtinger gradient dL/d(conj h) (re, im) and dL/dtheta, and the
+ * loss into *loss (device double).  Stream-ordered, no host sync. */
+int focss_ldbp_gradients(focss_ctx* ctx, int steps, int taps, int samples, int batch, const float* params,
+                         const void* in, const void* target, void* workspace, size_t workspace_bytes,
+                         double* grad, double* loss);
+/* One training step: gradients then Adam (t = step number from 1, optional
+ * per-parameter learning-rate scale).  *loss_host receives the batch loss
+ * before the update. */
+int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int samples, int batch, float* params,
+                          float* adam_m, float* adam_v, int t, float lr, const float* lr_scale,
+                          const void* in, const void* target, void* workspace, size_t workspace_bytes,
+                          double* loss_host);
+
 /* Tuning: choose the tile shape of the two-pass kernels for one length
  * (0 = default).  Exposed for the autotuner. */
 int focss_set_tile(focss_ctx* ctx, int precision, int log2_samples, int variant);
