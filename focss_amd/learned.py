@@ -1,0 +1,225 @@
+"""Learned few-step DBP experiment on one GPU.
+
+    python -m focss_amd.learned [--train 128] [--test 64] [--iters 300] ...
+
+1. Training and test waveforms come from the project's own simulator: seeded
+   16QAM at 32 GBd, RRC 0.1, simulated at 4 samples/symbol over 20 x 80 km
+   SSMF (ADAPTIVE grid, EDFA 16 dB + ASE NF 5 dB, launch 0..4 dBm) by the
+   device link program, then decimated to 2 samples/symbol (2^14 samples).
+2. Plain DBP at 16 steps per span (FP64, device link program) is the judge.
+3. Learned DBP: 2 steps per span = 40 x (15-tap complex FIR, Kerr phase
+   theta), initialised from the physics (each FIR a least-squares fit of
+   the half-span's loss + dispersion inverse over the signal band, theta =
+   -kappa gamma L_eff), trained with Adam on the waveform MSE against the
+   transmitted 2 sps waveform.
+4. Q = q2_factor after RRC matched filter and constant-phase alignment; the
+   report gives Q per method and waveforms/s of each backpropagation.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+LIGHT_SPEED = 299792458.0
+PLANCK = 6.62607004081818181818181e-34
+WAVELENGTH = 1.55e-6
+F_C = LIGHT_SPEED / WAVELENGTH
+BAUD, ROLL_OFF = 32e9, 0.1
+SPAN_LEN, N_SPANS = 80e3, 20
+ALPHA = 0.2e-3 * np.log(10) / 10
+BETA2 = -21.7e-27
+GAMMA = 1.3e-3
+NF_DB = 5.0
+
+
+def fiber(F, length=SPAN_LEN):
+    return F.Fiber(ALPHA, BETA2, GAMMA, length, WAVELENGTH, 1.0)
+
+
+def make_dataset(F, ctx, n_waves, n_samples, seed, p_lo=0.0, p_hi=4.0, sim_sps=4):
+    """Returns (rx at 2 sps [n][n_samples], tx waveform at 2 sps, symbols,
+    launch powers).  The forward link runs on the device in FP64."""
+    from .transmitter import qam16, rrc_shaping, decimate
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_sym = n_samples // 2
+    n_sim = n_sym * sim_sps
+    dt_sim = 1.0 / (BAUD * sim_sps)
+    powers = rng.uniform(p_lo, p_hi, n_waves)
+    syms, tx2, tx_sim = [], [], []
+    for p in powers:
+        s, _ = qam16(rng, n_sym)
+        s = s * np.sqrt(1e-3 * 10 ** (p / 10))
+        syms.append(s)
+        tx_sim.append(rrc_shaping(s, sim_sps, ROLL_OFF))
+        tx2.append(rrc_shaping(s, 2, ROLL_OFF))
+    gain = np.exp(ALPHA * SPAN_LEN)
+    nf = 10 ** (NF_DB / 10)
+    sigma = np.sqrt(PLANCK * F_C * nf * (gain - 1) / 2 / dt_sim / 2)
+    links = [F.make_link(fiber(F), F.Grid(F.ADAPTIVE, 0, 1e-3, 1e3), F.FORWARD, N_SPANS, n_sim, dt_sim,
+                         center_frequency=F_C, post_scale=np.sqrt(gain), noise_sigma=sigma,
+                         noise_seed=seed * 1000 + i) for i in range(n_waves)]
+    host = np.concatenate(tx_sim)
+    buf = ctx.alloc(host.nbytes)
+    buf.upload(host)
+    ctx.link_run(links, buf, np.arange(n_waves, dtype=np.int64) * n_sim, precision=F.F64)
+    rx_sim = buf.download(np.empty_like(host)).reshape(n_waves, n_sim)
+    buf.free()
+    rx2 = np.stack([decimate(r, sim_sps // 2) for r in rx_sim])
+    return rx2, np.stack(tx2), np.stack(syms), powers
+
+
+def plain_dbp(F, ctx, rx2, steps_per_span, precision=None):
+    """backward_propagation on the device for a batch; returns (out, seconds)."""
+    precision = F.F64 if precision is None else precision
+    n_waves, n = rx2.shape
+    dt = 1.0 / (BAUD * 2)
+    gain = np.exp(ALPHA * SPAN_LEN)
+    links = [F.make_link(fiber(F), F.Grid(F.UNIFORM, steps_per_span, 0.0, 0.0), F.BACKWARD, N_SPANS, n, dt,
+                         center_frequency=F_C, pre_scale=1 / np.sqrt(gain)) for _ in range(n_waves)]
+    host = rx2.astype(np.complex128 if precision == F.F64 else np.complex64).ravel()
+    buf = ctx.alloc(host.nbytes)
+    buf.upload(host)
+    offs = np.arange(n_waves, dtype=np.int64) * n
+    ctx.link_run(links, buf, offs, precision=precision)  # warm-up (graph capture, tuning)
+    buf.upload(host)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.link_run(links, buf, offs, precision=precision)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    out = buf.download(np.empty_like(host)).reshape(n_waves, n).astype(np.complex128)
+    buf.free()
+    return out, t
+
+
+def init_params(n, steps_per_span=2, taps=15):
+    """Physics initialisation of the learned model: FIR j is the least-squares
+    15-tap fit, over the signal band, of exp(alpha h/2) exp(-i beta2 h w^2/2)
+    (the BACKWARD linear step over h = L / steps_per_span, times the span's
+    drop_power on the first step of a span); theta_j = -kappa gamma L_eff(h)."""
+    h = SPAN_LEN / steps_per_span
+    dt = 1.0 / (BAUD * 2)
+    k = np.fft.fftfreq(n, d=dt) * 2 * np.pi
+    band = np.abs(k) <= 2 * np.pi * BAUD * (1 + ROLL_OFF) / 2 * 1.05
+    w = k[band]
+    c = taps // 2
+    A = np.exp(-1j * np.outer(w, (np.arange(taps) - c) * dt))
+    gain = np.exp(ALPHA * SPAN_LEN)
+    firs, thetas = [], []
+    l_eff = (1 - np.exp(-ALPHA * h)) / ALPHA
+    for span in range(N_SPANS):
+        for j in range(steps_per_span):
+            Hd = np.exp(ALPHA * h / 2) * np.exp(1j * BETA2 / 2 * (-h) * w ** 2)
+            if j == 0:
+                Hd = Hd / np.sqrt(gain)
+            coef, *_ = np.linalg.lstsq(A, Hd, rcond=None)
+            firs.append(coef)
+            thetas.append(-GAMMA * l_eff)
+    return np.array(firs), np.array(thetas)
+
+
+def q_of(out2, syms):
+    from .transmitter import phase_align, q2_factor, rrc_sampling
+    qs = []
+    for o, s in zip(out2, syms):
+        r = rrc_sampling(o, 2, ROLL_OFF)
+        qs.append(q2_factor(s, phase_align(s, r), cut=64))
+    return np.array(qs)
+
+
+def run(args, log=print):
+    sys.path.insert(0, ROOT)
+    import focss_amd as F
+    n = args.samples
+    steps = 2 * N_SPANS
+    taps = 15
+    with F.Context(0) as ctx:
+        t0 = time.perf_counter()
+        rx_tr, tx_tr, _, _ = make_dataset(F, ctx, args.train, n, seed=args.seed)
+        rx_te, tx_te, sy_te, p_te = make_dataset(F, ctx, args.test, n, seed=args.seed + 1)
+        log(f"datasets: {args.train} + {args.test} waveforms of {n} samples, {time.perf_counter() - t0:.1f} s")
+        # judge: plain DBP at 16 and 2 steps per span (FP64)
+        dbp16, t16 = plain_dbp(F, ctx, rx_te, 16)
+        dbp2, t2 = plain_dbp(F, ctx, rx_te, 2)
+        q16, q2 = q_of(dbp16, sy_te), q_of(dbp2, sy_te)
+        # learned model
+        fir0, th0 = init_params(n, 2, taps)
+        batch = min(args.batch, args.train)
+        model = F.LearnedDBP(ctx, steps, taps, n, batch, fir0, th0)
+        scale = np.concatenate([np.full(2 * steps * taps, 1.0), np.abs(th0)]).astype(np.float32)
+        model.set_lr_scale(scale)
+        xin = ctx.alloc(batch * n * 8)
+        tgt = ctx.alloc(batch * n * 8)
+        # the untrained (physics-initialised) model on the test set
+        te_in = ctx.alloc(args.test * n * 8)
+        te_out = ctx.alloc(args.test * n * 8)
+        te_in.upload(rx_te.astype(np.complex64).ravel())
+
+        def evaluate():
+            m = F.LearnedDBP(ctx, steps, taps, n, args.test, *model.get())
+            m.forward(te_in, te_out)
+            return te_out.download(np.empty(args.test * n, dtype=np.complex64)).reshape(args.test, n)
+
+        q_init = q_of(evaluate().astype(np.complex128), sy_te)
+        rng = np.random.default_rng(args.seed)
+        losses = []
+        t_train = time.perf_counter()
+        for it in range(args.iters):
+            idx = rng.choice(args.train, batch, replace=False)
+            xin.upload(rx_tr[idx].astype(np.complex64).ravel())
+            tgt.upload(tx_tr[idx].astype(np.complex64).ravel())
+            lr = args.lr * (0.5 ** (it / max(1, args.iters // 3)))
+            losses.append(model.train_step(xin, tgt, lr))
+            if it % 50 == 0:
+                log(f"iter {it}: loss {losses[-1]:.4e}")
+        t_train = time.perf_counter() - t_train
+        out_l = evaluate()
+        q_l = q_of(out_l.astype(np.complex128), sy_te)
+        # learned inference throughput on the full test batch
+        m = F.LearnedDBP(ctx, steps, taps, n, args.test, *model.get())
+        m.forward(te_in, te_out)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            m.forward(te_in, te_out)
+        ctx.synchronize()
+        t_l = (time.perf_counter() - t0) / reps
+        rep = {
+            "waveforms": args.test, "samples": n, "train_waveforms": args.train, "iters": args.iters,
+            "train_seconds": round(t_train, 2),
+            "q_db": {"plain_dbp_16_steps_per_span_f64": float(np.mean(q16)),
+                     "plain_dbp_2_steps_per_span_f64": float(np.mean(q2)),
+                     "learned_2_steps_per_span_init": float(np.mean(q_init)),
+                     "learned_2_steps_per_span_trained": float(np.mean(q_l))},
+            "q_gap_learned_vs_dbp16_db": float(np.mean(q_l) - np.mean(q16)),
+            "waveforms_per_s": {"plain_dbp_16_f64": args.test / t16, "plain_dbp_2_f64": args.test / t2,
+                                "learned_f32": args.test / t_l},
+            "speedup_learned_vs_dbp16": t16 / t_l,
+            "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
+            "launch_dbm_range": [float(np.min(p_te)), float(np.max(p_te))],
+        }
+        return rep
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=128)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--samples", type=int, default=1 << 14)
+    ap.add_argument("--iters", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=4)
+    args = ap.parse_args()
+    rep = run(args, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()

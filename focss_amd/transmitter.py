@@ -61,3 +61,41 @@ def wdm_signal(seed, n_channels, n_symbols, osf, baud, spacing, power_dbm, roll_
         f = (ch - n_channels // 2) * spacing
         field += rrc_shaping(sym, osf, roll_off) * np.exp(1j * 2 * np.pi * f * dt * s)
     return field, dt, syms
+
+
+def rrc_sampling(signal, osf, roll_off, modes=1):
+    """RRC matched filter + downsample (modulation.cpp:92-96): the same
+    circular FFT filter as rrc_shaping, then every osf-th sample."""
+    n = signal.size // modes
+    out = []
+    for m in range(modes):
+        x = signal[m * n:(m + 1) * n]
+        h = rrc_pulse(n // osf, osf, roll_off)
+        zp = np.roll(h, -(n // 2))
+        y = np.fft.ifft(np.fft.fft(x) * np.fft.fft(zp)) / n
+        out.append(y[::osf])
+    return np.concatenate(out)
+
+
+def decimate(x, factor):
+    """Field::decimate (field.cpp:299-320): keep the lowest n/factor bins."""
+    n = x.size
+    ds = n // factor
+    X = np.fft.fft(x) / n
+    cut = n // factor // 2
+    Y = np.concatenate([X[:cut], X[n - (ds - cut):]])
+    return np.fft.ifft(Y) * ds
+
+
+def q2_factor(tx, rx, cut=0):
+    """q2_factor (functions.cpp:27-80): 10 log10(sum |tx|^2 / sum |tx - rx|^2)."""
+    if cut:
+        tx, rx = tx[cut:-cut], rx[cut:-cut]
+    return float(-10 * np.log10(np.sum(np.abs(tx - rx) ** 2) / np.sum(np.abs(tx) ** 2)))
+
+
+def phase_align(tx, rx):
+    """Constant-phase equalizer: the rotation minimising |tx - rx e^{i phi}|^2,
+    the closed form of PhaseShiftEqualizer's angle search
+    (phase_shift_equalizer.cpp:30-46)."""
+    return rx * np.exp(1j * np.angle(np.vdot(rx, tx)))
