@@ -709,6 +709,9 @@ bool ldbp_kernels(int taps, LdbpKernel* fwd, LdbpKernel* bwd) {
     if (taps == 15) {
         *fwd = &k_ldbp_fwd<15>;
         *bwd = &k_ldbp_bwd<15>;
+    } else if (taps == 63) {
+        *fwd = &k_ldbp_fwd<63>;
+        *bwd = &k_ldbp_bwd<63>;
     } else if (taps == 5) {
         *fwd = &k_ldbp_fwd<5>;
         *bwd = &k_ldbp_bwd<5>;
@@ -720,7 +723,7 @@ bool ldbp_kernels(int taps, LdbpKernel* fwd, LdbpKernel* bwd) {
 
 int ldbp_check(int steps, int taps, int samples, int batch) {
     LdbpKernel f, b;
-    if (!ldbp_kernels(taps, &f, &b)) return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5 or 15");
+    if (!ldbp_kernels(taps, &f, &b)) return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15 or 63");
     if (steps < 1 || batch < 1) return fail(FOCSS_E_ARG, "learned DBP: steps and batch must be >= 1");
     int l = ilog2_int(samples);
     if (l < 6 || samples > LDBP_MAX_N) return fail(FOCSS_E_ARG, "learned DBP: samples must be a power of two in [64, 16384]");

@@ -52,6 +52,7 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_fwd(LdbpArgs a) {
     extern __shared__ unsigned char smem_raw[];
     cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);
     __shared__ double red[LDBP_NT / 64];
+    __shared__ cx<float> h[K];
     const int N = a.N, mask = N - 1, c = K / 2;
     const int b = blockIdx.x;
     const cx<float>* in = a.in + (long long)b * N;
@@ -59,13 +60,12 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_fwd(LdbpArgs a) {
     __syncthreads();
     cx<float>* zs = a.zs + (long long)b * N;
     for (int j = 0; j < a.J; ++j) {
-        cx<float> h[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) h[k] = a.fir[j * K + k];
+        if (threadIdx.x < K) h[threadIdx.x] = a.fir[j * K + threadIdx.x];
+        __syncthreads();
         const float th = a.theta[j];
         for (int t = threadIdx.x; t < N; t += LDBP_NT) {
             cx<float> y = mk<float>(0, 0);
-#pragma unroll
+#pragma unroll 16
             for (int k = 0; k < K; ++k) y = y + h[k] * buf[(t - k + c) & mask];
             if (a.ys) {
                 a.xs[((long long)b * a.J + j) * N + t] = buf[t];
@@ -99,6 +99,8 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     extern __shared__ unsigned char smem_raw[];
     cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);
     __shared__ double red[LDBP_NT / 64];
+    __shared__ cx<float> h[K];
+    constexpr int KC = 8;  // taps per gradient sweep (register budget)
     const int N = a.N, mask = N - 1, c = K / 2;
     const int b = blockIdx.x;
     const float scale = 1.0f / ((float)N * (float)a.B);
@@ -124,40 +126,43 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
             g[t] = gz * mk<float>(co, -s) + yv * (2.0f * th * q);
         }
         __syncthreads();
-        // FIR tap gradients G_h_k = sum_t G_y(t) conj(x(t - k + c)); per-thread
-        // partial sums in float, block sums in double
-        float dre[K], dim[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            dre[k] = 0;
-            dim[k] = 0;
-        }
-        for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-            cx<float> gy = g[t];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                cx<float> v = gy * conj(x[(t - k + c) & mask]);
-                dre[k] += v.x;
-                dim[k] += v.y;
-            }
-        }
+        // FIR tap gradients G_h_k = sum_t G_y(t) conj(x(t - k + c)), KC taps per
+        // sweep; per-thread partial sums in float, block sums in double
         double* gr = a.grad + (long long)j * K * 2;
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            float dre[KC], dim[KC];
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                dre[q] = 0;
+                dim[q] = 0;
+            }
+            for (int t = threadIdx.x; t < N; t += LDBP_NT) {
+                cx<float> gy = g[t];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    if (k0 + q < K) {
+                        cx<float> v = gy * conj(x[(t - (k0 + q) + c) & mask]);
+                        dre[q] += v.x;
+                        dim[q] += v.y;
+                    }
+                }
+            }
 #pragma unroll 1
-        for (int k = 0; k < K; ++k) {
-            double sr = ldbp_block_sum((double)dre[k], red);
-            if (threadIdx.x == 0) atomicAdd(&gr[2 * k], sr);
-            double si = ldbp_block_sum((double)dim[k], red);
-            if (threadIdx.x == 0) atomicAdd(&gr[2 * k + 1], si);
+            for (int q = 0; q < KC && k0 + q < K; ++q) {
+                double sr = ldbp_block_sum((double)dre[q], red);
+                if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q)], sr);
+                double si = ldbp_block_sum((double)dim[q], red);
+                if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q) + 1], si);
+            }
         }
         double st = ldbp_block_sum(dth, red);
         if (threadIdx.x == 0) atomicAdd(&a.grad[(long long)a.J * K * 2 + j], st);
         // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c), staged in zs
-        cx<float> h[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) h[k] = conj(a.fir[j * K + k]);
+        if (threadIdx.x < K) h[threadIdx.x] = conj(a.fir[j * K + threadIdx.x]);
+        __syncthreads();
         for (int s = threadIdx.x; s < N; s += LDBP_NT) {
             cx<float> acc = mk<float>(0, 0);
-#pragma unroll
+#pragma unroll 16
             for (int k = 0; k < K; ++k) acc = acc + h[k] * g[(s + k - c) & mask];
             zs[s] = acc;
         }

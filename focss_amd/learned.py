@@ -7,11 +7,11 @@
    SSMF (ADAPTIVE grid, EDFA 16 dB + ASE NF 5 dB, launch 0..4 dBm) by the
    device link program, then decimated to 2 samples/symbol (2^14 samples).
 2. Plain DBP at 16 steps per span (FP64, device link program) is the judge.
-3. Learned DBP: 2 steps per span = 40 x (15-tap complex FIR, Kerr phase
-   theta), initialised from the physics (each FIR a least-squares fit of
-   the half-span's loss + dispersion inverse over the signal band, theta =
-   -kappa gamma L_eff), trained with Adam on the waveform MSE against the
-   transmitted 2 sps waveform.
+3. Learned DBP: 2 steps per span = 40 x (complex FIR, Kerr phase theta),
+   63 taps by default (15 taps cannot hold a 40 km dispersion step at 2
+   samples/symbol: the 40-step chain's composite response diverges),
+   initialised from the physics (init_params) and trained with Adam on the
+   waveform MSE against the transmitted 2 sps waveform.
 4. Q = q2_factor after RRC matched filter and constant-phase alignment; the
    report gives Q per method and waveforms/s of each backpropagation.
 """
@@ -97,27 +97,47 @@ def plain_dbp(F, ctx, rx2, steps_per_span, precision=None):
     return out, t
 
 
-def init_params(n, steps_per_span=2, taps=15):
-    """Physics initialisation of the learned model: FIR j is the least-squares
-    15-tap fit, over the signal band, of exp(alpha h/2) exp(-i beta2 h w^2/2)
-    (the BACKWARD linear step over h = L / steps_per_span, times the span's
-    drop_power on the first step of a span); theta_j = -kappa gamma L_eff(h)."""
+def _signal_psd(k):
+    """Raised-cosine power spectrum of the RRC-shaped signal at angular
+    frequency k (1 in the flat band, cosine roll-off, 0 outside)."""
+    a = np.abs(k) / (2 * np.pi * BAUD)
+    r = ROLL_OFF
+    p = np.zeros_like(a)
+    p[a <= (1 - r) / 2] = 1
+    m = (a > (1 - r) / 2) & (a <= (1 + r) / 2)
+    p[m] = 0.5 * (1 + np.cos(np.pi / r * (a[m] - (1 - r) / 2)))
+    return p
+
+
+def init_params(n, steps_per_span=2, taps=63, floor=0.01):
+    """Physics initialisation.  FIR j is a weighted least-squares fit of the
+    BACKWARD linear operator of its h = L / steps_per_span (loss gain
+    e^{alpha h/2}, dispersion e^{-i beta2 h w^2/2}, the span's drop_power on
+    its first step), weighted by the signal spectrum (floor elsewhere), and
+    fitted to the running error of the chain so far (target = ideal response
+    of steps 0..j divided by the fitted response of steps 0..j-1), so the
+    per-step truncation errors do not pile up along the 40 steps.
+    theta_j = -kappa gamma L_eff(h) for the field at the start of the step."""
     h = SPAN_LEN / steps_per_span
     dt = 1.0 / (BAUD * 2)
     k = np.fft.fftfreq(n, d=dt) * 2 * np.pi
-    band = np.abs(k) <= 2 * np.pi * BAUD * (1 + ROLL_OFF) / 2 * 1.05
-    w = k[band]
     c = taps // 2
-    A = np.exp(-1j * np.outer(w, (np.arange(taps) - c) * dt))
+    A = np.exp(-1j * np.outer(k, (np.arange(taps) - c) * dt))
+    wt = np.maximum(_signal_psd(k), floor)
     gain = np.exp(ALPHA * SPAN_LEN)
-    firs, thetas = [], []
     l_eff = (1 - np.exp(-ALPHA * h)) / ALPHA
+    firs, thetas = [], []
+    ideal = np.ones(n, dtype=np.complex128)
+    chain = np.ones(n, dtype=np.complex128)
     for span in range(N_SPANS):
         for j in range(steps_per_span):
-            Hd = np.exp(ALPHA * h / 2) * np.exp(1j * BETA2 / 2 * (-h) * w ** 2)
+            Hd = np.exp(ALPHA * h / 2) * np.exp(1j * BETA2 / 2 * (-h) * k ** 2)
             if j == 0:
                 Hd = Hd / np.sqrt(gain)
-            coef, *_ = np.linalg.lstsq(A, Hd, rcond=None)
+            ideal = ideal * Hd
+            target = ideal / np.where(np.abs(chain) > 1e-6, chain, 1.0)
+            coef, *_ = np.linalg.lstsq(A * wt[:, None], target * wt, rcond=None)
+            chain = chain * (A @ coef)
             firs.append(coef)
             thetas.append(-GAMMA * l_eff)
     return np.array(firs), np.array(thetas)
@@ -137,7 +157,7 @@ def run(args, log=print):
     import focss_amd as F
     n = args.samples
     steps = 2 * N_SPANS
-    taps = 15
+    taps = args.taps
     with F.Context(0) as ctx:
         t0 = time.perf_counter()
         rx_tr, tx_tr, _, _ = make_dataset(F, ctx, args.train, n, seed=args.seed)
@@ -216,6 +236,7 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--taps", type=int, default=63)
     args = ap.parse_args()
     rep = run(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(rep), flush=True)

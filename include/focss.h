@@ -177,7 +177,7 @@ int focss_wdm_ppe(focss_ctx* ctx, int precision, int c_radius, const double* C, 
  * z = y exp(i theta_j |y|^2).  params (device, float): the FIR taps
  * [steps][taps] as interleaved complex64, followed by theta[steps].
  * Fields are complex64 device buffers [batch][samples], samples a power of
- * two <= 16384, taps 5 or 15.  Loss = sum_b (1/(samples batch)) sum_t
+ * two <= 16384, taps 5, 15 or 63.  Loss = sum_b (1/(samples batch)) sum_t
  * |z - target|^2.  Judged against plain DBP (focss_link_run). */
 size_t focss_ldbp_workspace_bytes(int steps, int samples, int batch);
 int focss_ldbp_forward(focss_ctx* ctx, int steps, int taps, int samples, int batch, const float* params,

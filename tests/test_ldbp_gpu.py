@@ -32,7 +32,7 @@ def case(steps, taps, n, batch, seed=0):
     return fir, theta, x.astype(np.complex64), tgt.astype(np.complex64)
 
 
-@pytest.mark.parametrize("taps,n,batch", [(15, 1 << 14, 2), (5, 4096, 3)])
+@pytest.mark.parametrize("taps,n,batch", [(15, 1 << 14, 2), (5, 4096, 3), (63, 8192, 2)])
 def test_ldbp_forward_and_gradients(ctx, taps, n, batch):
     import focss_amd as F
     steps = 6
