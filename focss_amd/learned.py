@@ -171,7 +171,8 @@ def run(args, log=print):
         fir0, th0 = init_params(n, 2, taps)
         batch = min(args.batch, args.train)
         model = F.LearnedDBP(ctx, steps, taps, n, batch, fir0, th0)
-        scale = np.concatenate([np.full(2 * steps * taps, 1.0), np.abs(th0)]).astype(np.float32)
+        scale = np.concatenate([np.full(2 * steps * taps, 1.0),
+                                args.theta_lr_mult * np.abs(th0)]).astype(np.float32)
         model.set_lr_scale(scale)
         xin = ctx.alloc(batch * n * 8)
         tgt = ctx.alloc(batch * n * 8)
@@ -237,6 +238,7 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--taps", type=int, default=63)
+    ap.add_argument("--theta-lr-mult", type=float, default=1.0)
     args = ap.parse_args()
     rep = run(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(rep), flush=True)

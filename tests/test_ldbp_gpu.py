@@ -1,7 +1,7 @@
 """Learned few-step DBP kernels against the CPU oracle: forward output and
 loss, and the Wirtinger gradients of the FIR taps and Kerr scales (the
 oracle's analytic gradients are themselves pinned by finite differences in
-test_oracle.py).  FP32 kernels: tolerances 5e-5 (field) and 1e-3 (grads)."""
+test_oracle.py).  FP32 kernels: tolerances 2e-4 (field, 63-tap chains) and 1e-3 (grads)."""
 import numpy as np
 import pytest
 
@@ -22,7 +22,7 @@ def case(steps, taps, n, batch, seed=0):
     r = np.random.default_rng(seed)
     fir = np.zeros((steps, taps), dtype=np.complex128)
     fir[:, taps // 2] = 1.0
-    fir += 0.15 * (r.standard_normal((steps, taps)) + 1j * r.standard_normal((steps, taps)))
+    fir += 0.15 * np.sqrt(15 / taps) * (r.standard_normal((steps, taps)) + 1j * r.standard_normal((steps, taps)))
     theta = 0.4 * r.standard_normal(steps)
     x = (r.standard_normal(batch * n) + 1j * r.standard_normal(batch * n)) * 0.5
     tgt = (r.standard_normal(batch * n) + 1j * r.standard_normal(batch * n)) * 0.5
@@ -49,7 +49,7 @@ def test_ldbp_forward_and_gradients(ctx, taps, n, batch):
     for b in range(batch):
         sl = slice(b * n, (b + 1) * n)
         o, l = O.ldbp_forward(fir, theta, x[sl].astype(np.complex128), tgt[sl].astype(np.complex128))
-        assert np.linalg.norm(out[sl] - o) / np.linalg.norm(o) < 5e-5
+        assert np.linalg.norm(out[sl] - o) / np.linalg.norm(o) < 2e-4
         lb, gf, gt = O.ldbp_backward(fir, theta, x[sl].astype(np.complex128), tgt[sl].astype(np.complex128))
         loss_ref += lb / batch
         gf_ref += gf / batch

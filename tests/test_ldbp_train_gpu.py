@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 def test_learned_dbp_gains_over_untrained():
     from focss_amd.learned import run
-    args = argparse.Namespace(train=24, test=8, samples=4096, iters=150, batch=12, lr=1e-3, seed=7, taps=63)
+    args = argparse.Namespace(train=24, test=8, samples=4096, iters=200, batch=12, lr=1e-4, seed=7, taps=63, theta_lr_mult=1.0)
     rep = run(args, log=print)
     q = rep["q_db"]
     print(rep)
