@@ -139,6 +139,7 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32", "both"], default="both")
     ap.add_argument("--cpu-km", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the link batch would shard "
@@ -197,6 +198,20 @@ def main():
                        "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
                        "time_pass_ms": round(r["time_pass_ms"], 5),
                        "spec_pass_ms": round(r["spec_pass_ms"], 5)}
+    if not args.no_learned:
+        # the metric's second half: Q of learned few-step DBP vs plain FP64 DBP
+        from focss_amd.learned import parser as learned_parser, run as learned_run
+        la = learned_parser().parse_args([])
+        rep = learned_run(la, log=lambda m: print(m, file=sys.stderr, flush=True))
+        line["learned_dbp"] = {
+            "workload": f"{rep['waveforms']} test waveforms x 2^14 samples (16QAM 32 GBd, 2 sps), 20x80 km, "
+                        f"launch {rep['launch_dbm_range'][0]:.1f}..{rep['launch_dbm_range'][1]:.1f} dBm, "
+                        f"trained on {rep['train_waveforms']} simulator waveforms, {rep['iters']} Adam steps",
+            "q_db": {k: round(v, 3) for k, v in rep["q_db"].items()},
+            "q_gap_learned_vs_dbp16_db": round(rep["q_gap_learned_vs_dbp16_db"], 3),
+            "waveforms_per_s": {k: round(v, 1) for k, v in rep["waveforms_per_s"].items()},
+            "speedup_learned_vs_dbp16": round(rep["speedup_learned_vs_dbp16"], 2),
+            "train_seconds": rep["train_seconds"]}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(field, dt, args.cpu_km)
     line["finite"] = all(r["finite"] for r in res.values())

@@ -228,18 +228,22 @@ def run(args, log=print):
         return rep
 
 
-def main():
+def parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=128)
     ap.add_argument("--test", type=int, default=64)
     ap.add_argument("--samples", type=int, default=1 << 14)
-    ap.add_argument("--iters", type=int, default=300)
+    ap.add_argument("--iters", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--taps", type=int, default=63)
-    ap.add_argument("--theta-lr-mult", type=float, default=1.0)
-    args = ap.parse_args()
+    ap.add_argument("--theta-lr-mult", type=float, default=10.0)
+    return ap
+
+
+def main():
+    args = parser().parse_args()
     rep = run(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(rep), flush=True)
 
