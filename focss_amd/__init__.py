@@ -59,6 +59,7 @@ _sig = {
     "focss_link_run_host": (C.c_int, [_P, C.c_int, C.POINTER(Link), _P, _P]),
     "focss_fft": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "focss_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "focss_set_tile_pass": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "focss_tile_variants": (C.c_int, [C.c_int, C.c_int]),
     "focss_build_info": (C.c_char_p, []),
     "focss_link_profile": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, C.c_int, _P]),
@@ -148,8 +149,13 @@ class Context:
     def synchronize(self):
         _check(_lib.focss_synchronize(self.handle))
 
-    def set_tile(self, precision, log2_samples, variant):
-        _check(_lib.focss_set_tile(self.handle, precision, log2_samples, variant))
+    def set_tile(self, precision, log2_samples, variant, pass_=None):
+        """Tile variant for both two-pass kernels, or for one (pass_ 0 = time
+        pass, 1 = spectral pass)."""
+        if pass_ is None:
+            _check(_lib.focss_set_tile(self.handle, precision, log2_samples, variant))
+        else:
+            _check(_lib.focss_set_tile_pass(self.handle, precision, log2_samples, pass_, variant))
 
     def link_run_host(self, link, field, precision=F64):
         """Run one link program on a host complex128 array (copied in/out)."""

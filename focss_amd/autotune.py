@@ -1,6 +1,7 @@
 """Tile-shape autotuner for the two-pass SSFM kernels: times every tile
 variant of one length class on the caller's own workload (first split steps,
-HIP-event timed) and keeps the fastest per (precision, log2 samples)."""
+HIP-event timed) and keeps the fastest per (precision, log2 samples), for the
+time-domain pass and the spectral pass separately."""
 import numpy as np
 
 from . import F32, F64, tile_variants
@@ -14,16 +15,19 @@ def tune(ctx, links, batch, offsets, host_fields, precision=F64, n_pairs=24, log
         nv = tile_variants(precision, log_n)
         if nv <= 1:
             continue
-        timings = []
+        t_time, t_spec = [], []
         for v in range(nv):
             ctx.set_tile(precision, log_n, v)
             batch.upload(host_fields)
             ms = ctx.link_profile(links, batch, offsets, n_pairs, precision=precision)
-            timings.append(ms[0] + ms[1])
-        v_best = int(np.argmin(timings))
-        ctx.set_tile(precision, log_n, v_best)
-        best[log_n] = (v_best, timings)
+            t_time.append(float(ms[0]))
+            t_spec.append(float(ms[1]))
+        v_time, v_spec = int(np.argmin(t_time)), int(np.argmin(t_spec))
+        ctx.set_tile(precision, log_n, v_time, pass_=0)
+        ctx.set_tile(precision, log_n, v_spec, pass_=1)
+        best[log_n] = ((v_time, v_spec), (t_time, t_spec))
         if log:
-            log(f"autotune p={precision} 2^{log_n}: variant {v_best}, step ms {timings}")
+            log(f"autotune p={precision} 2^{log_n}: time pass variant {v_time} {t_time} ms, "
+                f"spectral pass variant {v_spec} {t_spec} ms")
     batch.upload(host_fields)
     return best

@@ -2,6 +2,7 @@
 // generator used for ASE, and workgroup reductions.
 #pragma once
 #include "cx.h"
+#include "trig.h"
 
 namespace focss_hip {
 
@@ -88,7 +89,7 @@ __device__ __forceinline__ cx<R> phase_factor(double phase, double mag);
 template <>
 __device__ __forceinline__ cx<double> phase_factor<double>(double phase, double mag) {
     double s, c;
-    sincos(phase, &s, &c);
+    sincos_fast(phase, &s, &c);
     return mk<double>(c * mag, s * mag);
 }
 
@@ -97,7 +98,7 @@ __device__ __forceinline__ cx<float> phase_factor<float>(double phase, double ma
     const double two_pi = 6.283185307179586476925286766559;
     double red = phase - two_pi * rint(phase * (1.0 / two_pi));
     float s, c;
-    sincosf((float)red, &s, &c);
+    sincos_fast((float)red, &s, &c);
     float m = (float)mag;
     return mk<float>(c * m, s * m);
 }
@@ -109,13 +110,13 @@ __device__ __forceinline__ cx<R> kerr_factor(R coef_p);
 template <>
 __device__ __forceinline__ cx<double> kerr_factor<double>(double ph) {
     double s, c;
-    sincos(ph, &s, &c);
+    sincos_fast(ph, &s, &c);
     return mk<double>(c, s);
 }
 template <>
 __device__ __forceinline__ cx<float> kerr_factor<float>(float ph) {
     float s, c;
-    sincosf(ph, &s, &c);
+    sincos_fast(ph, &s, &c);
     return mk<float>(c, s);
 }
 

@@ -166,4 +166,119 @@ __device__ __forceinline__ void fft_lds(cx<R>* __restrict__ buf, const TW& tw) {
     if constexpr (L > 1) fft_stages<R, L, S, NT, SIGN, 1, TW>(buf, tw);
 }
 
+// ------------------------------------------------------------------
+// Split-stage DFTs for the two-pass kernels.  A length-L DFT (L >= 64) runs
+// the mixed-radix schedule P(L) = (8, 2^(log2 L mod 3), 8, ..., 8), or its
+// reverse; both begin and end with a radix-8 stage.  The first stage reads
+// x[j + r L/8] and the last one writes y[j + r L/8] (r < 8, j < L/8), so a
+// thread that owns item j holds exactly the elements the neighbouring
+// operation needs: the first stage is fed from HBM, the last one feeds
+// either HBM or the pointwise operator and the next DFT's first stage
+// (which must use the reversed schedule), with no LDS round trip.
+// Stage twiddles W_{NS R}^{r k} = W_L^{r k L/(NS R)} come from a length-L
+// table in LDS, filled at kernel start.
+// ------------------------------------------------------------------
+template <int L, bool REV>
+struct Sched {
+    static constexpr int P = ilog2<L>::value;
+    static constexpr int REM = P % 3;
+    static constexpr int N = P / 3 + (REM ? 1 : 0);
+    static constexpr int fwd_radix(int i) { return (REM && i == 1) ? (1 << REM) : 8; }
+    static constexpr int radix(int i) { return REV ? fwd_radix(N - 1 - i) : fwd_radix(i); }
+    static constexpr int ns(int i) {
+        int p = 1;
+        for (int k = 0; k < i; ++k) p *= radix(k);
+        return p;
+    }
+};
+
+// w[r] = W_L^{r k STRIDE} (SIGN -1) or its conjugate (SIGN +1), 1 <= r < RADIX,
+// read from the length-L table t in LDS
+template <typename R, int RADIX, int SIGN, int STRIDE>
+__device__ __forceinline__ void stage_twiddles(const cx<R>* __restrict__ t, int k, cx<R>* w) {
+#pragma unroll
+    for (int r = 1; r < RADIX; ++r) {
+        w[r] = t[r * k * STRIDE];
+        if constexpr (SIGN > 0) w[r] = conj(w[r]);
+    }
+}
+
+// One Stockham stage (radix RADIX, NS = product of earlier radices) over S
+// interleaved sequences in LDS, NT threads; barriers before the writes and
+// after them.  tw is the length-L table W_L^m in LDS.
+template <typename R, int L, int S, int NT, int SIGN, int RADIX, int NS>
+__device__ __forceinline__ void stockham_stage(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw) {
+    constexpr int NB = L / RADIX;
+    constexpr int TOTAL = NB * S;
+    constexpr int PER = (TOTAL + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    cx<R> v[PER][RADIX];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = tid + b * NT;
+        if (TOTAL % NT == 0 || id < TOTAL) {
+            const int s = id % S;
+            const int j = id / S;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) v[b][r] = buf[pad((j + r * NB) * S + s)];
+            if constexpr (NS > 1) {
+                cx<R> w[RADIX];
+                stage_twiddles<R, RADIX, SIGN, L / (NS * RADIX)>(tw, j % NS, w);
+#pragma unroll
+                for (int r = 1; r < RADIX; ++r) v[b][r] = v[b][r] * w[r];
+            }
+            butterfly<R, RADIX, SIGN>::run(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = tid + b * NT;
+        if (TOTAL % NT == 0 || id < TOTAL) {
+            const int s = id % S;
+            const int j = id / S;
+            const int k = j % NS;
+            const int base = (j / NS) * NS * RADIX + k;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) buf[pad((base + r * NS) * S + s)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// stages 1 .. N-2 of the schedule (the first and last run in registers)
+template <typename R, int L, int S, int NT, int SIGN, bool REV, int I = 1>
+__device__ __forceinline__ void middle_stages(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw) {
+    using SC = Sched<L, REV>;
+    if constexpr (I < SC::N - 1) {
+        stockham_stage<R, L, S, NT, SIGN, SC::radix(I), SC::ns(I)>(buf, tw);
+        middle_stages<R, L, S, NT, SIGN, REV, I + 1>(buf, tw);
+    }
+}
+
+// first stage: radix-8 DFT of x[j + r L/8] held in v, written to the LDS
+// positions 8 j + r of sequence s (no barrier)
+template <typename R, int L, int S, int SIGN>
+__device__ __forceinline__ void first_stage_store(cx<R>* __restrict__ buf, cx<R>* v, int s, int j) {
+    butterfly<R, 8, SIGN>::run(v);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[pad((j * 8 + r) * S + s)] = v[r];
+}
+
+// last stage: reads x[j + r L/8] of sequence s, twiddles by W_L^{r j} from
+// the length-L table tw in LDS, radix-8 DFT; v[r] is then output element
+// j + r L/8
+template <typename R, int L, int S, int SIGN>
+__device__ __forceinline__ void last_stage_load(const cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw,
+                                                cx<R>* v, int s, int j) {
+    constexpr int Q = L / 8;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = buf[pad((j + r * Q) * S + s)];
+    cx<R> w[8];
+    stage_twiddles<R, 8, SIGN, 1>(tw, j, w);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = v[r] * w[r];
+    butterfly<R, 8, SIGN>::run(v);
+}
+
 }  // namespace focss_hip

@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/focss.h"
@@ -59,6 +60,8 @@ struct Registry {
     KernelFn<R> spec_k[21][kTileVariants] = {};
     int time_cols[21][2][kTileVariants] = {};
     int spec_rows[21][kTileVariants] = {};
+    int time_threads[21][2][kTileVariants] = {};
+    int spec_threads[21][kTileVariants] = {};
 };
 
 template <typename R, int N, int M>
@@ -75,6 +78,7 @@ void reg_time(Registry<R>& r) {
     if constexpr (C >= 1) {
         r.time_k[LOGN][M - 1][V] = &k_time_pass<R, N1, C, M>;
         r.time_cols[LOGN][M - 1][V] = C;
+        r.time_threads[LOGN][M - 1][V] = pass_threads(C * N1 / 8);
     }
 }
 
@@ -86,6 +90,7 @@ void reg_spec(Registry<R>& r) {
     if constexpr (S >= 1) {
         r.spec_k[LOGN][V] = &k_spec_pass<R, N2, S>;
         r.spec_rows[LOGN][V] = S;
+        r.spec_threads[LOGN][V] = pass_threads(S * N2 / 8);
     }
 }
 
@@ -245,13 +250,14 @@ struct focss_ctx {
     int* h_done = nullptr;  // pinned, [2]
     hipEvent_t ev_done[2] = {nullptr, nullptr};
     size_t cap_waves = 0;
-    int tile_variant[2][21] = {};
+    int tile_variant[2][2][21] = {};  // [pass: 0 time, 1 spectral][precision][log2 samples]
     GraphCache graph;
     void* scratch = nullptr;  // out-of-place buffer of the two-pass DFT
     size_t scratch_bytes = 0;
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
     long long max_split_steps = 20000000;
     bool use_graph = true;  // FOCSS_NO_GRAPH=1: plain launches (profilers)
+    int n_cu = 256;          // compute units of the device
 };
 
 namespace {
@@ -304,15 +310,35 @@ bool resident_ok(int precision, int log_n, int modes) {
     return reg32().resident[log_n][modes - 1] != nullptr;
 }
 
+bool time_ok(int precision, int log_n, int modes, int variant) {
+    if (log_n < 0 || log_n > 20 || variant < 0 || variant >= kTileVariants) return false;
+    if (precision == FOCSS_F64) return reg64().time_k[log_n][modes - 1][variant] != nullptr;
+    return reg32().time_k[log_n][modes - 1][variant] != nullptr;
+}
+
+bool spec_ok(int precision, int log_n, int variant) {
+    if (log_n < 0 || log_n > 20 || variant < 0 || variant >= kTileVariants) return false;
+    if (precision == FOCSS_F64) return reg64().spec_k[log_n][variant] != nullptr;
+    return reg32().spec_k[log_n][variant] != nullptr;
+}
+
 bool two_pass_ok(int precision, int log_n, int modes, int variant) {
-    if (log_n < 0 || log_n > 20) return false;
-    if (precision == FOCSS_F64)
-        return reg64().time_k[log_n][modes - 1][variant] && reg64().spec_k[log_n][variant];
-    return reg32().time_k[log_n][modes - 1][variant] && reg32().spec_k[log_n][variant];
+    return time_ok(precision, log_n, modes, variant) && spec_ok(precision, log_n, variant);
+}
+
+// the selected variant if it exists for this class, else the first that does
+int usable_variant(bool time_pass, int precision, int log_n, int modes, int want) {
+    auto ok = [&](int v) { return time_pass ? time_ok(precision, log_n, modes, v) : spec_ok(precision, log_n, v); };
+    if (ok(want)) return want;
+    for (int v = 0; v < kTileVariants; ++v)
+        if (ok(v)) return v;
+    return -1;
 }
 
 struct LengthClass {
-    int log_n, modes, resident, variant;
+    int log_n, modes, resident;
+    int variant;       // time pass tile variant
+    int spec_variant;  // spectral pass tile variant
     int first;  // index into the waves array
     int count;
 };
@@ -324,20 +350,24 @@ void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     int n2 = 1 << (lc.log_n - lc.log_n / 2);
     a.tiles_per_wave = n2 / cols;
     a.waves += lc.first;
+    a.class_waves = lc.count;
     a.log_n = lc.log_n;
-    hipLaunchKernelGGL(r.time_k[lc.log_n][lc.modes - 1][lc.variant], dim3(lc.count * a.tiles_per_wave),
-                       dim3(NT), 0, c->stream, a, tb);
+    KernelFn<R> k = r.time_k[lc.log_n][lc.modes - 1][lc.variant];
+    hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]),
+                       0, c->stream, a, tb);
 }
 
 template <typename R>
 void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
     const Registry<R>& r = reg<R>();
-    int rows = r.spec_rows[lc.log_n][lc.variant];
+    int rows = r.spec_rows[lc.log_n][lc.spec_variant];
     int n1 = 1 << (lc.log_n / 2);
     a.tiles_per_wave = lc.modes * n1 / rows;
     a.waves += lc.first;
+    a.class_waves = lc.count;
     a.log_n = lc.log_n;
-    hipLaunchKernelGGL(r.spec_k[lc.log_n][lc.variant], dim3(lc.count * a.tiles_per_wave), dim3(NT), 0,
+    KernelFn<R> k = r.spec_k[lc.log_n][lc.spec_variant];
+    hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0,
                        c->stream, a, tb);
 }
 
@@ -345,6 +375,7 @@ template <typename R>
 void launch_resident(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
     const Registry<R>& r = reg<R>();
     a.waves += lc.first;
+    a.class_waves = lc.count;
     a.log_n = lc.log_n;
     a.tiles_per_wave = 1;
     hipLaunchKernelGGL(r.resident[lc.log_n][lc.modes - 1], dim3(lc.count), dim3(NT), 0, c->stream, a, tb);
@@ -417,14 +448,15 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         LengthClass lc;
         lc.log_n = kv.first.first;
         lc.modes = kv.first.second;
-        lc.variant = c->tile_variant[precision][lc.log_n];
+        lc.variant = usable_variant(true, precision, lc.log_n, lc.modes, c->tile_variant[0][precision][lc.log_n]);
+        lc.spec_variant = usable_variant(false, precision, lc.log_n, lc.modes, c->tile_variant[1][precision][lc.log_n]);
         lc.first = (int)order.size();
         lc.count = (int)kv.second.size();
         for (int i : kv.second) order.push_back(i);
         if (resident_ok(precision, lc.log_n, lc.modes)) {
             lc.resident = 1;
             resident.push_back(lc);
-        } else if (two_pass_ok(precision, lc.log_n, lc.modes, lc.variant)) {
+        } else if (lc.variant >= 0 && lc.spec_variant >= 0) {
             lc.resident = 0;
             two_pass.push_back(lc);
         } else {
@@ -439,6 +471,8 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     Tables<R> tb;
     tb.bank = (const cx<R>*)c->bank[precision];
     tb.fine = (const cx<R>*)c->fine[precision];
+    tb.bank64 = (const cx<double>*)c->bank[FOCSS_F64];
+    tb.fine64 = (const cx<double>*)c->fine[FOCSS_F64];
     PassArgs a;
     a.field = batch;
     a.statics = c->d_statics;
@@ -595,6 +629,8 @@ int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
     const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
     tb.bank = (const cx<R>*)c->bank[prec];
     tb.fine = (const cx<R>*)c->fine[prec];
+    tb.bank64 = (const cx<double>*)c->bank[FOCSS_F64];
+    tb.fine64 = (const cx<double>*)c->fine[FOCSS_F64];
     const FftRegistry<R>& r = fft_reg<R>();
     cx<R>* d = (cx<R>*)data;
     if (r.whole[log_n][si]) {
@@ -755,8 +791,12 @@ int focss_ctx_create(int device, focss_ctx** out) {
     const char* ng = getenv("FOCSS_NO_GRAPH");
     c->use_graph = !(ng && ng[0] == '1');
     for (int p = 0; p <= 20; ++p) {
-        c->tile_variant[FOCSS_F64][p] = 1;  // 2048-element tiles for complex128
-        c->tile_variant[FOCSS_F32][p] = 0;  // 4096-element tiles for complex64
+        // measured defaults at 2^18 samples: time pass 2048 (complex128) /
+        // 4096 (complex64) elements per tile, spectral pass 1024
+        c->tile_variant[0][FOCSS_F64][p] = 1;
+        c->tile_variant[0][FOCSS_F32][p] = 0;
+        c->tile_variant[1][FOCSS_F64][p] = 2;
+        c->tile_variant[1][FOCSS_F32][p] = 2;
     }
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
@@ -764,6 +804,7 @@ int focss_ctx_create(int device, focss_ctx** out) {
         return fail(FOCSS_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
     }
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     std::vector<cx<double>> b64, f64;
     std::vector<cx<float>> b32, f32;
     host_tables<double>(b64, f64);
@@ -1197,7 +1238,17 @@ int focss_set_tile(focss_ctx* c, int precision, int log2_samples, int variant) {
     if (!c || precision < 0 || precision > 1 || log2_samples < 0 || log2_samples > 20 || variant < 0 ||
         variant >= kTileVariants)
         return fail(FOCSS_E_ARG, "bad tile selection");
-    c->tile_variant[precision][log2_samples] = variant;
+    c->tile_variant[0][precision][log2_samples] = variant;
+    c->tile_variant[1][precision][log2_samples] = variant;
+    drop_graph(c);
+    return 0;
+}
+
+int focss_set_tile_pass(focss_ctx* c, int precision, int log2_samples, int pass, int variant) {
+    if (!c || precision < 0 || precision > 1 || log2_samples < 0 || log2_samples > 20 || pass < 0 || pass > 1 ||
+        variant < 0 || variant >= kTileVariants)
+        return fail(FOCSS_E_ARG, "bad tile selection");
+    c->tile_variant[pass][precision][log2_samples] = variant;
     drop_graph(c);
     return 0;
 }

@@ -24,10 +24,18 @@ namespace focss_hip {
 
 constexpr int NT = 256;  // threads per workgroup (4 wave64)
 
+// Workgroup size of a two-pass kernel with `items` 8-point work items: one
+// item per thread, 64 to 512 threads.
+__host__ __device__ constexpr int pass_threads(int items) {
+    return items < 64 ? 64 : (items > 512 ? 512 : items);
+}
+
 template <typename R>
 struct Tables {
     const cx<R>* bank;  // tw bank, W_{2^p}^m for p <= TW_MAX_LOG
     const cx<R>* fine;  // W_{2^20}^m, m < 1024
+    const cx<double>* bank64;  // the same two tables in FP64
+    const cx<double>* fine64;
 };
 
 // W_N^e for N = 2^log_n <= 2^20, e in [0, N): W_{2^20}^{E}, E = e 2^{20-log_n},
@@ -55,6 +63,7 @@ struct PassArgs {
     int* done_count;
     const int* waves;    // wave ids of this length class
     int n_waves;         // total waveforms (stride of states/red)
+    int class_waves;     // waveforms in this launch (entries of waves)
     int tiles_per_wave;
     int counter;         // pass counter (time pass c or spectral pass r)
     int log_n;           // log2 of samples for this class
@@ -180,17 +189,53 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
 // ------------------------------------------------------------------
 // two-pass kernels
 // ------------------------------------------------------------------
-// time pass: S = C * M sequences (C columns x M modes) of length N1
+// Both passes run their two DFTs as split-stage schedules (fft_lds.h): the
+// first stage of the first DFT reads HBM, the last stage of the first DFT,
+// the pointwise operator and the first stage of the second DFT share
+// registers, and the last stage of the second DFT writes HBM, so each
+// element crosses LDS 8 times per pass.  A work item is (sequence group c,
+// j < L/8); it owns elements j + r L/8, r < 8.
+
+// Inter-pass twiddles W_N^{n2 (j + r Q)}, r < 8, formed in FP64 from two bank
+// lookups and products of depth <= 4, then rounded once to R.
+template <typename R>
+__device__ __forceinline__ void inter_twiddles(const Tables<R>& tb, unsigned n2, unsigned j, unsigned Q, int log_n,
+                                               cx<R>* t) {
+    const unsigned mask = (1u << log_n) - 1u;
+    const unsigned sh = 20 - log_n;
+    auto big = [&](unsigned e) {
+        unsigned E = (e << sh) & ((1u << 20) - 1u);
+        return tb.bank64[tw_offset(10) + (E >> 10)] * tb.fine64[E & 1023u];
+    };
+    cx<double> t0 = big((n2 * j) & mask);
+    cx<double> q1 = big((n2 * Q) & mask);
+    cx<double> q2 = q1 * q1;
+    cx<double> q4 = q2 * q2;
+    cx<double> d[8];
+    d[0] = t0;
+    d[1] = t0 * q1;
+    d[2] = t0 * q2;
+    d[3] = d[1] * q2;
+    d[4] = t0 * q4;
+    d[5] = d[1] * q4;
+    d[6] = d[2] * q4;
+    d[7] = d[3] * q4;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) t[r] = mk<R>((R)d[r].x, (R)d[r].y);
+}
+
+// time pass: C columns x M modes, each column of length N1 with stride N2
 template <typename R, int N1, int C, int M>
-__global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
+__global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs a, Tables<R> tb) {
     constexpr int S = C * M;
-    constexpr int TB = 32;  // inter-pass twiddle split k1 = hi * TB + lo
-    static_assert(N1 % TB == 0, "column length must be a multiple of 32");
+    constexpr int Q = N1 / 8;
+    constexpr int ITEMS = C * Q;
+    constexpr int TN = pass_threads(ITEMS);
+    constexpr int PER = (ITEMS + TN - 1) / TN;
+    static_assert(N1 >= 64, "split-stage DFT needs length >= 64");
     __shared__ cx<R> buf[padded(N1 * S)];
     __shared__ cx<R> tw_t[N1];
-    __shared__ cx<R> tw_lo[C][TB];       // W_N^{n2 lo}
-    __shared__ cx<R> tw_hi[C][N1 / TB];  // W_N^{n2 TB hi}
-    __shared__ double redbuf[2 * (NT / 64)];
+    __shared__ double redbuf[2 * (TN / 64)];
     const int blk = xcd_block();
     const int w_id = a.waves[blk / a.tiles_per_wave];
     const int tile = blk % a.tiles_per_wave;
@@ -203,55 +248,150 @@ __global__ __launch_bounds__(NT) void k_time_pass(PassArgs a, Tables<R> tb) {
     cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset;
     const bool inverse = st.top != TOP_ENTRY;
     const bool forward = st.top != TOP_EXIT;
-    load_tw_full<R, N1, NT>(tw_t, tb.bank);
-    const TwFull<R> tw{tw_t};
-    for (int i = threadIdx.x; i < C * TB; i += NT) {
-        int c = i / TB, l = i % TB;
-        tw_lo[c][l] = tw_big<R>(tb, (unsigned)((n2_0 + c) * l) & (unsigned)(N - 1), a.log_n);
+    load_tw_full<R, N1, TN>(tw_t, tb.bank);
+    cx<R> v[PER][M][8];
+    cx<R> itw[PER][8];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int c = id % C, j = id / C;
+            const int n2 = n2_0 + c;
+            inter_twiddles<R>(tb, (unsigned)n2, (unsigned)j, (unsigned)Q, a.log_n, itw[b]);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) v[b][m][r] = g[(long long)m * N + (long long)(j + r * Q) * N2 + n2];
+            }
+            if (inverse) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) v[b][m][r] = v[b][m][r] * conj(itw[b][r]);
+                    first_stage_store<R, N1, S, +1>(buf, v[b][m], m * C + c, j);
+                }
+            }
+        }
     }
-    for (int i = threadIdx.x; i < C * (N1 / TB); i += NT) {
-        int c = i / (N1 / TB), h = i % (N1 / TB);
-        tw_hi[c][h] = tw_big<R>(tb, (unsigned)((n2_0 + c) * (TB * h)) & (unsigned)(N - 1), a.log_n);
+    if (inverse) {
+        __syncthreads();
+        middle_stages<R, N1, S, TN, +1, false>(buf, tw_t);
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+#pragma unroll
+                for (int m = 0; m < M; ++m) last_stage_load<R, N1, S, +1>(buf, tw_t, v[b][m], m * C + c, j);
+            }
+        }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < N1 * S; i += NT) {
-        int n1 = i / S, s = i % S;
-        int m = s / C, c = s % C;
-        cx<R> v = g[(long long)m * N + (long long)n1 * N2 + n2_0 + c];
-        if (inverse) v = v * conj(tw_hi[c][n1 / TB] * tw_lo[c][n1 % TB]);
-        buf[pad(n1 * S + s)] = v;
-    }
-    __syncthreads();
-    if (inverse) fft_lds<R, N1, S, NT, +1>(buf, tw);
     double pk = 0, sm = 0;
-    for (int i = threadIdx.x; i < N1 * C; i += NT) {
-        int n1 = i / C, c = i % C;
-        cx<R> v[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = buf[pad(n1 * S + m * C + c)];
-        time_op<R, M>(v, w, st, (long long)n1 * N2 + n2_0 + c, pk, sm);
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int c = id % C, j = id / C;
 #pragma unroll
-        for (int m = 0; m < M; ++m) buf[pad(n1 * S + m * C + c)] = v[m];
+            for (int r = 0; r < 8; ++r) {
+                cx<R> u[M];
+#pragma unroll
+                for (int m = 0; m < M; ++m) u[m] = v[b][m][r];
+                time_op<R, M>(u, w, st, (long long)(j + r * Q) * N2 + n2_0 + c, pk, sm);
+#pragma unroll
+                for (int m = 0; m < M; ++m) v[b][m][r] = u[m];
+            }
+        }
     }
-    block_max_sum<NT>(pk, sm, redbuf);
+    // the reduction's barriers also order the last-stage LDS reads above
+    // before the first-stage writes below
+    block_max_sum<TN>(pk, sm, redbuf);
     if (threadIdx.x == 0) {
         Reduction* r = a.red + (a.counter % 3) * a.n_waves + w_id;
         atomicMax(&r->peak_bits, double_to_bits(pk));
         atomicAdd(&r->sum, sm);
     }
-    if (forward) fft_lds<R, N1, S, NT, -1>(buf, tw);
-    for (int i = threadIdx.x; i < N1 * S; i += NT) {
-        int n1 = i / S, s = i % S;
-        int m = s / C, c = s % C;
-        cx<R> v = buf[pad(n1 * S + s)];
-        if (forward) v = v * (tw_hi[c][n1 / TB] * tw_lo[c][n1 % TB]);
-        g[(long long)m * N + (long long)n1 * N2 + n2_0 + c] = v;
+    if (forward) {
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+#pragma unroll
+                for (int m = 0; m < M; ++m) first_stage_store<R, N1, S, -1>(buf, v[b][m], m * C + c, j);
+            }
+        }
+        __syncthreads();
+        middle_stages<R, N1, S, TN, -1, true>(buf, tw_t);
+    }
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int c = id % C, j = id / C;
+            const int n2 = n2_0 + c;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (forward) {
+                    last_stage_load<R, N1, S, -1>(buf, tw_t, v[b][m], m * C + c, j);
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) v[b][m][r] = v[b][m][r] * itw[b][r];
+                }
+#pragma unroll
+                for (int r = 0; r < 8; ++r) g[(long long)m * N + (long long)(j + r * Q) * N2 + n2] = v[b][m][r];
+            }
+        }
+    }
+}
+
+// Dispersion, loss and gain for the 8 bins k = k1 + N1 (j + r Q) of one work
+// item, multiplied into v.  The signed bins k_s form the progression
+// K0 + t D, t < 8, D = N/8, K0 = k1 + N1 j - 4 D (r = (t + 4) mod 8), since
+// bins above N/2 are k - N; the Nyquist bin's sign does not matter in k^2.
+// FP64: exp(i theta k_s^2) by the recurrence f_{t+1} = f_t g_t,
+// g_{t+1} = g_t exp(2 i theta D^2) (3 sincos per 8 bins, error ~ 30 ulp).
+// FP32: each phase reduced mod 2 pi in FP64, then FP32 sin/cos.
+template <typename R>
+__device__ __forceinline__ void spectral_multiply(cx<R>* v, const WaveStatic& w, int k1, int j, int N1, int Q,
+                                                  double arg, double mag) {
+    const long long N = w.samples;
+    const long long D = N / 8;
+    const long long K0 = (long long)k1 + (long long)N1 * j - 4 * D;
+    if constexpr (sizeof(R) == 8) {
+        const double th = arg * w.dw * w.dw;
+        double s, c;
+        const double dK0 = (double)K0, dD = (double)D;
+        sincos_fast(th * dK0 * dK0, &s, &c);
+        cx<double> f = mk<double>(c * mag, s * mag);
+        sincos_fast(th * dD * (2.0 * dK0 + dD), &s, &c);
+        cx<double> gg = mk<double>(c, s);
+        sincos_fast(2.0 * th * dD * dD, &s, &c);
+        const cx<double> cc = mk<double>(c, s);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int r = (t + 4) & 7;
+            v[r] = v[r] * f;
+            f = f * gg;
+            gg = gg * cc;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int t = (r + 4) & 7;
+            const double om = (double)(K0 + t * D) * w.dw;
+            v[r] = v[r] * phase_factor<R>(arg * om * om, mag);
+        }
     }
 }
 
 // spectral pass: S rows of length N2; row q = m * N1 + k1 is contiguous
 template <typename R, int N2, int S>
-__global__ __launch_bounds__(NT) void k_spec_pass(PassArgs a, Tables<R> tb) {
+__global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs a, Tables<R> tb) {
+    constexpr int Q = N2 / 8;
+    constexpr int ITEMS = S * Q;
+    constexpr int TN = pass_threads(ITEMS);
+    constexpr int PER = (ITEMS + TN - 1) / TN;
+    static_assert(N2 >= 64, "split-stage DFT needs length >= 64");
     __shared__ cx<R> buf[padded(N2 * S)];
     __shared__ cx<R> tw_t[N2];
     const int blk = xcd_block();
@@ -274,27 +414,48 @@ __global__ __launch_bounds__(NT) void k_spec_pass(PassArgs a, Tables<R> tb) {
     const int N = w.samples;
     const int N1 = N / N2;
     cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset + (long long)tile * S * N2;
-    load_tw_full<R, N2, NT>(tw_t, tb.bank);
-    const TwFull<R> tw{tw_t};
-    for (int i = threadIdx.x; i < N2 * S; i += NT) {
-        int s = i / N2, n2 = i % N2;
-        buf[pad(n2 * S + s)] = g[i];
+    load_tw_full<R, N2, TN>(tw_t, tb.bank);
+    cx<R> v[PER][8];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int s = id % S, j = id / S;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[b][r] = g[s * N2 + j + r * Q];
+            first_stage_store<R, N2, S, -1>(buf, v[b], s, j);
+        }
     }
     __syncthreads();
-    fft_lds<R, N2, S, NT, -1>(buf, tw);
+    middle_stages<R, N2, S, TN, -1, false>(buf, tw_t);
     const double arg = 0.5 * w.beta2 * op.s;
     const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
-    for (int i = threadIdx.x; i < N2 * S; i += NT) {
-        int k2 = i / S, s = i % S;
-        int q = tile * S + s;
-        int k1 = q % N1;
-        buf[pad(i)] = buf[pad(i)] * spectral_factor<R>(w, op, k1 + N1 * k2, arg, mag);
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int s = id % S, j = id / S;
+            last_stage_load<R, N2, S, -1>(buf, tw_t, v[b], s, j);
+            spectral_multiply<R>(v[b], w, (tile * S + s) % N1, j, N1, Q, arg, mag);
+        }
     }
     __syncthreads();
-    fft_lds<R, N2, S, NT, +1>(buf, tw);
-    for (int i = threadIdx.x; i < N2 * S; i += NT) {
-        int s = i / N2, n2 = i % N2;
-        g[i] = buf[pad(n2 * S + s)];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) first_stage_store<R, N2, S, +1>(buf, v[b], id % S, id / S);
+    }
+    __syncthreads();
+    middle_stages<R, N2, S, TN, +1, true>(buf, tw_t);
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int s = id % S, j = id / S;
+            last_stage_load<R, N2, S, +1>(buf, tw_t, v[b], s, j);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) g[s * N2 + j + r * Q] = v[b][r];
+        }
     }
 }
 

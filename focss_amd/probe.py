@@ -4,7 +4,7 @@ Runs the first `pairs` split steps of the benchmark workload (a batch of
 5-channel WDM fields, 2^18 samples, adaptive 80 km spans) with one kernel
 launch per pass, then one plain 16-byte copy kernel over `copy_mib` MiB,
 whose counters calibrate HBM bytes.  Usage:
-    python -m focss_amd.probe [--batch 16] [--pairs 12] [--precision f64|f32]
+    python -m focss_amd.probe [--batch 16] [--pairs 12] [--precision f64|f32] [--variant V]
 """
 import argparse
 import os
@@ -21,6 +21,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=12)
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
+    ap.add_argument("--variant", type=int, default=-1, help="tile variant (default: the library's choice)")
     args = ap.parse_args()
     sys.path.insert(0, ROOT)
     import bench
@@ -39,13 +40,15 @@ def main():
     offsets = np.arange(args.batch, dtype=np.int64) * bench.N_SAMPLES
     links = bench.make_links(F, dt, args.batch)
     with F.Context(0) as ctx:
+        if args.variant >= 0:
+            ctx.set_tile(prec, int(np.log2(bench.N_SAMPLES)), args.variant)
         buf = ctx.alloc(host.nbytes)
         buf.upload(host)
         ms = ctx.link_profile(links, buf, offsets, args.pairs, precision=prec)
         nbytes = args.copy_mib << 20
         a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
         ctx.device_copy(b, a, nbytes)
-        print(f"probe: time pass {ms[0] * 1e3:.2f} us, spectral pass {ms[1] * 1e3:.2f} us, "
+        print(f"probe {args.precision} variant {args.variant}: time pass {ms[0] * 1e3:.2f} us, spectral pass {ms[1] * 1e3:.2f} us, "
               f"field bytes {host.nbytes}, copy bytes {nbytes}", flush=True)
 
 

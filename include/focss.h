@@ -196,9 +196,12 @@ int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int samples, int 
                           const void* in, const void* target, void* workspace, size_t workspace_bytes,
                           double* loss_host);
 
-/* Tuning: choose the tile shape of the two-pass kernels for one length
- * (0 = default).  Exposed for the autotuner. */
+/* Tuning: choose the tile shape of the two-pass kernels for one length,
+ * for both passes (focss_set_tile) or for one (pass 0 = time-domain pass,
+ * 1 = spectral pass).  Exposed for the autotuner; no counterpart in the
+ * reference (FFTW's planner plays this role, field.cpp:530-541). */
 int focss_set_tile(focss_ctx* ctx, int precision, int log2_samples, int variant);
+int focss_set_tile_pass(focss_ctx* ctx, int precision, int log2_samples, int pass, int variant);
 int focss_tile_variants(int precision, int log2_samples);
 
 /* Version / capability string of the build. */

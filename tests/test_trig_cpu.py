@@ -1,0 +1,24 @@
+"""The phase-rotation sin/cos of the split-step kernels (focss_amd/csrc/trig.h),
+compiled for the host and checked against long-double libm: Cody-Waite
+reduction plus Taylor polynomials must stay within a few ulp of 1."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_sincos_fast_accuracy(tmp_path):
+    exe = tmp_path / "trig_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "focss_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "trig_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    e64, e64_big, e32, e_fb = map(float, out)
+    print("max abs error: f64 %.2e (|x|<=1e2), %.2e (|x|<=1e6), f32 %.2e, fallback %.2e" % (e64, e64_big, e32, e_fb))
+    assert e64 <= 4.5e-16
+    assert e64_big <= 4.5e-16
+    assert e32 <= 2.5e-7
+    assert e_fb <= 1e-15

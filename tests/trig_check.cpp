@@ -1,0 +1,44 @@
+// Checks focss_hip::sincos_fast against long-double libm over the ranges the
+// split-step kernels use; prints the worst absolute errors.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include "trig.h"
+
+int main() {
+    std::mt19937_64 rng(12345);
+    double e64 = 0, e32 = 0, e64_big = 0;
+    auto check64 = [&](double x, double& e) {
+        double s, c;
+        focss_hip::sincos_fast(x, &s, &c);
+        long double ls = sinl((long double)x), lc = cosl((long double)x);
+        e = fmax(e, (double)fabsl(s - ls));
+        e = fmax(e, (double)fabsl(c - lc));
+    };
+    auto check32 = [&](float x) {
+        float s, c;
+        focss_hip::sincos_fast(x, &s, &c);
+        long double ls = sinl((long double)x), lc = cosl((long double)x);
+        e32 = fmax(e32, (double)fabsl(s - ls));
+        e32 = fmax(e32, (double)fabsl(c - lc));
+    };
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    for (int i = 0; i < 2000000; ++i) {
+        double scale = std::pow(10.0, 6.0 * (u(rng) + 1.0) / 2.0 - 4.0);  // 1e-4 .. 1e2
+        check64(u(rng) * scale, e64);
+        check32((float)(u(rng) * scale));
+    }
+    for (int i = 0; i < 1000000; ++i) check64(u(rng) * 1.0e6, e64_big);
+    for (int k = -2000; k <= 2000; ++k) {  // neighbourhoods of multiples of pi/2
+        double x = k * 1.5707963267948966;
+        for (int d = -3; d <= 3; ++d) {
+            check64(std::nextafter(x, x + d), e64);
+            check32((float)x + d * 1e-7f);
+        }
+    }
+    double s, c;
+    focss_hip::sincos_fast(3.0e7, &s, &c);  // library fallback
+    double e_fb = fmax(fabs(s - sin(3.0e7)), fabs(c - cos(3.0e7)));
+    printf("%.3e %.3e %.3e %.3e\n", e64, e64_big, e32, e_fb);
+    return 0;
+}
