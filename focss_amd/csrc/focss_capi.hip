@@ -350,7 +350,6 @@ void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     int n2 = 1 << (lc.log_n - lc.log_n / 2);
     a.tiles_per_wave = n2 / cols;
     a.waves += lc.first;
-    a.class_waves = lc.count;
     a.log_n = lc.log_n;
     KernelFn<R> k = r.time_k[lc.log_n][lc.modes - 1][lc.variant];
     hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]),
@@ -364,7 +363,6 @@ void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     int n1 = 1 << (lc.log_n / 2);
     a.tiles_per_wave = lc.modes * n1 / rows;
     a.waves += lc.first;
-    a.class_waves = lc.count;
     a.log_n = lc.log_n;
     KernelFn<R> k = r.spec_k[lc.log_n][lc.spec_variant];
     hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0,
@@ -375,7 +373,6 @@ template <typename R>
 void launch_resident(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
     const Registry<R>& r = reg<R>();
     a.waves += lc.first;
-    a.class_waves = lc.count;
     a.log_n = lc.log_n;
     a.tiles_per_wave = 1;
     hipLaunchKernelGGL(r.resident[lc.log_n][lc.modes - 1], dim3(lc.count), dim3(NT), 0, c->stream, a, tb);

@@ -63,7 +63,6 @@ struct PassArgs {
     int* done_count;
     const int* waves;    // wave ids of this length class
     int n_waves;         // total waveforms (stride of states/red)
-    int class_waves;     // waveforms in this launch (entries of waves)
     int tiles_per_wave;
     int counter;         // pass counter (time pass c or spectral pass r)
     int log_n;           // log2 of samples for this class
