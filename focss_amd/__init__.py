@@ -71,11 +71,11 @@ _sig = {
     "focss_sino_train": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "focss_wdm_ppe": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P]),
     "focss_ldbp_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
-    "focss_ldbp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
-    "focss_ldbp_gradients": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_size_t,
-                                       _P, _P]),
-    "focss_ldbp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int, C.c_float,
-                                        _P, _P, _P, _P, C.c_size_t, _P]),
+    "focss_ldbp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "focss_ldbp_gradients": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
+                                       C.c_size_t, _P, _P]),
+    "focss_ldbp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int,
+                                        C.c_float, _P, _P, _P, _P, C.c_size_t, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -222,13 +222,19 @@ class Context:
 
 class LearnedDBP:
     """Device-resident learned few-step DBP: `steps` x (complex FIR of `taps`
-    taps, Kerr phase theta).  Parameters are float32 on the device: FIR taps
-    [steps][taps] complex64 then theta[steps]."""
+    taps, nonlinear phase from the intensity filtered by `kerr_taps` real
+    taps; one tap = the Kerr phase theta |y|^2).  Parameters are float32 on
+    the device: FIR taps [steps][taps] complex64, then g [steps][kerr_taps].
+    `kerr` is theta [steps] or g [steps][kerr_taps]."""
 
-    def __init__(self, ctx, steps, taps, samples, batch, fir, theta):
+    def __init__(self, ctx, steps, taps, samples, batch, fir, kerr):
+        kerr = np.asarray(kerr, dtype=np.float64)
+        if kerr.ndim == 1:
+            kerr = kerr.reshape(steps, 1)
         self.ctx, self.steps, self.taps, self.samples, self.batch = ctx, steps, taps, samples, batch
-        self.n_params = 2 * steps * taps + steps
-        host = self.pack(fir, theta)
+        self.kerr_taps = kerr.shape[1]
+        self.n_params = 2 * steps * taps + steps * self.kerr_taps
+        host = self.pack(fir, kerr)
         self.params = ctx.alloc(host.nbytes)
         self.params.upload(host)
         self.m = ctx.alloc(host.nbytes)
@@ -242,14 +248,17 @@ class LearnedDBP:
         self.grad = ctx.alloc(8 * (self.n_params + 1))
         self.lr_scale = None
 
-    def pack(self, fir, theta):
+    def pack(self, fir, kerr):
         fir = np.ascontiguousarray(fir, dtype=np.complex64).reshape(self.steps, self.taps)
-        return np.concatenate([fir.view(np.float32).ravel(), np.asarray(theta, dtype=np.float32)])
+        kerr = np.asarray(kerr, dtype=np.float32).reshape(self.steps, self.kerr_taps)
+        return np.concatenate([fir.view(np.float32).ravel(), kerr.ravel()])
 
     def get(self):
+        """(fir [steps][taps] complex, g [steps][kerr_taps])"""
         host = self.params.download(np.empty(self.n_params, dtype=np.float32))
-        fir = host[:2 * self.steps * self.taps].view(np.complex64).reshape(self.steps, self.taps)
-        return fir.astype(np.complex128), host[2 * self.steps * self.taps:].astype(np.float64)
+        nf = 2 * self.steps * self.taps
+        fir = host[:nf].view(np.complex64).reshape(self.steps, self.taps)
+        return fir.astype(np.complex128), host[nf:].astype(np.float64).reshape(self.steps, self.kerr_taps)
 
     def set_lr_scale(self, scale):
         scale = np.ascontiguousarray(scale, dtype=np.float32)
@@ -259,27 +268,28 @@ class LearnedDBP:
     def forward(self, inp, out=None, target=None, batch=None):
         loss = C.c_double(0)
         b = batch or self.batch
-        _check(_lib.focss_ldbp_forward(self.ctx.handle, self.steps, self.taps, self.samples, b, self.params.ptr,
-                                       inp.ptr, out.ptr if out else None, target.ptr if target else None,
-                                       C.byref(loss)))
+        _check(_lib.focss_ldbp_forward(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples, b,
+                                       self.params.ptr, inp.ptr, out.ptr if out else None,
+                                       target.ptr if target else None, C.byref(loss) if target else None))
         return loss.value
 
     def gradients(self, inp, target):
-        """(loss, Wirtinger grad of the taps [steps][taps] complex, dL/dtheta)."""
+        """(loss, Wirtinger grad of the taps [steps][taps] complex, dL/dg [steps][kerr_taps])."""
         self.grad.upload(np.zeros(self.n_params + 1, dtype=np.float64))
         gp = self.grad.ptr.value
-        _check(_lib.focss_ldbp_gradients(self.ctx.handle, self.steps, self.taps, self.samples, self.batch,
-                                         self.params.ptr, inp.ptr, target.ptr, self.ws.ptr, self.ws_bytes,
-                                         C.c_void_p(gp + 8), C.c_void_p(gp)))
+        _check(_lib.focss_ldbp_gradients(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples,
+                                         self.batch, self.params.ptr, inp.ptr, target.ptr, self.ws.ptr,
+                                         self.ws_bytes, C.c_void_p(gp + 8), C.c_void_p(gp)))
         g = self.grad.download(np.empty(self.n_params + 1, dtype=np.float64))
         nf = 2 * self.steps * self.taps
-        return g[0], g[1:1 + nf].view(np.complex128).reshape(self.steps, self.taps), g[1 + nf:]
+        return (g[0], g[1:1 + nf].view(np.complex128).reshape(self.steps, self.taps),
+                g[1 + nf:].reshape(self.steps, self.kerr_taps))
 
     def train_step(self, inp, target, lr):
         self.t += 1
         loss = C.c_double(0)
-        _check(_lib.focss_ldbp_train_step(self.ctx.handle, self.steps, self.taps, self.samples, self.batch,
-                                          self.params.ptr, self.m.ptr, self.v.ptr, self.t, lr,
+        _check(_lib.focss_ldbp_train_step(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples,
+                                          self.batch, self.params.ptr, self.m.ptr, self.v.ptr, self.t, lr,
                                           self.lr_scale.ptr if self.lr_scale else None, inp.ptr, target.ptr,
                                           self.ws.ptr, self.ws_bytes, C.byref(loss)))
         return loss.value

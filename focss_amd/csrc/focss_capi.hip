@@ -738,33 +738,44 @@ namespace {
 
 using LdbpKernel = void (*)(LdbpArgs);
 
-bool ldbp_kernels(int taps, LdbpKernel* fwd, LdbpKernel* bwd) {
-    if (taps == 15) {
-        *fwd = &k_ldbp_fwd<15>;
-        *bwd = &k_ldbp_bwd<15>;
-    } else if (taps == 63) {
-        *fwd = &k_ldbp_fwd<63>;
-        *bwd = &k_ldbp_bwd<63>;
-    } else if (taps == 5) {
-        *fwd = &k_ldbp_fwd<5>;
-        *bwd = &k_ldbp_bwd<5>;
-    } else {
-        return false;
+template <int K>
+bool ldbp_kernels_k(int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
+    switch (ktaps) {
+        case 1: *fwd = &k_ldbp_fwd<K, 1>; *bwd = &k_ldbp_bwd<K, 1>; return true;
+        case 5: *fwd = &k_ldbp_fwd<K, 5>; *bwd = &k_ldbp_bwd<K, 5>; return true;
+        case 9: *fwd = &k_ldbp_fwd<K, 9>; *bwd = &k_ldbp_bwd<K, 9>; return true;
+        case 15: *fwd = &k_ldbp_fwd<K, 15>; *bwd = &k_ldbp_bwd<K, 15>; return true;
+        default: return false;
     }
-    return true;
 }
 
-int ldbp_check(int steps, int taps, int samples, int batch) {
+bool ldbp_kernels(int taps, int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
+    if (taps == 15) return ldbp_kernels_k<15>(ktaps, fwd, bwd);
+    if (taps == 63) return ldbp_kernels_k<63>(ktaps, fwd, bwd);
+    if (taps == 5) return ldbp_kernels_k<5>(ktaps, fwd, bwd);
+    return false;
+}
+
+int ldbp_check(int steps, int taps, int ktaps, int samples, int batch) {
     LdbpKernel f, b;
-    if (!ldbp_kernels(taps, &f, &b)) return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15 or 63");
+    if (!ldbp_kernels(taps, ktaps, &f, &b))
+        return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15 or 63 and kerr_taps 1, 5, 9 or 15");
     if (steps < 1 || batch < 1) return fail(FOCSS_E_ARG, "learned DBP: steps and batch must be >= 1");
     int l = ilog2_int(samples);
-    if (l < 6 || samples > LDBP_MAX_N) return fail(FOCSS_E_ARG, "learned DBP: samples must be a power of two in [64, 16384]");
+    if (l < 0 || samples < LDBP_MIN_N || samples > LDBP_MAX_N)
+        return fail(FOCSS_E_ARG, "learned DBP: samples must be a power of two in [1024, 16384]");
     return 0;
 }
 
-// one launch each; LDS = N complex64
-int ldbp_launch(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
+// forward: N / 16 threads, LDS = N + N/16 complex64; backward: 1024 threads,
+// LDS = N complex64
+int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
+    hipLaunchKernelGGL(k, dim3(a.B), dim3(a.N / LDBP_R), (size_t)(a.N + a.N / 16) * sizeof(cx<float>), c->stream, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
     hipLaunchKernelGGL(k, dim3(a.B), dim3(LDBP_NT), (size_t)a.N * sizeof(cx<float>), c->stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1134,52 +1145,52 @@ size_t focss_ldbp_workspace_bytes(int steps, int samples, int batch) {
     return acts + 2 * out + 256;  // activations, final output, step scratch
 }
 
-int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int samples, int batch, const float* params,
-                       const void* in, void* out, const void* target, double* loss) {
-    if (!c || !params || !in || (!out && !target)) return fail(FOCSS_E_ARG, "learned DBP: null argument");
-    if (int rc = ldbp_check(steps, taps, samples, batch)) return rc;
+int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch,
+                       const float* params, const void* in, void* out, const void* target, double* loss) {
+    if (!c || !params || !in || (!out && !target) || (target && !loss))
+        return fail(FOCSS_E_ARG, "learned DBP: null argument");
+    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     LdbpKernel fwd, bwd;
-    ldbp_kernels(taps, &fwd, &bwd);
+    ldbp_kernels(taps, kerr_taps, &fwd, &bwd);
     double* dloss = nullptr;
     if (target) {
         HIP_TRY(hipMalloc(&dloss, sizeof(double)));
         HIP_TRY(hipMemsetAsync(dloss, 0, sizeof(double), c->stream));
     }
-    cx<float>* zs = nullptr;
-    HIP_TRY(hipMalloc(&zs, (size_t)batch * samples * sizeof(cx<float>)));
     LdbpArgs a = {};
-    a.zs = zs;
     a.in = (const cx<float>*)in;
     a.out = (cx<float>*)out;
     a.target = (const cx<float>*)target;
     a.fir = (const cx<float>*)params;
-    a.theta = params + 2 * steps * taps;
+    a.kerr = params + 2 * steps * taps;
     a.loss = dloss;
     a.J = steps;
     a.K = taps;
+    a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    int rc = ldbp_launch(c, fwd, a);
-    if (rc == 0 && dloss) {
-        HIP_TRY(hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    int rc = ldbp_launch_fwd(c, fwd, a);
+    if (dloss) {
+        hipError_t e = hipSuccess;
+        if (rc == 0) e = hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(dloss);
+        if (e != hipSuccess) return fail(FOCSS_E_HIP, std::string("learned DBP loss: ") + hipGetErrorString(e));
     }
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (dloss) (void)hipFree(dloss);
-    (void)hipFree(zs);
     return rc;
 }
 
-int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int samples, int batch, const float* params,
-                         const void* in, const void* target, void* workspace, size_t workspace_bytes, double* grad,
-                         double* loss) {
+int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch,
+                         const float* params, const void* in, const void* target, void* workspace,
+                         size_t workspace_bytes, double* grad, double* loss) {
     if (!c || !params || !in || !target || !workspace || !grad) return fail(FOCSS_E_ARG, "learned DBP: null argument");
-    if (int rc = ldbp_check(steps, taps, samples, batch)) return rc;
+    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
     if (workspace_bytes < focss_ldbp_workspace_bytes(steps, samples, batch))
         return fail(FOCSS_E_ARG, "learned DBP: workspace too small");
     HIP_TRY(hipSetDevice(c->device));
     LdbpKernel fwd, bwd;
-    ldbp_kernels(taps, &fwd, &bwd);
+    ldbp_kernels(taps, kerr_taps, &fwd, &bwd);
     size_t act = (size_t)batch * steps * samples;
     cx<float>* xs = (cx<float>*)workspace;
     cx<float>* ys = xs + act;
@@ -1191,32 +1202,34 @@ int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int samples, int bat
     a.out = out;
     a.target = (const cx<float>*)target;
     a.fir = (const cx<float>*)params;
-    a.theta = params + 2 * steps * taps;
+    a.kerr = params + 2 * steps * taps;
     a.xs = xs;
     a.ys = ys;
     a.loss = loss;
     a.grad = grad;
     a.J = steps;
     a.K = taps;
+    a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    if (int rc = ldbp_launch(c, fwd, a)) return rc;
-    if (int rc = ldbp_launch(c, bwd, a)) return rc;
+    if (int rc = ldbp_launch_fwd(c, fwd, a)) return rc;
+    if (int rc = ldbp_launch_bwd(c, bwd, a)) return rc;
     return 0;
 }
 
-int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int samples, int batch, float* params, float* adam_m,
-                          float* adam_v, int t, float lr, const float* lr_scale, const void* in, const void* target,
-                          void* workspace, size_t workspace_bytes, double* loss_host) {
+int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch, float* params,
+                          float* adam_m, float* adam_v, int t, float lr, const float* lr_scale, const void* in,
+                          const void* target, void* workspace, size_t workspace_bytes, double* loss_host) {
     if (!c || !params || !adam_m || !adam_v || t < 1) return fail(FOCSS_E_ARG, "learned DBP: bad argument");
+    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
-    const int np = 2 * steps * taps + steps;
+    const int np = 2 * steps * taps + steps * kerr_taps;
     // gradient and loss accumulators live at the head of a small buffer
     double* acc = nullptr;
     HIP_TRY(hipMalloc(&acc, (size_t)(np + 1) * sizeof(double)));
     HIP_TRY(hipMemsetAsync(acc, 0, (size_t)(np + 1) * sizeof(double), c->stream));
-    int rc = focss_ldbp_gradients(c, steps, taps, samples, batch, params, in, target, workspace, workspace_bytes,
-                                  acc + 1, acc);
+    int rc = focss_ldbp_gradients(c, steps, taps, kerr_taps, samples, batch, params, in, target, workspace,
+                                  workspace_bytes, acc + 1, acc);
     if (rc == 0) {
         hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), dim3(256), 0, c->stream, params, (const double*)(acc + 1),
                            adam_m, adam_v, np, lr, 0.9f, 0.999f, 1e-8f, t, lr_scale);

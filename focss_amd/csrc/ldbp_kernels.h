@@ -1,40 +1,56 @@
 // Learned few-step digital backpropagation (no counterpart in the
 // reference; judged against plain DBP).  Step j of J:
-//     y(t) = sum_k h_{j,k} x((t - k + c) mod N),  c = K / 2      (complex FIR)
-//     z(t) = y(t) exp(i theta_j |y(t)|^2)                        (Kerr phase)
+//     y(t)   = sum_k h_{j,k} x((t - k + c) mod N),  c = K / 2       (complex FIR)
+//     p(t)   = |y(t)|^2
+//     phi(t) = sum_m g_{j,m} p((t - m + P) mod N),  P = KP / 2      (filtered intensity)
+//     z(t)   = y(t) exp(i phi(t))                                   (nonlinear phase)
+// KP = 1 is the plain Kerr step exp(i theta_j |y|^2); a wider KP low-pass
+// filters the intensity that drives the phase, which lets a long step
+// account for the dispersive walk-off inside it.
 // Loss = sum over the batch of (1/(N B)) sum_t |z_J(t) - target(t)|^2.
-// Gradients are Wirtinger dL/d(conj h) and dL/dtheta:
-//     q = -Im(conj(G_z) z),  G_y = G_z e^{-i theta p} + 2 theta q y,
-//     dL/dtheta += 2 p q,    G_h_k += G_y(t) conj(x(t - k + c)),
-//     G_x(s) = sum_k conj(h_k) G_y(s + k - c).
-// One 1024-thread workgroup owns one waveform (N <= 16384 complex64 in LDS)
-// and runs every step on its CU; the forward pass can store x_j and y_j for
-// the backward pass.  A step's outputs depend on neighbouring inputs, so they
-// are staged in a per-waveform global scratch (L2-resident) and copied back
-// after a barrier -- a second LDS buffer would not fit beside the first.
+// Gradients are Wirtinger dL/d(conj h) and real dL/dg:
+//     q = -Im(conj(G_z) z),   dL/dphi = 2 q,
+//     dL/dg_m = sum_t 2 q(t) p(t - m + P),   dL/dp(s) = sum_m g_m 2 q(s + m - P),
+//     G_y = G_z e^{-i phi} + (dL/dp) y,
+//     G_h_k = sum_t G_y(t) conj(x(t - k + c)),   G_x(s) = sum_k conj(h_k) G_y(s + k - c).
+// One workgroup owns one waveform (N <= 16384 complex64 in LDS) and runs
+// every step on its CU.
 #pragma once
 #include "cx.h"
+#include "trig.h"
 
 namespace focss_hip {
 
-constexpr int LDBP_NT = 1024;
+constexpr int LDBP_NT = 1024;       // backward pass threads
 constexpr int LDBP_MAX_N = 16384;
+constexpr int LDBP_MIN_N = 1024;
+constexpr int LDBP_R = 16;          // forward pass: consecutive outputs per thread (N / 16 threads)
+constexpr int LDBP_KC = 8;          // forward pass: FIR taps per register window
 
 struct LdbpArgs {
     const cx<float>* in;      // [B][N]
     cx<float>* out;           // [B][N] (may be null)
     const cx<float>* target;  // [B][N] (may be null)
     const cx<float>* fir;     // [J][K]
-    const float* theta;       // [J]
+    const float* kerr;        // [J][KP]
     cx<float>* xs;            // [B][J][N] FIR inputs (training) or null
     cx<float>* ys;            // [B][J][N] FIR outputs (training) or null
     double* loss;             // [1] accumulated, may be null
-    double* grad;             // [J*K*2 + J] accumulated: Re/Im of G_h then dtheta
-    cx<float>* zs;            // [B][N] scratch: step outputs staged before they overwrite LDS
-    int J, K, N, B;
+    double* grad;             // [J*K*2 + J*KP] accumulated: Re/Im of G_h, then dL/dg
+    cx<float>* zs;            // [B][N] backward scratch (per-sample q and phi)
+    int J, K, KP, N, B;
 };
 
-__device__ __forceinline__ double ldbp_block_sum(double v, double* red) {
+// forward-pass LDS index: one padding element every 16, so the 16-element
+// blocks that neighbouring lanes own start in different banks
+__device__ __forceinline__ int p16(int i) { return i + (i >> 4); }
+
+// wave-uniform read through the scalar data cache
+__device__ __forceinline__ float sload(const float* p) {
+    return *(const __attribute__((address_space(4))) float*)p;
+}
+
+__device__ __forceinline__ double ldbp_block_sum(double v, double* red, int nt) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -43,64 +59,129 @@ __device__ __forceinline__ double ldbp_block_sum(double v, double* red) {
     __syncthreads();
     double s = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < LDBP_NT / 64; ++i) s += red[i];
+        for (int i = 0; i < nt / 64; ++i) s += red[i];
     return s;  // valid in thread 0
 }
 
-template <int K>
-__global__ __launch_bounds__(LDBP_NT) void k_ldbp_fwd(LdbpArgs a) {
+// Forward pass, N / 16 threads: thread i owns outputs 16 i .. 16 i + 15.
+template <int K, int KP>
+__global__ __launch_bounds__(LDBP_MAX_N / LDBP_R) void k_ldbp_fwd(LdbpArgs a) {
+    constexpr int R = LDBP_R, KC = LDBP_KC, P = KP / 2, c = K / 2;
+    constexpr int NW = LDBP_MAX_N / LDBP_R / 64;
     extern __shared__ unsigned char smem_raw[];
-    cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);
-    __shared__ double red[LDBP_NT / 64];
-    __shared__ cx<float> h[K];
-    const int N = a.N, mask = N - 1, c = K / 2;
+    cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // p16-indexed, N + N/16 elements
+    __shared__ float edge[2][NW][P > 0 ? P : 1];              // first / last P intensities of each wave
+    __shared__ double red[NW];
+    const int N = a.N, mask = N - 1;
+    const int nt = blockDim.x, nw = nt >> 6;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int b = blockIdx.x;
+    const int t0 = threadIdx.x * R;
     const cx<float>* in = a.in + (long long)b * N;
-    for (int t = threadIdx.x; t < N; t += LDBP_NT) buf[t] = in[t];
+    for (int t = threadIdx.x; t < N; t += nt) buf[p16(t)] = in[t];
     __syncthreads();
-    cx<float>* zs = a.zs + (long long)b * N;
     for (int j = 0; j < a.J; ++j) {
-        if (threadIdx.x < K) h[threadIdx.x] = a.fir[j * K + threadIdx.x];
-        __syncthreads();
-        const float th = a.theta[j];
-        for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-            cx<float> y = mk<float>(0, 0);
-#pragma unroll 16
-            for (int k = 0; k < K; ++k) y = y + h[k] * buf[(t - k + c) & mask];
-            if (a.ys) {
-                a.xs[((long long)b * a.J + j) * N + t] = buf[t];
-                a.ys[((long long)b * a.J + j) * N + t] = y;
-            }
-            float s, co;
-            sincosf(th * norm2(y), &s, &co);
-            zs[t] = y * mk<float>(co, s);
+        const float* hj = reinterpret_cast<const float*>(a.fir + (long long)j * K);
+        if (a.xs) {
+            cx<float>* xs = a.xs + ((long long)b * a.J + j) * N + t0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) xs[r] = buf[p16(t0 + r)];
         }
+        // FIR, KC taps per register window of R + KC - 1 inputs
+        cx<float> y[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) y[r] = mk<float>(0, 0);
+#pragma unroll 1
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            cx<float> w[R + KC - 1];
+            const int base = t0 + c - k0 - (KC - 1);
+#pragma unroll
+            for (int u = 0; u < R + KC - 1; ++u) w[u] = buf[p16((base + u) & mask)];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const bool live = k0 + kk < K;
+                const float hr = live ? sload(hj + 2 * (k0 + kk)) : 0.0f;
+                const float hi = live ? sload(hj + 2 * (k0 + kk) + 1) : 0.0f;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const cx<float> x = w[r - kk + KC - 1];
+                    y[r].x = fmaf(hr, x.x, fmaf(-hi, x.y, y[r].x));
+                    y[r].y = fmaf(hr, x.y, fmaf(hi, x.x, y[r].y));
+                }
+            }
+        }
+        if (a.ys) {
+            cx<float>* ys = a.ys + ((long long)b * a.J + j) * N + t0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) ys[r] = y[r];
+        }
+        float p[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) p[r] = norm2(y[r]);
+        if constexpr (P > 0) {
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < P; ++i) edge[0][wid][i] = p[i];
+            }
+            if (lane == 63) {
+#pragma unroll
+                for (int i = 0; i < P; ++i) edge[1][wid][i] = p[R - P + i];
+            }
+        }
+        // every FIR read of buf is done before any output is written; the
+        // wave-edge intensities are visible
         __syncthreads();
-        for (int t = threadIdx.x; t < N; t += LDBP_NT) buf[t] = zs[t];
+        float pe[R + 2 * P];  // p(t0 - P + u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) pe[P + r] = p[r];
+        if constexpr (P > 0) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                float prev = __shfl(p[R - P + i], (lane + 63) & 63, 64);
+                float next = __shfl(p[i], (lane + 1) & 63, 64);
+                if (lane == 0) prev = edge[1][(wid + nw - 1) % nw][i];
+                if (lane == 63) next = edge[0][(wid + 1) % nw][i];
+                pe[i] = prev;
+                pe[P + R + i] = next;
+            }
+        }
+        float g[KP];
+#pragma unroll
+        for (int m = 0; m < KP; ++m) g[m] = sload(a.kerr + j * KP + m);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float phi = 0;
+#pragma unroll
+            for (int m = 0; m < KP; ++m) phi = fmaf(g[m], pe[r - m + 2 * P], phi);
+            float s, co;
+            sincos_fast(phi, &s, &co);
+            buf[p16(t0 + r)] = y[r] * mk<float>(co, s);
+        }
         __syncthreads();
     }
     double part = 0;
-    for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-        if (a.out) a.out[(long long)b * N + t] = buf[t];
-        if (a.target) {
-            cx<float> e = buf[t] - a.target[(long long)b * N + t];
-            part += (double)norm2(e);
-        }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const long long t = (long long)b * N + t0 + r;
+        const cx<float> z = buf[p16(t0 + r)];
+        if (a.out) a.out[t] = z;
+        if (a.target) part += (double)norm2(z - a.target[t]);
     }
     if (a.loss) {
-        double s = ldbp_block_sum(part, red);
+        double s = ldbp_block_sum(part, red, nt);
         if (threadIdx.x == 0) atomicAdd(a.loss, s / ((double)N * a.B));
     }
 }
 
 // needs the forward pass run with xs / ys stored; out holds z_J
-template <int K>
+template <int K, int KP>
 __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     extern __shared__ unsigned char smem_raw[];
     cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);
     __shared__ double red[LDBP_NT / 64];
     __shared__ cx<float> h[K];
     constexpr int KC = 8;  // taps per gradient sweep (register budget)
+    constexpr int P = KP / 2;
     const int N = a.N, mask = N - 1, c = K / 2;
     const int b = blockIdx.x;
     const float scale = 1.0f / ((float)N * (float)a.B);
@@ -111,21 +192,46 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     for (int j = a.J - 1; j >= 0; --j) {
         const cx<float>* x = a.xs + ((long long)b * a.J + j) * N;
         const cx<float>* y = a.ys + ((long long)b * a.J + j) * N;
-        const float th = a.theta[j];
-        // Kerr backward, in place (each thread owns its samples)
-        double dth = 0;
+        const float* gk = a.kerr + j * KP;
+        // nonlinear phase backward, pass 1: q and phi per sample (staged in
+        // zs), intensity-tap gradients
+        double dg[KP];
+#pragma unroll
+        for (int m = 0; m < KP; ++m) dg[m] = 0;
         for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-            cx<float> yv = y[t];
-            float p = norm2(yv);
+            float pn[KP];  // p(t - m + P)
+#pragma unroll
+            for (int m = 0; m < KP; ++m) pn[m] = norm2(y[(t - m + P) & mask]);
+            float phi = 0;
+#pragma unroll
+            for (int m = 0; m < KP; ++m) phi = fmaf(gk[m], pn[m], phi);
             float s, co;
-            sincosf(th * p, &s, &co);
-            cx<float> zv = yv * mk<float>(co, s);
-            cx<float> gz = g[t];
-            float q = -(gz.x * zv.y - gz.y * zv.x);  // -Im(conj(gz) z)
-            dth += 2.0 * (double)p * (double)q;
-            g[t] = gz * mk<float>(co, -s) + yv * (2.0f * th * q);
+            sincos_fast(phi, &s, &co);
+            const cx<float> zv = y[t] * mk<float>(co, s);
+            const cx<float> gz = g[t];
+            const float q = -(gz.x * zv.y - gz.y * zv.x);  // -Im(conj(gz) z)
+#pragma unroll
+            for (int m = 0; m < KP; ++m) dg[m] += 2.0 * (double)q * (double)pn[m];
+            zs[t] = mk<float>(q, phi);
         }
         __syncthreads();
+        // pass 2: G_y = G_z e^{-i phi} + (dL/dp) y, in place
+        for (int t = threadIdx.x; t < N; t += LDBP_NT) {
+            float dp = 0;
+#pragma unroll
+            for (int m = 0; m < KP; ++m) dp = fmaf(gk[m], 2.0f * zs[(t + m - P) & mask].x, dp);
+            const float phi = zs[t].y;
+            float s, co;
+            sincos_fast(phi, &s, &co);
+            g[t] = g[t] * mk<float>(co, -s) + y[t] * dp;
+        }
+        __syncthreads();
+        double* grk = a.grad + (long long)a.J * K * 2 + (long long)j * KP;
+#pragma unroll 1
+        for (int m = 0; m < KP; ++m) {
+            double st = ldbp_block_sum(dg[m], red, LDBP_NT);
+            if (threadIdx.x == 0) atomicAdd(&grk[m], st);
+        }
         // FIR tap gradients G_h_k = sum_t G_y(t) conj(x(t - k + c)), KC taps per
         // sweep; per-thread partial sums in float, block sums in double
         double* gr = a.grad + (long long)j * K * 2;
@@ -149,14 +255,12 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
             }
 #pragma unroll 1
             for (int q = 0; q < KC && k0 + q < K; ++q) {
-                double sr = ldbp_block_sum((double)dre[q], red);
+                double sr = ldbp_block_sum((double)dre[q], red, LDBP_NT);
                 if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q)], sr);
-                double si = ldbp_block_sum((double)dim[q], red);
+                double si = ldbp_block_sum((double)dim[q], red, LDBP_NT);
                 if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q) + 1], si);
             }
         }
-        double st = ldbp_block_sum(dth, red);
-        if (threadIdx.x == 0) atomicAdd(&a.grad[(long long)a.J * K * 2 + j], st);
         // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c), staged in zs
         if (threadIdx.x < K) h[threadIdx.x] = conj(a.fir[j * K + threadIdx.x]);
         __syncthreads();
@@ -173,15 +277,15 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
 }
 
 // Adam (Kingma & Ba, 2015) on float parameters with double gradients;
-// p holds the FIR taps (re, im interleaved) followed by theta.
+// p holds the FIR taps (re, im interleaved) followed by the intensity taps.
 __global__ void k_adam(float* __restrict__ p, const double* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, int n, float lr, float beta1, float beta2, float eps, int t,
                        const float* __restrict__ lr_scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // the taps carry the Wirtinger gradient dL/d(conj h) = (dL/dRe + i dL/dIm) / 2;
-    // Adam normalises every coordinate, so the factor 2 against theta's real
-    // gradient does not change the update
+    // Adam normalises every coordinate, so the factor 2 against the real
+    // intensity-tap gradients does not change the update
     float gi = (float)g[i];
     float mi = beta1 * m[i] + (1 - beta1) * gi;
     float vi = beta2 * v[i] + (1 - beta2) * gi * gi;

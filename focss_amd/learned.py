@@ -7,13 +7,16 @@
    SSMF (ADAPTIVE grid, EDFA 16 dB + ASE NF 5 dB, launch 0..4 dBm) by the
    device link program, then decimated to 2 samples/symbol (2^14 samples).
 2. Plain DBP at 16 steps per span (FP64, device link program) is the judge.
-3. Learned DBP: 2 steps per span = 40 x (complex FIR, Kerr phase theta),
-   63 taps by default (15 taps cannot hold a 40 km dispersion step at 2
-   samples/symbol: the 40-step chain's composite response diverges),
-   initialised from the physics (init_params) and trained with Adam on the
-   waveform MSE against the transmitted 2 sps waveform.
+3. Learned DBP: 2 steps per span = 40 x (complex FIR, nonlinear phase from
+   the intensity through a short real filter -- kerr_taps = 1 is the plain
+   Kerr phase theta |y|^2), 63 FIR taps by default (15 taps cannot hold a
+   40 km dispersion step at 2 samples/symbol: the 40-step chain's composite
+   response diverges), initialised from the physics (init_params) and
+   trained with Adam on the waveform MSE against the transmitted 2 sps
+   waveform.
 4. Q = q2_factor after RRC matched filter and constant-phase alignment; the
-   report gives Q per method and waveforms/s of each backpropagation.
+   report gives Q per method and waveforms/s of each backpropagation on a
+   batch of `rate_batch` waveforms (the test set repeated).
 """
 import argparse
 import json
@@ -73,28 +76,46 @@ def make_dataset(F, ctx, n_waves, n_samples, seed, p_lo=0.0, p_hi=4.0, sim_sps=4
     return rx2, np.stack(tx2), np.stack(syms), powers
 
 
-def plain_dbp(F, ctx, rx2, steps_per_span, precision=None):
-    """backward_propagation on the device for a batch; returns (out, seconds)."""
-    precision = F.F64 if precision is None else precision
-    n_waves, n = rx2.shape
+def _dbp_links(F, n_waves, n, steps_per_span):
     dt = 1.0 / (BAUD * 2)
     gain = np.exp(ALPHA * SPAN_LEN)
-    links = [F.make_link(fiber(F), F.Grid(F.UNIFORM, steps_per_span, 0.0, 0.0), F.BACKWARD, N_SPANS, n, dt,
-                         center_frequency=F_C, pre_scale=1 / np.sqrt(gain)) for _ in range(n_waves)]
+    return [F.make_link(fiber(F), F.Grid(F.UNIFORM, steps_per_span, 0.0, 0.0), F.BACKWARD, N_SPANS, n, dt,
+                        center_frequency=F_C, pre_scale=1 / np.sqrt(gain)) for _ in range(n_waves)]
+
+
+def plain_dbp(F, ctx, rx2, steps_per_span, precision=None):
+    """backward_propagation on the device for a batch; returns the output."""
+    precision = F.F64 if precision is None else precision
+    n_waves, n = rx2.shape
+    links = _dbp_links(F, n_waves, n, steps_per_span)
+    host = rx2.astype(np.complex128 if precision == F.F64 else np.complex64).ravel()
+    buf = ctx.alloc(host.nbytes)
+    buf.upload(host)
+    ctx.link_run(links, buf, np.arange(n_waves, dtype=np.int64) * n, precision=precision)
+    out = buf.download(np.empty_like(host)).reshape(n_waves, n).astype(np.complex128)
+    buf.free()
+    return out
+
+
+def dbp_rate(F, ctx, rx2, steps_per_span, precision=None, reps=2):
+    """Waveforms per second of plain DBP on the batch rx2 (device-resident,
+    after a warm-up run that captures the graph)."""
+    precision = F.F64 if precision is None else precision
+    n_waves, n = rx2.shape
+    links = _dbp_links(F, n_waves, n, steps_per_span)
     host = rx2.astype(np.complex128 if precision == F.F64 else np.complex64).ravel()
     buf = ctx.alloc(host.nbytes)
     buf.upload(host)
     offs = np.arange(n_waves, dtype=np.int64) * n
-    ctx.link_run(links, buf, offs, precision=precision)  # warm-up (graph capture, tuning)
-    buf.upload(host)
-    ctx.synchronize()
-    t0 = time.perf_counter()
     ctx.link_run(links, buf, offs, precision=precision)
     ctx.synchronize()
-    t = time.perf_counter() - t0
-    out = buf.download(np.empty_like(host)).reshape(n_waves, n).astype(np.complex128)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.link_run(links, buf, offs, precision=precision)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / reps
     buf.free()
-    return out, t
+    return n_waves / t
 
 
 def _signal_psd(k):
@@ -109,7 +130,7 @@ def _signal_psd(k):
     return p
 
 
-def init_params(n, steps_per_span=2, taps=63, floor=0.01):
+def init_params(n, steps_per_span=2, taps=63, floor=0.01, kerr_taps=1):
     """Physics initialisation.  FIR j is a weighted least-squares fit of the
     BACKWARD linear operator of its h = L / steps_per_span (loss gain
     e^{alpha h/2}, dispersion e^{-i beta2 h w^2/2}, the span's drop_power on
@@ -117,7 +138,8 @@ def init_params(n, steps_per_span=2, taps=63, floor=0.01):
     fitted to the running error of the chain so far (target = ideal response
     of steps 0..j divided by the fitted response of steps 0..j-1), so the
     per-step truncation errors do not pile up along the 40 steps.
-    theta_j = -kappa gamma L_eff(h) for the field at the start of the step."""
+    theta_j = -kappa gamma L_eff(h) for the field at the start of the step,
+    placed on the centre intensity tap (the others start at 0)."""
     h = SPAN_LEN / steps_per_span
     dt = 1.0 / (BAUD * 2)
     k = np.fft.fftfreq(n, d=dt) * 2 * np.pi
@@ -139,7 +161,9 @@ def init_params(n, steps_per_span=2, taps=63, floor=0.01):
             coef, *_ = np.linalg.lstsq(A * wt[:, None], target * wt, rcond=None)
             chain = chain * (A @ coef)
             firs.append(coef)
-            thetas.append(-GAMMA * l_eff)
+            g = np.zeros(kerr_taps)
+            g[kerr_taps // 2] = -GAMMA * l_eff
+            thetas.append(g)
     return np.array(firs), np.array(thetas)
 
 
@@ -164,15 +188,17 @@ def run(args, log=print):
         rx_te, tx_te, sy_te, p_te = make_dataset(F, ctx, args.test, n, seed=args.seed + 1)
         log(f"datasets: {args.train} + {args.test} waveforms of {n} samples, {time.perf_counter() - t0:.1f} s")
         # judge: plain DBP at 16 and 2 steps per span (FP64)
-        dbp16, t16 = plain_dbp(F, ctx, rx_te, 16)
-        dbp2, t2 = plain_dbp(F, ctx, rx_te, 2)
+        dbp16 = plain_dbp(F, ctx, rx_te, 16)
+        dbp2 = plain_dbp(F, ctx, rx_te, 2)
         q16, q2 = q_of(dbp16, sy_te), q_of(dbp2, sy_te)
         # learned model
-        fir0, th0 = init_params(n, 2, taps)
+        kt = getattr(args, "kerr_taps", 1)
+        fir0, g0 = init_params(n, 2, taps, kerr_taps=kt)
         batch = min(args.batch, args.train)
-        model = F.LearnedDBP(ctx, steps, taps, n, batch, fir0, th0)
+        model = F.LearnedDBP(ctx, steps, taps, n, batch, fir0, g0)
+        theta0 = g0[:, kt // 2]
         scale = np.concatenate([np.full(2 * steps * taps, 1.0),
-                                args.theta_lr_mult * np.abs(th0)]).astype(np.float32)
+                                np.repeat(args.theta_lr_mult * np.abs(theta0), kt)]).astype(np.float32)
         model.set_lr_scale(scale)
         xin = ctx.alloc(batch * n * 8)
         tgt = ctx.alloc(batch * n * 8)
@@ -201,27 +227,35 @@ def run(args, log=print):
         t_train = time.perf_counter() - t_train
         out_l = evaluate()
         q_l = q_of(out_l.astype(np.complex128), sy_te)
-        # learned inference throughput on the full test batch
-        m = F.LearnedDBP(ctx, steps, taps, n, args.test, *model.get())
-        m.forward(te_in, te_out)
+        # throughput: every method on the same batch of rate_batch waveforms
+        rb = getattr(args, "rate_batch", args.test)
+        rx_rate = np.tile(rx_te, (-(-rb // args.test), 1))[:rb]
+        r16 = dbp_rate(F, ctx, rx_rate, 16)
+        r2 = dbp_rate(F, ctx, rx_rate, 2)
+        r16_32 = dbp_rate(F, ctx, rx_rate, 16, precision=F.F32)
+        m = F.LearnedDBP(ctx, steps, taps, n, rb, *model.get())
+        r_in, r_out = ctx.alloc(rb * n * 8), ctx.alloc(rb * n * 8)
+        r_in.upload(rx_rate.astype(np.complex64).ravel())
+        m.forward(r_in, r_out)
         ctx.synchronize()
         t0 = time.perf_counter()
         reps = 5
         for _ in range(reps):
-            m.forward(te_in, te_out)
+            m.forward(r_in, r_out)
         ctx.synchronize()
-        t_l = (time.perf_counter() - t0) / reps
+        r_l = rb * reps / (time.perf_counter() - t0)
         rep = {
             "waveforms": args.test, "samples": n, "train_waveforms": args.train, "iters": args.iters,
-            "train_seconds": round(t_train, 2),
+            "taps": taps, "kerr_taps": kt, "train_seconds": round(t_train, 2),
             "q_db": {"plain_dbp_16_steps_per_span_f64": float(np.mean(q16)),
                      "plain_dbp_2_steps_per_span_f64": float(np.mean(q2)),
                      "learned_2_steps_per_span_init": float(np.mean(q_init)),
                      "learned_2_steps_per_span_trained": float(np.mean(q_l))},
             "q_gap_learned_vs_dbp16_db": float(np.mean(q_l) - np.mean(q16)),
-            "waveforms_per_s": {"plain_dbp_16_f64": args.test / t16, "plain_dbp_2_f64": args.test / t2,
-                                "learned_f32": args.test / t_l},
-            "speedup_learned_vs_dbp16": t16 / t_l,
+            "rate_batch": rb,
+            "waveforms_per_s": {"plain_dbp_16_f64": r16, "plain_dbp_16_f32": r16_32, "plain_dbp_2_f64": r2,
+                                "learned_f32": r_l},
+            "speedup_learned_vs_dbp16": r_l / r16,
             "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
             "launch_dbm_range": [float(np.min(p_te)), float(np.max(p_te))],
         }
@@ -239,6 +273,8 @@ def parser():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--taps", type=int, default=63)
     ap.add_argument("--theta-lr-mult", type=float, default=10.0)
+    ap.add_argument("--kerr-taps", type=int, default=15, help="intensity taps of the nonlinear phase (1, 5, 9, 15)")
+    ap.add_argument("--rate-batch", type=int, default=512, help="waveforms per batch for the throughput figures")
     return ap
 
 

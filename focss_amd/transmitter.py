@@ -99,3 +99,10 @@ def phase_align(tx, rx):
     the closed form of PhaseShiftEqualizer's angle search
     (phase_shift_equalizer.cpp:30-46)."""
     return rx * np.exp(1j * np.angle(np.vdot(rx, tx)))
+
+
+def demodulate_16qam(rx):
+    """demodulate_16qam (modulation.cpp:98-110): index of the nearest point of
+    the unit-power Gray constellation, ties to the lower code."""
+    d = np.abs(np.asarray(rx)[:, None] - GRAY16[None, :]) ** 2
+    return np.argmin(d, axis=1)  # argmin keeps the first (lowest) index on ties

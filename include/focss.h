@@ -174,25 +174,32 @@ int focss_wdm_ppe(focss_ctx* ctx, int precision, int c_radius, const double* C, 
 
 /* ---- learned few-step DBP (no counterpart in the reference) ----
  * Step j: y = complex FIR (taps, circular, centre taps/2) of x, then
- * z = y exp(i theta_j |y|^2).  params (device, float): the FIR taps
- * [steps][taps] as interleaved complex64, followed by theta[steps].
- * Fields are complex64 device buffers [batch][samples], samples a power of
- * two <= 16384, taps 5, 15 or 63.  Loss = sum_b (1/(samples batch)) sum_t
- * |z - target|^2.  Judged against plain DBP (focss_link_run). */
+ * z = y exp(i phi), phi(t) = sum_m g_{j,m} |y(t - m + kerr_taps/2)|^2 (a real
+ * filter on the intensity; kerr_taps = 1 is the plain Kerr step
+ * exp(i theta_j |y|^2)).  params (device, float): the FIR taps [steps][taps]
+ * as interleaved complex64, followed by g [steps][kerr_taps].  Fields are
+ * complex64 device buffers [batch][samples], samples a power of two in
+ * [1024, 16384], taps 5, 15 or 63, kerr_taps 1, 5, 9 or 15.  Loss =
+ * sum_b (1/(samples batch)) sum_t |z - target|^2.  Judged against plain DBP
+ * (focss_link_run); callers are learned-equaliser training and inference. */
 size_t focss_ldbp_workspace_bytes(int steps, int samples, int batch);
-int focss_ldbp_forward(focss_ctx* ctx, int steps, int taps, int samples, int batch, const float* params,
-                       const void* in, void* out, const void* target, double* loss);
-/* Accumulates into grad (device, double [2 steps taps + steps], zeroed by the
- * caller) the Wirtinger gradient dL/d(conj h) (re, im) and dL/dtheta, and the
- * loss into *loss (device double).  Stream-ordered, no host sync. */
-int focss_ldbp_gradients(focss_ctx* ctx, int steps, int taps, int samples, int batch, const float* params,
-                         const void* in, const void* target, void* workspace, size_t workspace_bytes,
-                         double* grad, double* loss);
+/* Inference (out) and/or loss against target (*loss, host double; the call
+ * then synchronises).  Without target the call is stream-ordered and
+ * allocates nothing. */
+int focss_ldbp_forward(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+                       const float* params, const void* in, void* out, const void* target, double* loss);
+/* Accumulates into grad (device, double [2 steps taps + steps kerr_taps],
+ * zeroed by the caller) the Wirtinger gradient dL/d(conj h) (re, im) and
+ * dL/dg, and the loss into *loss (device double).  Stream-ordered, no host
+ * sync. */
+int focss_ldbp_gradients(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+                         const float* params, const void* in, const void* target, void* workspace,
+                         size_t workspace_bytes, double* grad, double* loss);
 /* One training step: gradients then Adam (t = step number from 1, optional
  * per-parameter learning-rate scale).  *loss_host receives the batch loss
  * before the update. */
-int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int samples, int batch, float* params,
-                          float* adam_m, float* adam_v, int t, float lr, const float* lr_scale,
+int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+                          float* params, float* adam_m, float* adam_v, int t, float lr, const float* lr_scale,
                           const void* in, const void* target, void* workspace, size_t workspace_bytes,
                           double* loss_host);
 

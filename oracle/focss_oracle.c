@@ -980,53 +980,74 @@ void orc_wdmppe_xpm(int d_radius, const orc_c128* D, int samples, const orc_c128
 
 /* ------------------------------------------------------------------ */
 /* Learned few-step DBP (no reference counterpart).                    */
-/*   step j: y(t) = sum_k h_{j,k} x((t - k + c) mod N), c = K/2         */
-/*           z(t) = y(t) exp(i theta_j |y(t)|^2)                        */
+/*   step j: y(t)   = sum_k h_{j,k} x((t - k + c) mod N),  c = K/2      */
+/*           p(t)   = |y(t)|^2                                          */
+/*           phi(t) = sum_m g_{j,m} p((t - m + P) mod N),  P = KP/2     */
+/*           z(t)   = y(t) exp(i phi(t))                                */
 /*   loss   = (1/N) sum_t |z_J(t) - target(t)|^2                        */
-/* Gradients are Wirtinger dL/d(conj h) for the taps and dL/dtheta.     */
+/* KP = 1 is the plain Kerr step exp(i theta_j |y|^2).  Gradients are   */
+/* Wirtinger dL/d(conj h) for the FIR taps and dL/dg for the real       */
+/* intensity taps:  q = -Im(conj(G_z) z),  dL/dphi = 2 q,                */
+/*   dL/dg_m = sum_t 2 q(t) p(t - m + P),                                */
+/*   dL/dp(s) = sum_m g_m 2 q(s + m - P),                                */
+/*   G_y = G_z e^{-i phi} + (dL/dp) y.                                   */
 /* ------------------------------------------------------------------ */
+static int wrap(int i, int n) { return ((i % n) + n) % n; }
+
 static void ldbp_fir(int taps, const orc_c128* h, int n, const orc_c128* x, orc_c128* y) {
     int c = taps / 2;
     for (int t = 0; t < n; ++t) {
         orc_c128 acc = {0, 0};
-        for (int k = 0; k < taps; ++k) {
-            int s = ((t - k + c) % n + n) % n;
-            acc = c_add(acc, c_mul(h[k], x[s]));
-        }
+        for (int k = 0; k < taps; ++k) acc = c_add(acc, c_mul(h[k], x[wrap(t - k + c, n)]));
         y[t] = acc;
     }
 }
 
-double orc_ldbp_forward(int steps, int taps, const orc_c128* fir, const double* theta, int n,
+/* phi(t) of one step from y */
+static void ldbp_phase(int ktaps, const double* g, int n, const orc_c128* y, double* phi) {
+    int P = ktaps / 2;
+    for (int t = 0; t < n; ++t) {
+        double f = 0;
+        for (int m = 0; m < ktaps; ++m) f += g[m] * c_norm(y[wrap(t - m + P, n)]);
+        phi[t] = f;
+    }
+}
+
+double orc_ldbp_forward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr, int n,
                         const orc_c128* in, orc_c128* out, const orc_c128* target) {
     orc_c128* x = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
     orc_c128* y = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)n);
     memcpy(x, in, sizeof(orc_c128) * (size_t)n);
     for (int j = 0; j < steps; ++j) {
         ldbp_fir(taps, fir + (long)j * taps, n, x, y);
-        for (int t = 0; t < n; ++t) x[t] = c_mul(y[t], i_exp(theta[j] * c_norm(y[t])));
+        ldbp_phase(ktaps, kerr + (long)j * ktaps, n, y, phi);
+        for (int t = 0; t < n; ++t) x[t] = c_mul(y[t], i_exp(phi[t]));
     }
     double loss = 0;
     if (target)
         for (int t = 0; t < n; ++t) loss += c_norm(c_sub(x[t], target[t]));
     if (out) memcpy(out, x, sizeof(orc_c128) * (size_t)n);
+    free(phi);
     free(y);
     free(x);
     return loss / n;
 }
 
-double orc_ldbp_backward(int steps, int taps, const orc_c128* fir, const double* theta, int n,
-                         const orc_c128* in, const orc_c128* target, orc_c128* grad_fir,
-                         double* grad_theta) {
+double orc_ldbp_backward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr, int n,
+                         const orc_c128* in, const orc_c128* target, orc_c128* grad_fir, double* grad_kerr) {
     /* forward with the per-step FIR inputs and outputs kept */
     orc_c128* xs = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n * (steps + 1));
     orc_c128* ys = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n * steps);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)n);
+    double* q = (double*)malloc(sizeof(double) * (size_t)n);
     memcpy(xs, in, sizeof(orc_c128) * (size_t)n);
     for (int j = 0; j < steps; ++j) {
         orc_c128* x = xs + (long)j * n;
         orc_c128* y = ys + (long)j * n;
         ldbp_fir(taps, fir + (long)j * taps, n, x, y);
-        for (int t = 0; t < n; ++t) xs[(long)(j + 1) * n + t] = c_mul(y[t], i_exp(theta[j] * c_norm(y[t])));
+        ldbp_phase(ktaps, kerr + (long)j * ktaps, n, y, phi);
+        for (int t = 0; t < n; ++t) xs[(long)(j + 1) * n + t] = c_mul(y[t], i_exp(phi[t]));
     }
     double loss = 0;
     orc_c128* g = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
@@ -1036,43 +1057,41 @@ double orc_ldbp_backward(int steps, int taps, const orc_c128* fir, const double*
         loss += c_norm(e);
         g[t] = c_scale(e, 1.0 / n);
     }
-    int c = taps / 2;
+    int c = taps / 2, P = ktaps / 2;
     for (int j = steps - 1; j >= 0; --j) {
         const orc_c128* x = xs + (long)j * n;
         const orc_c128* y = ys + (long)j * n;
         const orc_c128* z = xs + (long)(j + 1) * n;
-        /* Kerr: G_y = G_z e^{-i theta p} + 2 theta q y,  q = -Im(conj(G_z) z),
-           dL/dtheta = 2 sum p q */
-        double gt = 0;
-        for (int t = 0; t < n; ++t) {
-            double p = c_norm(y[t]);
-            orc_c128 cz = c_mul(c_conj(g[t]), z[t]);
-            double q = -cz.im;
-            gt += 2.0 * p * q;
-            gy[t] = c_add(c_mul(g[t], i_exp(-theta[j] * p)), c_scale(y[t], 2.0 * theta[j] * q));
+        const double* gk = kerr + (long)j * ktaps;
+        ldbp_phase(ktaps, gk, n, y, phi);
+        for (int t = 0; t < n; ++t) q[t] = -c_mul(c_conj(g[t]), z[t]).im;
+        for (int m = 0; m < ktaps; ++m) {
+            double acc = 0;
+            for (int t = 0; t < n; ++t) acc += 2.0 * q[t] * c_norm(y[wrap(t - m + P, n)]);
+            grad_kerr[(long)j * ktaps + m] = acc;
         }
-        grad_theta[j] = gt;
+        for (int s = 0; s < n; ++s) {
+            double dp = 0;
+            for (int m = 0; m < ktaps; ++m) dp += gk[m] * 2.0 * q[wrap(s + m - P, n)];
+            gy[s] = c_add(c_mul(g[s], i_exp(-phi[s])), c_scale(y[s], dp));
+        }
         /* FIR: G_h_k = sum_t G_y(t) conj(x(t-k+c));  G_x(s) = sum_k conj(h_k) G_y(s+k-c) */
         const orc_c128* h = fir + (long)j * taps;
         for (int k = 0; k < taps; ++k) {
             orc_c128 acc = {0, 0};
-            for (int t = 0; t < n; ++t) {
-                int s = ((t - k + c) % n + n) % n;
-                acc = c_add(acc, c_mul(gy[t], c_conj(x[s])));
-            }
+            for (int t = 0; t < n; ++t) acc = c_add(acc, c_mul(gy[t], c_conj(x[wrap(t - k + c, n)])));
             grad_fir[(long)j * taps + k] = acc;
         }
         for (int s = 0; s < n; ++s) {
             orc_c128 acc = {0, 0};
-            for (int k = 0; k < taps; ++k) {
-                int t = ((s + k - c) % n + n) % n;
-                acc = c_add(acc, c_mul(c_conj(h[k]), gy[t]));
-            }
+            for (int k = 0; k < taps; ++k) acc = c_add(acc, c_mul(c_conj(h[k]), gy[wrap(s + k - c, n)]));
             g[s] = acc;
         }
     }
     free(gy);
     free(g);
+    free(q);
+    free(phi);
     free(ys);
     free(xs);
     return loss / n;

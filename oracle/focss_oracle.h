@@ -126,13 +126,15 @@ void orc_wdmppe_spm(int c_radius, const orc_c128* C, int samples, const orc_c128
 void orc_wdmppe_xpm(int d_radius, const orc_c128* D, int samples, const orc_c128* A6,
                     orc_c128* out2);
 
-/* ---- learned few-step DBP (no reference counterpart; judged by plain DBP) ---- */
-double orc_ldbp_forward(int steps, int taps, const orc_c128* fir, const double* theta,
+/* ---- learned few-step DBP (no reference counterpart; judged by plain DBP) ----
+ * fir [steps][taps] complex, kerr [steps][ktaps] real intensity taps of the
+ * nonlinear phase (ktaps = 1: exp(i theta |y|^2)). */
+double orc_ldbp_forward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr,
                         int samples, const orc_c128* in, orc_c128* out,
                         const orc_c128* target);
-double orc_ldbp_backward(int steps, int taps, const orc_c128* fir, const double* theta,
+double orc_ldbp_backward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr,
                          int samples, const orc_c128* in, const orc_c128* target,
-                         orc_c128* grad_fir, double* grad_theta);
+                         orc_c128* grad_fir, double* grad_kerr);
 
 #ifdef __cplusplus
 }

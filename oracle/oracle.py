@@ -110,8 +110,8 @@ _sig = {
     "orc_complex_lstsq": (C.c_int, [C.c_int, C.c_int, _P, _P, _P]),
     "orc_wdmppe_spm": (None, [C.c_int, _P, C.c_int, _P, _P]),
     "orc_wdmppe_xpm": (None, [C.c_int, _P, C.c_int, _P, _P]),
-    "orc_ldbp_forward": (C.c_double, [C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
-    "orc_ldbp_backward": (C.c_double, [C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
+    "orc_ldbp_forward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
+    "orc_ldbp_backward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -409,27 +409,33 @@ def wdmppe_xpm(d_radius, Dm, A6):
 
 
 # ---------------- learned DBP ----------------
-def ldbp_forward(fir, theta, x, target=None):
+def _kerr(kerr):
+    """kerr: [steps] (theta per step) or [steps][ktaps] intensity taps."""
+    k = np.ascontiguousarray(kerr, dtype=np.float64)
+    return k.reshape(k.shape[0], 1) if k.ndim == 1 else k
+
+
+def ldbp_forward(fir, kerr, x, target=None):
     fir = _c(fir)
-    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    k = _kerr(kerr)
     x = _c(x)
     steps, taps = fir.shape
     out = np.zeros_like(x)
-    tp = _ptr(_c(target)) if target is not None else None
     tgt = _c(target) if target is not None else None
-    loss = _lib.orc_ldbp_forward(steps, taps, _ptr(fir), _ptr(theta), x.size, _ptr(x), _ptr(out),
+    loss = _lib.orc_ldbp_forward(steps, taps, k.shape[1], _ptr(fir), _ptr(k), x.size, _ptr(x), _ptr(out),
                                  _ptr(tgt) if tgt is not None else None)
-    del tp
     return out, loss
 
 
-def ldbp_backward(fir, theta, x, target):
+def ldbp_backward(fir, kerr, x, target):
+    """Returns (loss, dL/d conj(fir), dL/d kerr) with kerr's shape."""
     fir = _c(fir)
-    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    shape = np.shape(kerr)
+    k = _kerr(kerr)
     x, target = _c(x), _c(target)
     steps, taps = fir.shape
     gf = np.zeros_like(fir)
-    gt = np.zeros_like(theta)
-    loss = _lib.orc_ldbp_backward(steps, taps, _ptr(fir), _ptr(theta), x.size, _ptr(x),
-                                  _ptr(target), _ptr(gf), _ptr(gt))
-    return loss, gf, gt
+    gk = np.zeros_like(k)
+    loss = _lib.orc_ldbp_backward(steps, taps, k.shape[1], _ptr(fir), _ptr(k), x.size, _ptr(x),
+                                  _ptr(target), _ptr(gf), _ptr(gk))
+    return loss, gf, gk.reshape(shape)

@@ -1,7 +1,10 @@
 """Learned few-step DBP kernels against the CPU oracle: forward output and
-loss, and the Wirtinger gradients of the FIR taps and Kerr scales (the
+loss, and the Wirtinger gradients of the FIR taps and of the nonlinear
+phase's intensity taps (one tap: the Kerr scale theta) (the
 oracle's analytic gradients are themselves pinned by finite differences in
-test_oracle.py).  FP32 kernels: tolerances 2e-4 (field, 63-tap chains) and 1e-3 (grads)."""
+test_oracle.py).  FP32 kernels: tolerances 2e-4 on the field (5e-4 for the
+chain that amplifies FP32 rounding most) and 1e-3 .. 2.5e-3 on gradients;
+a well-conditioned case holds the field to 5e-5 and the gradients to 2e-4."""
 import numpy as np
 import pytest
 
@@ -18,12 +21,12 @@ def ctx():
     c.close()
 
 
-def case(steps, taps, n, batch, seed=0):
+def case(steps, taps, n, batch, seed=0, ktaps=1):
     r = np.random.default_rng(seed)
     fir = np.zeros((steps, taps), dtype=np.complex128)
     fir[:, taps // 2] = 1.0
     fir += 0.15 * np.sqrt(15 / taps) * (r.standard_normal((steps, taps)) + 1j * r.standard_normal((steps, taps)))
-    theta = 0.4 * r.standard_normal(steps)
+    theta = 0.4 * r.standard_normal(steps) if ktaps == 1 else 0.4 / np.sqrt(ktaps) * r.standard_normal((steps, ktaps))
     x = (r.standard_normal(batch * n) + 1j * r.standard_normal(batch * n)) * 0.5
     tgt = (r.standard_normal(batch * n) + 1j * r.standard_normal(batch * n)) * 0.5
     # parameters as the device sees them (float32)
@@ -32,11 +35,23 @@ def case(steps, taps, n, batch, seed=0):
     return fir, theta, x.astype(np.complex64), tgt.astype(np.complex64)
 
 
-@pytest.mark.parametrize("taps,n,batch", [(15, 1 << 14, 2), (5, 4096, 3), (63, 8192, 2)])
-def test_ldbp_forward_and_gradients(ctx, taps, n, batch):
+# Field and gradient tolerances per case.  The random chains below amplify
+# FP32 rounding: the (63, 1, 8192) one grows 4.6x in amplitude over its 6
+# steps, so its Kerr phases reach ~8 rad and float32 arithmetic in the
+# kernel's summation order (emulated on the CPU) leaves 1.9e-4 .. 2.0e-4
+# relative field error against the FP64 oracle; gradients, accumulated in
+# FP32 back through the chain, carry up to ~10x the field error.  The last
+# case has small phases (FP32 emulation: 1.3e-6 field error) and checks field
+# and gradients tightly.
+@pytest.mark.parametrize("taps,ktaps,n,batch,tscale,tol,gtol", [
+    (15, 1, 1 << 14, 2, 1.0, 2e-4, 1e-3), (5, 1, 4096, 3, 1.0, 2e-4, 1e-3), (63, 1, 8192, 2, 1.0, 5e-4, 2.5e-3),
+    (63, 9, 4096, 2, 1.0, 2e-4, 2.5e-3), (15, 5, 1 << 14, 1, 1.0, 2e-4, 2.5e-3), (5, 15, 2048, 3, 1.0, 2e-4, 2.5e-3),
+    (15, 9, 4096, 2, 0.25, 5e-5, 2e-4)])
+def test_ldbp_forward_and_gradients(ctx, taps, ktaps, n, batch, tscale, tol, gtol):
     import focss_amd as F
     steps = 6
-    fir, theta, x, tgt = case(steps, taps, n, batch)
+    fir, theta, x, tgt = case(steps, taps, n, batch, ktaps=ktaps)
+    theta = (theta * tscale).astype(np.float32).astype(np.float64)
     model = F.LearnedDBP(ctx, steps, taps, n, batch, fir, theta)
     bx, bt, bo = ctx.alloc(x.nbytes), ctx.alloc(tgt.nbytes), ctx.alloc(x.nbytes)
     bx.upload(x)
@@ -49,7 +64,7 @@ def test_ldbp_forward_and_gradients(ctx, taps, n, batch):
     for b in range(batch):
         sl = slice(b * n, (b + 1) * n)
         o, l = O.ldbp_forward(fir, theta, x[sl].astype(np.complex128), tgt[sl].astype(np.complex128))
-        assert np.linalg.norm(out[sl] - o) / np.linalg.norm(o) < 2e-4
+        assert np.linalg.norm(out[sl] - o) / np.linalg.norm(o) < tol
         lb, gf, gt = O.ldbp_backward(fir, theta, x[sl].astype(np.complex128), tgt[sl].astype(np.complex128))
         loss_ref += lb / batch
         gf_ref += gf / batch
@@ -57,8 +72,9 @@ def test_ldbp_forward_and_gradients(ctx, taps, n, batch):
     assert loss == pytest.approx(loss_ref, rel=1e-5)
     l2, gf, gt = model.gradients(bx, bt)
     assert l2 == pytest.approx(loss_ref, rel=1e-5)
-    assert np.linalg.norm(gf - gf_ref) / np.linalg.norm(gf_ref) < 1e-3
-    assert np.linalg.norm(gt - gt_ref) / np.linalg.norm(gt_ref) < 1e-3
+    assert np.linalg.norm(gf - gf_ref) / np.linalg.norm(gf_ref) < gtol
+    gt = gt.reshape(np.shape(gt_ref))
+    assert np.linalg.norm(gt - gt_ref) / np.linalg.norm(gt_ref) < gtol
 
 
 def test_ldbp_adam_decreases_loss(ctx):

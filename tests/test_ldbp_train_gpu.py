@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 def test_learned_dbp_gains_over_untrained():
     from focss_amd.learned import run
-    args = argparse.Namespace(train=24, test=8, samples=4096, iters=200, batch=12, lr=1e-4, seed=7, taps=63, theta_lr_mult=1.0)
+    args = argparse.Namespace(train=24, test=8, samples=4096, iters=200, batch=12, lr=1e-4, seed=7, taps=63,
+                              theta_lr_mult=1.0, kerr_taps=1, rate_batch=8)
     rep = run(args, log=print)
     q = rep["q_db"]
     print(rep)

@@ -407,3 +407,50 @@ def test_ldbp_identity_and_gradients_fd():
         tm[j] -= eps
         d = (O.ldbp_forward(fir, tp, x, target)[1] - O.ldbp_forward(fir, tm, x, target)[1]) / (2 * eps)
         assert d == pytest.approx(gt[j], rel=1e-6, abs=1e-9)
+
+
+def test_ldbp_filtered_kerr_closed_form_and_fd():
+    """Intensity-filtered nonlinear phase phi(t) = sum_m g_m |y(t - m + P)|^2:
+    with an identity FIR one step is x exp(i phi) computed independently by a
+    circular convolution (an asymmetric g pins the index direction); ktaps 3
+    with g = (0, theta, 0) is the plain Kerr step; gradients of both kinds of
+    taps match central finite differences."""
+    r = rng(23)
+    n, steps, taps, ktaps = 48, 2, 5, 5
+    x = crandn(r, n) * 0.6
+    target = crandn(r, n) * 0.6
+    ident = np.zeros((1, taps), dtype=np.complex128)
+    ident[0, taps // 2] = 1.0
+    g = np.array([[0.05, -0.2, 0.7, 0.1, 0.3]])
+    out, _ = O.ldbp_forward(ident, g, x)
+    p = np.abs(x) ** 2
+    P = ktaps // 2
+    phi = np.array([sum(g[0, m] * p[(t - m + P) % n] for m in range(ktaps)) for t in range(n)])
+    assert np.allclose(out, x * np.exp(1j * phi), rtol=0, atol=1e-14)
+    theta = np.array([0.4, -0.25])
+    fir = np.zeros((steps, taps), dtype=np.complex128)
+    fir[:, taps // 2] = 1.0
+    fir = fir + 0.1 * crandn(r, steps * taps).reshape(steps, taps)
+    g3 = np.zeros((steps, 3))
+    g3[:, 1] = theta
+    a, _ = O.ldbp_forward(fir, theta, x)
+    b, _ = O.ldbp_forward(fir, g3, x)
+    assert np.array_equal(a, b)
+    kerr = 0.2 * r.standard_normal((steps, ktaps))
+    loss, gf, gk = O.ldbp_backward(fir, kerr, x, target)
+    eps = 1e-6
+    for (j, k) in [(0, 1), (1, 3)]:
+        for part in (1.0, 1j):
+            fp, fm = fir.copy(), fir.copy()
+            fp[j, k] += eps * part
+            fm[j, k] -= eps * part
+            d = (O.ldbp_forward(fp, kerr, x, target)[1] - O.ldbp_forward(fm, kerr, x, target)[1]) / (2 * eps)
+            ana = 2 * (gf[j, k].real if part == 1.0 else gf[j, k].imag)
+            assert d == pytest.approx(ana, rel=1e-6, abs=1e-9)
+    for j in range(steps):
+        for m in range(ktaps):
+            kp, km = kerr.copy(), kerr.copy()
+            kp[j, m] += eps
+            km[j, m] -= eps
+            d = (O.ldbp_forward(fir, kp, x, target)[1] - O.ldbp_forward(fir, km, x, target)[1]) / (2 * eps)
+            assert d == pytest.approx(gk[j, m], rel=1e-6, abs=1e-9)
