@@ -207,8 +207,9 @@ def main():
             "workload": f"{rep['waveforms']} test waveforms x 2^14 samples (16QAM 32 GBd, 2 sps), 20x80 km, "
                         f"launch {rep['launch_dbm_range'][0]:.1f}..{rep['launch_dbm_range'][1]:.1f} dBm, "
                         f"trained on {rep['train_waveforms']} simulator waveforms, {rep['iters']} Adam steps; "
-                        f"model: 40 x ({rep['taps']}-tap complex FIR, nonlinear phase from a "
-                        f"{rep['kerr_taps']}-tap intensity filter); rates on batches of {rep['rate_batch']}",
+                        f"model: {rep['steps_per_span']} steps/span x 20 spans of ({rep['taps']}-tap complex FIR, "
+                        f"nonlinear phase from a {rep['kerr_taps']}-tap intensity filter), inference on FP16 "
+                        f"matrix cores with FP32 accumulation; rates on batches of {rep['rate_batch']}",
             "q_db": {k: round(v, 3) for k, v in rep["q_db"].items()},
             "q_gap_learned_vs_dbp16_db": round(rep["q_gap_learned_vs_dbp16_db"], 3),
             "waveforms_per_s": {k: round(v, 1) for k, v in rep["waveforms_per_s"].items()},

@@ -72,6 +72,7 @@ _sig = {
     "focss_wdm_ppe": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P]),
     "focss_ldbp_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "focss_ldbp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "focss_ldbp_infer": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
     "focss_ldbp_gradients": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
                                        C.c_size_t, _P, _P]),
     "focss_ldbp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int,
@@ -272,6 +273,13 @@ class LearnedDBP:
                                        self.params.ptr, inp.ptr, out.ptr if out else None,
                                        target.ptr if target else None, C.byref(loss) if target else None))
         return loss.value
+
+    def infer(self, inp, out, batch=None):
+        """Inference with the FIR on FP16 matrix-core products (FP32
+        accumulation); stream-ordered."""
+        b = batch or self.batch
+        _check(_lib.focss_ldbp_infer(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples, b,
+                                     self.params.ptr, inp.ptr, out.ptr))
 
     def gradients(self, inp, target):
         """(loss, Wirtinger grad of the taps [steps][taps] complex, dL/dg [steps][kerr_taps])."""

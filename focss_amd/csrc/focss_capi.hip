@@ -258,6 +258,8 @@ struct focss_ctx {
     long long max_split_steps = 20000000;
     bool use_graph = true;  // FOCSS_NO_GRAPH=1: plain launches (profilers)
     int n_cu = 256;          // compute units of the device
+    void* ldbp_frag = nullptr;  // learned-DBP inference: matrix-core operands of every step
+    size_t ldbp_frag_bytes = 0;
 };
 
 namespace {
@@ -751,6 +753,9 @@ bool ldbp_kernels_k(int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
 
 bool ldbp_kernels(int taps, int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
     if (taps == 15) return ldbp_kernels_k<15>(ktaps, fwd, bwd);
+    if (taps == 23) return ldbp_kernels_k<23>(ktaps, fwd, bwd);
+    if (taps == 31) return ldbp_kernels_k<31>(ktaps, fwd, bwd);
+    if (taps == 47) return ldbp_kernels_k<47>(ktaps, fwd, bwd);
     if (taps == 63) return ldbp_kernels_k<63>(ktaps, fwd, bwd);
     if (taps == 5) return ldbp_kernels_k<5>(ktaps, fwd, bwd);
     return false;
@@ -759,7 +764,7 @@ bool ldbp_kernels(int taps, int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
 int ldbp_check(int steps, int taps, int ktaps, int samples, int batch) {
     LdbpKernel f, b;
     if (!ldbp_kernels(taps, ktaps, &f, &b))
-        return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15 or 63 and kerr_taps 1, 5, 9 or 15");
+        return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15, 23, 31, 47 or 63 and kerr_taps 1, 5, 9 or 15");
     if (steps < 1 || batch < 1) return fail(FOCSS_E_ARG, "learned DBP: steps and batch must be >= 1");
     int l = ilog2_int(samples);
     if (l < 0 || samples < LDBP_MIN_N || samples > LDBP_MAX_N)
@@ -780,6 +785,23 @@ int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
     HIP_TRY(hipGetLastError());
     return 0;
 }
+
+using LdbpInferKernel = void (*)(LdbpArgs, const ldbp_h8*);
+using LdbpAfragKernel = void (*)(const cx<float>*, int, ldbp_h8*);
+
+template <int K>
+bool ldbp_infer_k(int ktaps, LdbpInferKernel* k, LdbpAfragKernel* f, int* nks) {
+    *f = &k_ldbp_afrag<K>;
+    *nks = InferShape<K>::NKS;
+    switch (ktaps) {
+        case 1: *k = &k_ldbp_infer<K, 1>; return true;
+        case 5: *k = &k_ldbp_infer<K, 5>; return true;
+        case 9: *k = &k_ldbp_infer<K, 9>; return true;
+        case 15: *k = &k_ldbp_infer<K, 15>; return true;
+        default: return false;
+    }
+}
+
 
 }  // namespace
 
@@ -842,6 +864,7 @@ int focss_ctx_destroy(focss_ctx* c) {
         if (c->bank[i]) (void)hipFree(c->bank[i]);
         if (c->fine[i]) (void)hipFree(c->fine[i]);
     }
+    if (c->ldbp_frag) (void)hipFree(c->ldbp_frag);
     if (c->d_statics) {
         (void)hipFree(c->d_statics);
         (void)hipFree(c->d_states);
@@ -1179,6 +1202,55 @@ int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int sam
         if (e != hipSuccess) return fail(FOCSS_E_HIP, std::string("learned DBP loss: ") + hipGetErrorString(e));
     }
     return rc;
+}
+
+int focss_ldbp_infer(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch, const float* params,
+                     const void* in, void* out) {
+    if (!c || !params || !in || !out) return fail(FOCSS_E_ARG, "learned DBP: null argument");
+    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
+    LdbpInferKernel k = nullptr;
+    LdbpAfragKernel f = nullptr;
+    int nks = 0;
+    bool ok = false;
+    switch (taps) {
+        case 5: ok = ldbp_infer_k<5>(kerr_taps, &k, &f, &nks); break;
+        case 15: ok = ldbp_infer_k<15>(kerr_taps, &k, &f, &nks); break;
+        case 23: ok = ldbp_infer_k<23>(kerr_taps, &k, &f, &nks); break;
+        case 31: ok = ldbp_infer_k<31>(kerr_taps, &k, &f, &nks); break;
+        case 47: ok = ldbp_infer_k<47>(kerr_taps, &k, &f, &nks); break;
+        case 63: ok = ldbp_infer_k<63>(kerr_taps, &k, &f, &nks); break;
+        default: break;
+    }
+    if (!ok) return fail(FOCSS_E_UNSUPPORTED, "learned DBP: no inference kernel for these taps");
+    HIP_TRY(hipSetDevice(c->device));
+    // A operands of every step, in the context's scratch (grown on demand)
+    const size_t frag_bytes = (size_t)steps * nks * 64 * sizeof(ldbp_h8);
+    if (c->ldbp_frag_bytes < frag_bytes) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->ldbp_frag) (void)hipFree(c->ldbp_frag);
+        c->ldbp_frag = nullptr;
+        c->ldbp_frag_bytes = 0;
+        HIP_TRY(hipMalloc(&c->ldbp_frag, frag_bytes));
+        c->ldbp_frag_bytes = frag_bytes;
+    }
+    ldbp_h8* afrag = (ldbp_h8*)c->ldbp_frag;
+    const int n_frag = steps * nks * 64;
+    hipLaunchKernelGGL(f, dim3((n_frag + 255) / 256), dim3(256), 0, c->stream, (const cx<float>*)params, steps, afrag);
+    HIP_TRY(hipGetLastError());
+    LdbpArgs a = {};
+    a.in = (const cx<float>*)in;
+    a.out = (cx<float>*)out;
+    a.fir = (const cx<float>*)params;
+    a.kerr = params + 2 * steps * taps;
+    a.J = steps;
+    a.K = taps;
+    a.KP = kerr_taps;
+    a.N = samples;
+    a.B = batch;
+    const size_t lds = (size_t)samples * (sizeof(cx_h) + sizeof(float));
+    hipLaunchKernelGGL(k, dim3(batch), dim3(LI_NT), lds, c->stream, a, (const ldbp_h8*)afrag);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch,

@@ -276,6 +276,169 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     }
 }
 
+// ------------------------------------------------------------------
+// Inference on the matrix cores.  The FIR of one step is a block-Toeplitz
+// product: 16 consecutive outputs y[16 b + m] of block b take the window
+// x[w0(b) + q], w0(b) = 16 b + c - (K - 1) - DELTA, through
+//     T[m][q] = h[m - q + K - 1 + DELTA]   (zero outside 0..K-1).
+// In real form, rows 2m + e (e: re/im of the output) and columns 2q + f (f:
+// re/im of the input) hold [[h_r, -h_i], [h_i, h_r]]; 32 blocks form the
+// 32 columns of one v_mfma_f32_32x32x16_f16 tile (512 outputs), and the
+// window is covered by NKS k-steps of 16.  Operands are FP16 with FP32
+// accumulation; the field is held in LDS as FP16 between steps (rounding
+// adds ~2.8e-4 relative noise per step, about -52 dB over 80 steps; BF16's
+// 8-bit mantissa would add ~-34 dB, enough to cost Q at 24 dB), the
+// nonlinear phase is computed in FP32 from the FP16 FIR output, and the last
+// step writes FP32.
+// Lane layout of the 32x32x16 MFMA (checked on the device): lane l holds A
+// row l % 32 and B column l % 32 at k = 8 (l / 32) + u, u < 8, and
+// accumulator v is D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// Neighbouring lanes own neighbouring blocks, so the field and the
+// intensities are stored block-transposed in LDS (sample 16 b + m at row m,
+// column b): lanes then touch consecutive words, in the matrix phase and in
+// the nonlinear phase, where one thread owns one block.
+// ------------------------------------------------------------------
+typedef _Float16 ldbp_h8 __attribute__((ext_vector_type(8)));
+typedef float ldbp_f16 __attribute__((ext_vector_type(16)));
+
+struct cx_h {
+    _Float16 x, y;
+};
+
+constexpr int LI_NT = 1024;  // 16 waves; each wave owns up to 2 tiles of 512 outputs
+
+template <int K>
+struct InferShape {
+    static constexpr int c = K / 2;
+    static constexpr int DELTA = ((c - (K - 1)) % 4 + 4) % 4;
+    static constexpr int NKS = (16 + K - 1 + DELTA + 7) / 8;
+};
+
+// A operands of every step: afrag[(j NKS + s) 64 + lane] (8 FP16 each)
+template <int K>
+__global__ void k_ldbp_afrag(const cx<float>* __restrict__ fir, int J, ldbp_h8* __restrict__ afrag) {
+    using IS = InferShape<K>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= J * IS::NKS * 64) return;
+    const int lane = i & 63, s = (i >> 6) % IS::NKS, j = (i >> 6) / IS::NKS;
+    const int hl = lane >> 5, col = lane & 31, m_row = col >> 1, e_row = col & 1;
+    ldbp_h8 A;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int q = 8 * s + 4 * hl + (u >> 1);
+        const int k = m_row - q + K - 1 + IS::DELTA;
+        float v = 0.0f;
+        if (k >= 0 && k < K) {
+            const cx<float> hk = fir[j * K + k];
+            v = e_row == 0 ? ((u & 1) == 0 ? hk.x : -hk.y) : ((u & 1) == 0 ? hk.y : hk.x);
+        }
+        A[u] = (_Float16)v;
+    }
+    afrag[i] = A;
+}
+
+template <int K, int KP>
+__global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8* __restrict__ afrag) {
+    using IS = InferShape<K>;
+    constexpr int c = IS::c, DELTA = IS::DELTA, NKS = IS::NKS, P = KP / 2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = a.N, nb = N >> 4, lnb = __builtin_ctz(N >> 4);
+    cx_h* xs = reinterpret_cast<cx_h*>(smem_raw);                                 // [16][nb]
+    float* pb = reinterpret_cast<float*>(smem_raw + (size_t)N * sizeof(cx_h));   // [16][nb]
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int hl = lane >> 5, col = lane & 31;
+    const int n_tiles = N / 512;
+    const int b = blockIdx.x;
+    const cx<float>* in = a.in + (long long)b * N;
+    for (int p = threadIdx.x; p < N; p += LI_NT) {
+        const int t = ((p & (nb - 1)) << 4) + (p >> lnb);  // sample stored at position p
+        const cx<float> v = in[t];
+        xs[p] = cx_h{(_Float16)v.x, (_Float16)v.y};
+    }
+    __syncthreads();
+    for (int j = 0; j < a.J; ++j) {
+        // FIR on the matrix cores
+        ldbp_h8 A[NKS];
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) A[s] = afrag[(j * NKS + s) * 64 + lane];
+        ldbp_f16 acc[2];
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            acc[it] = ldbp_f16{};
+            const int tile = wid + 16 * it;
+            if (tile < n_tiles) {
+                const int blk = tile * 32 + col;
+                const int off = c - (K - 1) - DELTA + 4 * hl;  // window start relative to 16 blk
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) {
+                    ldbp_h8 bf;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int o = off + 8 * s + v;  // sample 16 blk + o = row o & 15 of block blk + (o >> 4)
+                        const cx_h xv = xs[((o & 15) << lnb) + ((blk + (o >> 4)) & (nb - 1))];
+                        bf[2 * v] = xv.x;
+                        bf[2 * v + 1] = xv.y;
+                    }
+                    acc[it] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[s], bf, acc[it], 0, 0, 0);
+                }
+            }
+        }
+        // every read of the field is done before it is overwritten
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int tile = wid + 16 * it;
+            if (tile < n_tiles) {
+                const int blk = tile * 32 + col;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < 2; ++jj) {
+                        const int row = 4 * i + 2 * hl + jj;
+                        const float re = acc[it][4 * i + 2 * jj], im = acc[it][4 * i + 2 * jj + 1];
+                        xs[(row << lnb) + blk] = cx_h{(_Float16)re, (_Float16)im};
+                        pb[(row << lnb) + blk] = re * re + im * im;
+                    }
+            }
+        }
+        __syncthreads();
+        // nonlinear phase, one thread per block of 16 samples
+        const bool last = j == a.J - 1;
+        float g[KP];
+#pragma unroll
+        for (int mm = 0; mm < KP; ++mm) g[mm] = sload(a.kerr + j * KP + mm);
+        for (int blk = threadIdx.x; blk < nb; blk += LI_NT) {
+            float pe[16 + 2 * P];  // p(16 blk - P + u)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) pe[P + r] = pb[(r << lnb) + blk];
+            if constexpr (P > 0) {
+                const int bp = (blk + nb - 1) & (nb - 1), bn = (blk + 1) & (nb - 1);
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    pe[i] = pb[((16 - P + i) << lnb) + bp];
+                    pe[P + 16 + i] = pb[(i << lnb) + bn];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                float phi = 0.0f;
+#pragma unroll
+                for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
+                float sn, cs;
+                sincos_fast(phi, &sn, &cs);
+                const cx_h yh = xs[(r << lnb) + blk];
+                const cx<float> z = mk<float>((float)yh.x, (float)yh.y) * mk<float>(cs, sn);
+                if (last)
+                    a.out[(long long)b * N + 16 * blk + r] = z;
+                else
+                    xs[(r << lnb) + blk] = cx_h{(_Float16)z.x, (_Float16)z.y};
+            }
+        }
+        // the new field is visible before the next step's reads
+        if (!last) __syncthreads();
+    }
+}
+
 // Adam (Kingma & Ba, 2015) on float parameters with double gradients;
 // p holds the FIR taps (re, im interleaved) followed by the intensity taps.
 __global__ void k_adam(float* __restrict__ p, const double* __restrict__ g, float* __restrict__ m,

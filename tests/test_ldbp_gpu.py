@@ -92,3 +92,38 @@ def test_ldbp_adam_decreases_loss(ctx):
     bt.upload(tgt)
     losses = [model.train_step(bx, bt, 1e-2) for _ in range(60)]
     assert losses[-1] < 0.2 * losses[0]
+
+
+@pytest.mark.parametrize("taps,ktaps,n,batch,steps", [(31, 9, 1 << 14, 2, 6), (63, 15, 8192, 2, 6), (5, 1, 2048, 3, 6),
+                                                      (31, 9, 4096, 2, 80)])
+def test_ldbp_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
+    """Matrix-core inference (FP16 operands, FP32 accumulation, FP16 field
+    between steps) against the FP64 oracle: each step rounds the field to
+    FP16 (2^-11, ~2.8e-4 RMS relative), so the bound is 6e-4 sqrt(steps).
+    Well-conditioned chains: near-identity FIRs, small phases."""
+    import focss_amd as F
+    r = np.random.default_rng(42)
+    fir = np.zeros((steps, taps), dtype=np.complex128)
+    fir[:, taps // 2] = 1.0
+    fir += 0.03 * (r.standard_normal((steps, taps)) + 1j * r.standard_normal((steps, taps))) / np.sqrt(taps / 15)
+    g = 0.05 / ktaps * r.standard_normal((steps, ktaps))
+    fir = fir.astype(np.complex64).astype(np.complex128)
+    g = g.astype(np.float32).astype(np.float64)
+    x = ((r.standard_normal(batch * n) + 1j * r.standard_normal(batch * n)) * 0.5).astype(np.complex64)
+    model = F.LearnedDBP(ctx, steps, taps, n, batch, fir, g)
+    bx, bo, bf = ctx.alloc(x.nbytes), ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
+    bx.upload(x)
+    model.infer(bx, bo)
+    model.forward(bx, bf)
+    out = bo.download(np.empty_like(x))
+    out32 = bf.download(np.empty_like(x))
+    tol = 6e-4 * np.sqrt(steps)
+    for b in range(batch):
+        sl = slice(b * n, (b + 1) * n)
+        if steps <= 6:
+            o, _ = O.ldbp_forward(fir, g, x[sl].astype(np.complex128))
+        else:
+            o = out32[sl].astype(np.complex128)
+        err = np.linalg.norm(out[sl] - o) / np.linalg.norm(o)
+        print(taps, ktaps, n, steps, b, "rel L2 %.2e (bound %.1e)" % (err, tol))
+        assert err < tol
