@@ -59,6 +59,13 @@ _sig = {
     "focss_link_run_host": (C.c_int, [_P, C.c_int, C.POINTER(Link), _P, _P]),
     "focss_fft": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "focss_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "focss_frequency_shift": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P]),
+    "focss_bandpass_filter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P]),
+    "focss_wdm_select": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _P, _P]),
+    "focss_decimate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "focss_filter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "focss_rrc_shaping": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_double, C.c_double, _P]),
+    "focss_rrc_sampling": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_double, C.c_double, _P]),
     "focss_set_tile_pass": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "focss_tile_variants": (C.c_int, [C.c_int, C.c_int]),
     "focss_build_info": (C.c_char_p, []),
@@ -195,6 +202,32 @@ class Context:
 
     def fft(self, buf, modes, samples, sign, precision=F64):
         _check(_lib.focss_fft(self.handle, precision, sign, modes, samples, buf.ptr))
+
+    # ---- transmitter / receiver DSP (device buffers, mode-major) ----
+    def frequency_shift(self, buf, modes, samples, dt, frequency, precision=F64):
+        _check(_lib.focss_frequency_shift(self.handle, precision, modes, samples, dt, frequency, buf.ptr))
+
+    def bandpass_filter(self, buf, modes, samples, dt, frequency, precision=F64):
+        _check(_lib.focss_bandpass_filter(self.handle, precision, modes, samples, dt, frequency, buf.ptr))
+
+    def wdm_select(self, inp, out, modes, samples, dt, channel_index, channel_bandwidth, precision=F64):
+        _check(_lib.focss_wdm_select(self.handle, precision, modes, samples, dt, channel_index, channel_bandwidth,
+                                     inp.ptr, out.ptr))
+
+    def decimate(self, inp, out, modes, samples, factor, precision=F64):
+        _check(_lib.focss_decimate(self.handle, precision, modes, samples, factor, inp.ptr, out.ptr))
+
+    def filter(self, buf, modes, samples, impulse_response, precision=F64):
+        h = np.ascontiguousarray(impulse_response, dtype=np.complex128)
+        _check(_lib.focss_filter(self.handle, precision, modes, samples, h.size, h.ctypes.data, buf.ptr))
+
+    def rrc_shaping(self, symbols, out, modes, n_symbols, osf, roll_off, gain=1.0, precision=F64):
+        _check(_lib.focss_rrc_shaping(self.handle, precision, modes, n_symbols, symbols.ptr, osf, roll_off, gain,
+                                      out.ptr))
+
+    def rrc_sampling(self, signal, out, modes, samples, osf, roll_off, gain=1.0, precision=F64):
+        _check(_lib.focss_rrc_sampling(self.handle, precision, modes, samples, signal.ptr, osf, roll_off, gain,
+                                       out.ptr))
 
     def sino_cubic(self, weights, radius, inp, out, modes, samples, precision=F64):
         w = np.ascontiguousarray(weights, dtype=np.complex128)

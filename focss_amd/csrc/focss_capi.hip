@@ -17,6 +17,7 @@
 #include "fft_kernels.h"
 #include "triplet_kernels.h"
 #include "ldbp_kernels.h"
+#include "dsp_kernels.h"
 
 using namespace focss_hip;
 
@@ -260,6 +261,8 @@ struct focss_ctx {
     int n_cu = 256;          // compute units of the device
     void* ldbp_frag = nullptr;  // learned-DBP inference: matrix-core operands of every step
     size_t ldbp_frag_bytes = 0;
+    void* dsp_tmp = nullptr;    // transmitter/receiver DSP work space
+    size_t dsp_tmp_bytes = 0;
 };
 
 namespace {
@@ -617,8 +620,9 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
 }  // namespace
 
 namespace {
+// stream-ordered DFT of `modes` rows of `samples` (forward x 1/N)
 template <typename R>
-int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
+int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data) {
     int log_n = ilog2_int(samples);
     if (log_n < 1 || log_n > 20) return fail(FOCSS_E_ARG, "fft: samples must be a power of two in [2, 2^20]");
     if (modes < 1) return fail(FOCSS_E_ARG, "fft: modes must be >= 1");
@@ -637,6 +641,7 @@ int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
     } else {
         size_t bytes = (size_t)modes * samples * sizeof(cx<R>);
         if (c->scratch_bytes < bytes) {
+            HIP_TRY(hipStreamSynchronize(c->stream));  // earlier work may still use the old buffer
             if (c->scratch) HIP_TRY(hipFree(c->scratch));
             c->scratch = nullptr;
             c->scratch_bytes = 0;
@@ -653,6 +658,12 @@ int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
         HIP_TRY(hipMemcpyAsync(d, c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename R>
+int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
+    if (int rc = fft_enqueue<R>(c, sign, modes, samples, data)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -732,6 +743,176 @@ int cholesky_solve(std::vector<cx<double>>& A, int n, std::vector<cx<double>>& b
         b[i] = v * (1.0 / A[(size_t)i * n + i].x);
     }
     return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// ---------------- transmitter / receiver DSP ----------------
+inline unsigned grid_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+int dsp_workspace(focss_ctx* c, size_t bytes, char** out) {
+    if (c->dsp_tmp_bytes < bytes) {
+        HIP_TRY(hipStreamSynchronize(c->stream));  // earlier work may still use the old buffer
+        if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
+        c->dsp_tmp = nullptr;
+        c->dsp_tmp_bytes = 0;
+        HIP_TRY(hipMalloc(&c->dsp_tmp, bytes));
+        c->dsp_tmp_bytes = bytes;
+    }
+    *out = (char*)c->dsp_tmp;
+    return 0;
+}
+
+int dsp_check(int modes, int samples) {
+    if (modes < 1) return fail(FOCSS_E_ARG, "dsp: modes must be >= 1");
+    int l = ilog2_int(samples);
+    if (l < 1 || l > 20) return fail(FOCSS_E_ARG, "dsp: samples must be a power of two in [2, 2^20]");
+    return 0;
+}
+
+template <typename R>
+int frequency_shift_impl(focss_ctx* c, int modes, int samples, double dt, double frequency, void* data) {
+    const long long total = (long long)modes * samples;
+    hipLaunchKernelGGL(k_frequency_shift<R>, dim3(grid_for(total)), dim3(256), 0, c->stream, (cx<R>*)data, total,
+                       samples, 2.0 * M_PI * frequency * dt);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename R>
+int bandpass_impl(focss_ctx* c, int modes, int samples, double dt, double frequency, void* data) {
+    const int cut = (int)(frequency * (samples * dt)) + 1;  // modulation.cpp:266-277
+    const long long total = (long long)modes * samples;
+    if (int rc = fft_enqueue<R>(c, -1, modes, samples, data)) return rc;
+    hipLaunchKernelGGL(k_band_zero<R>, dim3(grid_for(total)), dim3(256), 0, c->stream, (cx<R>*)data, total, samples,
+                       cut);
+    HIP_TRY(hipGetLastError());
+    return fft_enqueue<R>(c, +1, modes, samples, data);
+}
+
+// Field::filter_inplace (field.cpp:358-372): the impulse response (taps,
+// centred at sample 0, odd lengths drop the last tap) is zero-padded to the
+// field length and transformed with the 1/N convention of focss::fft, then
+// multiplied into the field's spectrum.  spec_ws: samples complex of R.
+template <typename R>
+int filter_impl(focss_ctx* c, int modes, int samples, int taps, const cx<double>* h_host, void* data, cx<R>* spec_ws,
+                cx<double>* upload_ws) {
+    std::vector<cx<double>> zp((size_t)samples, mk<double>(0, 0));
+    const int half = taps / 2, shift = samples - half;
+    for (int i = 0; i < half; ++i) {
+        zp[i] = h_host[i + half];
+        zp[shift + i] = h_host[i];
+    }
+    HIP_TRY(hipMemcpyAsync(upload_ws, zp.data(), zp.size() * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_from_c128<R>, dim3(grid_for(samples)), dim3(256), 0, c->stream, upload_ws, spec_ws,
+                       (long long)samples);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));  // the host copy of zp goes out of scope
+    if (int rc = fft_enqueue<R>(c, -1, 1, samples, spec_ws)) return rc;
+    if (int rc = fft_enqueue<R>(c, -1, modes, samples, data)) return rc;
+    const long long total = (long long)modes * samples;
+    hipLaunchKernelGGL(k_mul_spectrum<R>, dim3(grid_for(total)), dim3(256), 0, c->stream, (cx<R>*)data,
+                       (const cx<R>*)spec_ws, total, samples);
+    HIP_TRY(hipGetLastError());
+    return fft_enqueue<R>(c, +1, modes, samples, data);
+}
+
+// pulse::rrc (modulation.cpp:47-74): root-raised-cosine taps over the frame,
+// normalised to power `size`
+std::vector<cx<double>> rrc_pulse_host(int n_symbols, int osf, double roll_off) {
+    const int size = n_symbols * osf;
+    std::vector<cx<double>> p((size_t)size);
+    double power = 0;
+    for (int i = 0; i < size; ++i) {
+        const double t = (double)i / osf - n_symbols / 2.0;
+        double v;
+        if (t == 0) {
+            v = 1 - roll_off + 4 * roll_off / M_PI;
+        } else if (std::fabs(t) == 1 / roll_off / 4) {
+            v = roll_off / std::sqrt(2.0) *
+                ((1 + 2 / M_PI) * std::sin(M_PI / roll_off / 4) + (1 - 2 / M_PI) * std::cos(M_PI / roll_off / 4));
+        } else {
+            v = (4 * roll_off * t * std::cos(M_PI * t * (1 + roll_off)) + std::sin(M_PI * t * (1 - roll_off))) /
+                (1 - 16 * roll_off * roll_off * t * t) / (M_PI * t);
+        }
+        p[i] = mk<double>(v, 0);
+        power += v * v;
+    }
+    for (auto& x : p) x = x * (size / power);
+    return p;
+}
+
+template <typename R>
+int decimate_impl(focss_ctx* c, int modes, int samples, int factor, const void* in, void* out) {
+    const int ds = samples / factor;
+    const size_t bytes = (size_t)modes * samples * sizeof(cx<R>);
+    char* ws;
+    if (int rc = dsp_workspace(c, bytes, &ws)) return rc;
+    HIP_TRY(hipMemcpyAsync(ws, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (int rc = fft_enqueue<R>(c, -1, modes, samples, ws)) return rc;
+    hipLaunchKernelGGL(k_decimate_gather<R>, dim3(grid_for((long long)modes * ds)), dim3(256), 0, c->stream,
+                       (const cx<R>*)ws, (cx<R>*)out, modes, samples, ds, samples / factor / 2);
+    HIP_TRY(hipGetLastError());
+    return fft_enqueue<R>(c, +1, modes, ds, out);
+}
+
+template <typename R>
+int rrc_shaping_impl(focss_ctx* c, int modes, int n_symbols, const void* symbols, int osf, double roll_off,
+                     double gain, void* out) {
+    const int samples = n_symbols * osf;
+    const long long total = (long long)modes * samples;
+    char* ws;
+    if (int rc = dsp_workspace(c, (size_t)samples * (sizeof(cx<double>) + sizeof(cx<R>)), &ws)) return rc;
+    HIP_TRY(hipMemsetAsync(out, 0, (size_t)total * sizeof(cx<R>), c->stream));
+    hipLaunchKernelGGL(k_upsample<R>, dim3(grid_for((long long)modes * n_symbols)), dim3(256), 0, c->stream,
+                       (const cx<R>*)symbols, (cx<R>*)out, (long long)modes * n_symbols, osf);
+    HIP_TRY(hipGetLastError());
+    const std::vector<cx<double>> rrc = rrc_pulse_host(n_symbols, osf, roll_off);
+    if (int rc = filter_impl<R>(c, modes, samples, samples, rrc.data(), out,
+                                (cx<R>*)(ws + (size_t)samples * sizeof(cx<double>)), (cx<double>*)ws))
+        return rc;
+    hipLaunchKernelGGL(k_scale<R>, dim3(grid_for(total)), dim3(256), 0, c->stream, (cx<R>*)out, total,
+                       std::sqrt(gain) * osf);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename R>
+int rrc_sampling_impl(focss_ctx* c, int modes, int samples, const void* signal, int osf, double roll_off, double gain,
+                      void* out) {
+    const int n_symbols = samples / osf;
+    const size_t work = (size_t)modes * samples * sizeof(cx<R>);
+    char* ws;
+    if (int rc = dsp_workspace(c, work + (size_t)samples * (sizeof(cx<double>) + sizeof(cx<R>)), &ws)) return rc;
+    cx<R>* tmp = (cx<R>*)ws;
+    HIP_TRY(hipMemcpyAsync(tmp, signal, work, hipMemcpyDeviceToDevice, c->stream));
+    const std::vector<cx<double>> rrc = rrc_pulse_host(n_symbols, osf, roll_off);
+    if (int rc = filter_impl<R>(c, modes, samples, samples, rrc.data(), tmp,
+                                (cx<R>*)(ws + work + (size_t)samples * sizeof(cx<double>)),
+                                (cx<double>*)(ws + work)))
+        return rc;
+    hipLaunchKernelGGL(k_downsample<R>, dim3(grid_for((long long)modes * n_symbols)), dim3(256), 0, c->stream,
+                       (const cx<R>*)tmp, (cx<R>*)out, (long long)modes * n_symbols, osf, std::sqrt(gain));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename R>
+int wdm_select_impl(focss_ctx* c, int modes, int samples, double dt, int channel_index, double bandwidth,
+                           const void* in, void* out) {
+    HIP_TRY(hipMemcpyAsync(out, in, (size_t)modes * samples * sizeof(cx<R>), hipMemcpyDeviceToDevice, c->stream));
+    if (int rc = frequency_shift_impl<R>(c, modes, samples, dt, -bandwidth * channel_index, out)) return rc;
+    return bandpass_impl<R>(c, modes, samples, dt, 0.5 * bandwidth, out);
+}
+
+template <typename R>
+int filter_entry(focss_ctx* c, int modes, int samples, int taps, const double* h, void* data) {
+    char* ws;
+    if (int rc = dsp_workspace(c, (size_t)samples * (sizeof(cx<double>) + sizeof(cx<R>)), &ws)) return rc;
+    return filter_impl<R>(c, modes, samples, taps, (const cx<double>*)h, data,
+                          (cx<R>*)(ws + (size_t)samples * sizeof(cx<double>)), (cx<double>*)ws);
 }
 
 }  // namespace
@@ -865,6 +1046,7 @@ int focss_ctx_destroy(focss_ctx* c) {
         if (c->fine[i]) (void)hipFree(c->fine[i]);
     }
     if (c->ldbp_frag) (void)hipFree(c->ldbp_frag);
+    if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
     if (c->d_statics) {
         (void)hipFree(c->d_statics);
         (void)hipFree(c->d_states);
@@ -1315,6 +1497,83 @@ int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int 
     (void)hipFree(acc);
     return rc;
 }
+
+#define FOCSS_DISPATCH(call_f64, call_f32)                                                   \
+    do {                                                                                   \
+        if (precision == FOCSS_F64) return call_f64;                                       \
+        if (precision == FOCSS_F32) return call_f32;                                       \
+        return fail(FOCSS_E_ARG, "precision must be FOCSS_F64 or FOCSS_F32");              \
+    } while (0)
+
+int focss_frequency_shift(focss_ctx* c, int precision, int modes, int samples, double dt, double frequency,
+                          void* data) {
+    if (!c || !data) return fail(FOCSS_E_ARG, "frequency_shift: null argument");
+    if (int rc = dsp_check(modes, samples)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((frequency_shift_impl<double>(c, modes, samples, dt, frequency, data)),
+                   (frequency_shift_impl<float>(c, modes, samples, dt, frequency, data)));
+}
+
+int focss_bandpass_filter(focss_ctx* c, int precision, int modes, int samples, double dt, double frequency,
+                          void* data) {
+    if (!c || !data) return fail(FOCSS_E_ARG, "bandpass_filter: null argument");
+    if (int rc = dsp_check(modes, samples)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((bandpass_impl<double>(c, modes, samples, dt, frequency, data)),
+                   (bandpass_impl<float>(c, modes, samples, dt, frequency, data)));
+}
+
+int focss_wdm_select(focss_ctx* c, int precision, int modes, int samples, double dt, int channel_index,
+                     double channel_bandwidth, const void* in, void* out) {
+    if (!c || !in || !out) return fail(FOCSS_E_ARG, "wdm_select: null argument");
+    if (int rc = dsp_check(modes, samples)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((wdm_select_impl<double>(c, modes, samples, dt, channel_index, channel_bandwidth, in, out)),
+                   (wdm_select_impl<float>(c, modes, samples, dt, channel_index, channel_bandwidth, in, out)));
+}
+
+int focss_decimate(focss_ctx* c, int precision, int modes, int samples, int factor, const void* in, void* out) {
+    if (!c || !in || !out) return fail(FOCSS_E_ARG, "decimate: null argument");
+    if (int rc = dsp_check(modes, samples)) return rc;
+    if (factor < 1 || samples % factor || ilog2_int(samples / factor) < 1)
+        return fail(FOCSS_E_ARG, "decimate: factor must divide samples, leaving a power of two >= 2");
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((decimate_impl<double>(c, modes, samples, factor, in, out)),
+                   (decimate_impl<float>(c, modes, samples, factor, in, out)));
+}
+
+int focss_filter(focss_ctx* c, int precision, int modes, int samples, int taps, const double* impulse_response,
+                 void* data) {
+    if (!c || !data || !impulse_response) return fail(FOCSS_E_ARG, "filter: null argument");
+    if (int rc = dsp_check(modes, samples)) return rc;
+    if (taps < 1 || taps > samples) return fail(FOCSS_E_ARG, "filter: taps must be in [1, samples]");
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((filter_entry<double>(c, modes, samples, taps, impulse_response, data)),
+                   (filter_entry<float>(c, modes, samples, taps, impulse_response, data)));
+}
+
+int focss_rrc_shaping(focss_ctx* c, int precision, int modes, int n_symbols, const void* symbols, int osf,
+                      double roll_off, double gain, void* out) {
+    if (!c || !symbols || !out) return fail(FOCSS_E_ARG, "rrc_shaping: null argument");
+    if (osf < 1 || !(roll_off > 0 && roll_off <= 1)) return fail(FOCSS_E_ARG, "rrc_shaping: bad osf or roll-off");
+    if (int rc = dsp_check(modes, n_symbols * osf)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((rrc_shaping_impl<double>(c, modes, n_symbols, symbols, osf, roll_off, gain, out)),
+                   (rrc_shaping_impl<float>(c, modes, n_symbols, symbols, osf, roll_off, gain, out)));
+}
+
+int focss_rrc_sampling(focss_ctx* c, int precision, int modes, int samples, const void* signal, int osf,
+                       double roll_off, double gain, void* out) {
+    if (!c || !signal || !out) return fail(FOCSS_E_ARG, "rrc_sampling: null argument");
+    if (osf < 1 || samples % osf || !(roll_off > 0 && roll_off <= 1))
+        return fail(FOCSS_E_ARG, "rrc_sampling: bad osf or roll-off");
+    if (int rc = dsp_check(modes, samples)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((rrc_sampling_impl<double>(c, modes, samples, signal, osf, roll_off, gain, out)),
+                   (rrc_sampling_impl<float>(c, modes, samples, signal, osf, roll_off, gain, out)));
+}
+
+#undef FOCSS_DISPATCH
 
 int focss_set_tile(focss_ctx* c, int precision, int log2_samples, int variant) {
     if (!c || precision < 0 || precision > 1 || log2_samples < 0 || log2_samples > 20 || variant < 0 ||

@@ -172,6 +172,39 @@ int focss_sino_train(focss_ctx* ctx, int radius, int type, const double* linear_
 int focss_wdm_ppe(focss_ctx* ctx, int precision, int c_radius, const double* C, int d_radius,
                   const double* D, int samples, const void* a6, void* spm, void* xpm);
 
+/* ---- transmitter / receiver DSP on the device ----
+ * Fields are mode-major device buffers [modes][samples] of the given
+ * precision, samples a power of two <= 2^20.  Each call is stream-ordered
+ * (focss_filter, focss_rrc_shaping and focss_rrc_sampling also synchronise
+ * once to upload the impulse response).  The DFTs follow Field::fft_inplace
+ * (forward x 1/N). */
+/* frequency_shift: f *= exp(i 2 pi frequency dt s)   (modulation.cpp:255-264) */
+int focss_frequency_shift(focss_ctx* ctx, int precision, int modes, int samples, double dt, double frequency,
+                          void* data);
+/* bandpass_filter: zero the bins cut .. N - cut, cut = frequency N dt + 1
+ *                                                    (modulation.cpp:266-277) */
+int focss_bandpass_filter(focss_ctx* ctx, int precision, int modes, int samples, double dt, double frequency,
+                          void* data);
+/* wdm_select: shift channel channel_index to baseband and bandpass it
+ *                                                    (modulation.cpp:248-253) */
+int focss_wdm_select(focss_ctx* ctx, int precision, int modes, int samples, double dt, int channel_index,
+                     double channel_bandwidth, const void* in, void* out);
+/* Field::decimate: spectral truncation to samples / factor (field.cpp:299-320);
+ * out holds modes x samples / factor. */
+int focss_decimate(focss_ctx* ctx, int precision, int modes, int samples, int factor, const void* in, void* out);
+/* Field::filter_inplace with a host impulse response of `taps` complex128
+ * values, centred at sample 0                        (field.cpp:358-372) */
+int focss_filter(focss_ctx* ctx, int precision, int modes, int samples, int taps, const double* impulse_response,
+                 void* data);
+/* rrc_shaping: upsample symbols [modes][n_symbols] by osf, RRC filter,
+ * x sqrt(gain) osf                                   (modulation.cpp:86-90) */
+int focss_rrc_shaping(focss_ctx* ctx, int precision, int modes, int n_symbols, const void* symbols, int osf,
+                      double roll_off, double gain, void* out);
+/* rrc_sampling: RRC filter, take every osf-th sample of the flat buffer,
+ * x sqrt(gain); out holds modes x samples / osf       (modulation.cpp:92-96) */
+int focss_rrc_sampling(focss_ctx* ctx, int precision, int modes, int samples, const void* signal, int osf,
+                       double roll_off, double gain, void* out);
+
 /* ---- learned few-step DBP (no counterpart in the reference) ----
  * Step j: y = complex FIR (taps, circular, centre taps/2) of x, then
  * z = y exp(i phi), phi(t) = sum_m g_{j,m} |y(t - m + kerr_taps/2)|^2 (a real
