@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--precision", choices=["f64", "f32", "both"], default="both")
     ap.add_argument("--cpu-km", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -103,21 +103,18 @@ __device__ __forceinline__ cx<float> phase_factor<float>(double phase, double ma
     return mk<float>(c * m, s * m);
 }
 
-// Kerr rotation exp(i coef P): computed in the field's precision.
+// Kerr rotation exp(i coef P), computed in the field's precision.  When every
+// lane of the wave has |phase| <= 1/16 (always so on an adaptive grid, whose
+// steps bound the phase by phi_max) the short series replaces the reduced
+// one; the choice is uniform over the wave.
 template <typename R>
-__device__ __forceinline__ cx<R> kerr_factor(R coef_p);
-
-template <>
-__device__ __forceinline__ cx<double> kerr_factor<double>(double ph) {
-    double s, c;
-    sincos_fast(ph, &s, &c);
-    return mk<double>(c, s);
-}
-template <>
-__device__ __forceinline__ cx<float> kerr_factor<float>(float ph) {
-    float s, c;
-    sincos_fast(ph, &s, &c);
-    return mk<float>(c, s);
+__device__ __forceinline__ cx<R> kerr_factor(R ph) {
+    R s, c;
+    if (__all(fabs(ph) <= (R)0.0625))
+        sincos_small(ph, &s, &c);
+    else
+        sincos_fast(ph, &s, &c);
+    return mk<R>(c, s);
 }
 
 }  // namespace focss_hip

@@ -88,6 +88,30 @@ FOCSS_TRIG_HD inline void sincos_reduced(float x, float* s, float* c) {
     quadrant((int)k, sr, cr, s, c);
 }
 
+// |x| <= 1/16 (the per-step nonlinear phases of an adaptive grid): the series
+// without reduction, truncated after x^9 (sin) and x^10 (cos); the first
+// omitted terms are below 2e-21 and 8e-24
+FOCSS_TRIG_HD inline void sincos_small(double x, double* s, double* c) {
+    const double x2 = x * x;
+    double ps = 2.7557319223985893e-06;
+    ps = fma(ps, x2, -0.0001984126984126984);
+    ps = fma(ps, x2, 0.008333333333333333);
+    ps = fma(ps, x2, -0.16666666666666666);
+    *s = fma(x * x2, ps, x);
+    double pc = -2.755731922398589e-07;
+    pc = fma(pc, x2, 2.48015873015873e-05);
+    pc = fma(pc, x2, -0.001388888888888889);
+    pc = fma(pc, x2, 0.041666666666666664);
+    *c = fma(x2 * x2, pc, fma(-0.5, x2, 1.0));
+}
+
+// |x| <= 1/16: truncated after x^5 and x^6 (omitted terms below 1e-12)
+FOCSS_TRIG_HD inline void sincos_small(float x, float* s, float* c) {
+    const float x2 = x * x;
+    *s = fmaf(x * x2, fmaf(x2, 0.008333333333333333f, -0.16666666666666666f), x);
+    *c = fmaf(x2 * x2, fmaf(x2, -0.001388888888888889f, 0.041666666666666664f), fmaf(-0.5f, x2, 1.0f));
+}
+
 FOCSS_TRIG_HD inline void sincos_fast(double x, double* s, double* c) {
     if (fabs(x) <= 1048576.0)
         sincos_reduced(x, s, c);

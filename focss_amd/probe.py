@@ -48,7 +48,7 @@ def main():
         nbytes = args.copy_mib << 20
         a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
         ctx.device_copy(b, a, nbytes)
-        print(f"probe {args.precision} variant {args.variant}: time pass {ms[0] * 1e3:.2f} us, spectral pass {ms[1] * 1e3:.2f} us, "
+        print(f"probe {args.precision} batch {args.batch} variant {args.variant}: time pass {ms[0] * 1e3:.2f} us, spectral pass {ms[1] * 1e3:.2f} us, "
               f"field bytes {host.nbytes}, copy bytes {nbytes}", flush=True)
 
 

@@ -36,9 +36,19 @@ int main() {
             check32((float)x + d * 1e-7f);
         }
     }
+    double e_small = 0, e_small32 = 0;
+    for (int i = -200000; i <= 200000; ++i) {  // the short series on |x| <= 1/16
+        double x = 0.0625 * i / 200000.0;
+        double ss, cc;
+        focss_hip::sincos_small(x, &ss, &cc);
+        e_small = fmax(e_small, fmax(fabs(ss - (double)sinl(x)), fabs(cc - (double)cosl(x))));
+        float sf, cf, xf = (float)x;
+        focss_hip::sincos_small(xf, &sf, &cf);
+        e_small32 = fmax(e_small32, fmax(fabs(sf - (double)sinl(xf)), fabs(cf - (double)cosl(xf))));
+    }
     double s, c;
     focss_hip::sincos_fast(3.0e7, &s, &c);  // library fallback
     double e_fb = fmax(fabs(s - sin(3.0e7)), fabs(c - cos(3.0e7)));
-    printf("%.3e %.3e %.3e %.3e\n", e64, e64_big, e32, e_fb);
+    printf("%.3e %.3e %.3e %.3e %.3e %.3e\n", e64, e64_big, e32, e_fb, e_small, e_small32);
     return 0;
 }
