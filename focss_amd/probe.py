@@ -4,7 +4,7 @@ Runs the first `pairs` split steps of the benchmark workload (a batch of
 5-channel WDM fields, 2^18 samples, adaptive 80 km spans) with one kernel
 launch per pass, then one plain 16-byte copy kernel over `copy_mib` MiB,
 whose counters calibrate HBM bytes.  Usage:
-    python -m focss_amd.probe [--batch 16] [--pairs 12] [--precision f64|f32] [--variant V]
+    python -m focss_amd.probe [--batch 32] [--pairs 12] [--precision f64|f32] [--variant V]
 """
 import argparse
 import os
@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--pairs", type=int, default=12)
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
