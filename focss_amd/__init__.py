@@ -66,6 +66,8 @@ _sig = {
     "focss_filter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
     "focss_rrc_shaping": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_double, C.c_double, _P]),
     "focss_rrc_sampling": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_double, C.c_double, _P]),
+    "focss_field_from_records": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "focss_field_to_records": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "focss_set_tile_pass": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "focss_tile_variants": (C.c_int, [C.c_int, C.c_int]),
     "focss_build_info": (C.c_char_p, []),
@@ -228,6 +230,44 @@ class Context:
     def rrc_sampling(self, signal, out, modes, samples, osf, roll_off, gain=1.0, precision=F64):
         _check(_lib.focss_rrc_sampling(self.handle, precision, modes, samples, signal.ptr, osf, roll_off, gain,
                                        out.ptr))
+
+    # ---- on-disk field records (focss_amd.formats) ----
+    def field_from_records(self, records, field, envelopes, samples, precision=F64):
+        """records: device copy of a save_field body, complex128 [samples][envelopes];
+        field: [envelopes][samples] of the given precision."""
+        _check(_lib.focss_field_from_records(self.handle, precision, envelopes, samples, records.ptr, field.ptr))
+
+    def field_to_records(self, field, records, envelopes, samples, precision=F64):
+        _check(_lib.focss_field_to_records(self.handle, precision, envelopes, samples, field.ptr, records.ptr))
+
+    def load_field(self, path, precision=F64):
+        """Reads a save_field file into a new device buffer in the field
+        layout.  Returns (buffer, header)."""
+        from .formats import read_records
+        header, rec = read_records(path)
+        envelopes = header.channels * header.modes
+        staging = self.alloc(max(rec.nbytes, 16))
+        try:
+            staging.upload(rec)
+            out = self.alloc(max(envelopes * header.samples * (16 if precision == F64 else 8), 16))
+            self.field_from_records(staging, out, envelopes, header.samples, precision)
+            self.synchronize()
+        finally:
+            staging.free()
+        return out, header
+
+    def save_field(self, path, field, channels, modes, samples, precision=F64):
+        """Writes a device field buffer [channels * modes][samples] as a
+        save_field file."""
+        from .formats import write_records
+        envelopes = channels * modes
+        staging = self.alloc(max(envelopes * samples * 16, 16))
+        try:
+            self.field_to_records(field, staging, envelopes, samples, precision)
+            rec = staging.download(np.empty((samples, envelopes), dtype=np.complex128))
+        finally:
+            staging.free()
+        write_records(path, rec, channels, modes)
 
     def sino_cubic(self, weights, radius, inp, out, modes, samples, precision=F64):
         w = np.ascontiguousarray(weights, dtype=np.complex128)

@@ -79,3 +79,59 @@ __global__ void k_from_c128(const cx<double>* __restrict__ in, cx<R>* __restrict
 }
 
 }  // namespace focss_hip
+
+namespace focss_hip {
+
+// Records of save_field / load_field (src/tasks/wdm_run.cpp:55-77,
+// src/tasks/wdm_scoring.cpp:79-103) hold complex<double> in sample-major
+// order [sample][envelope], envelope = channel * modes + mode; the field
+// layout is envelope-major [envelope][sample].  A workgroup moves a tile of
+// (IO_TILE >> te_log2) samples x (1 << te_log2) envelopes through LDS, so the
+// record side is read or written in runs of whole sample rows and the field
+// side in runs along samples.  Rows of the LDS tile are padded by one value.
+// TO_FIELD: records -> field of precision R; otherwise field -> records.
+constexpr int IO_LOG_TILE = 11;
+constexpr int IO_TILE = 1 << IO_LOG_TILE;
+constexpr int IO_NT = 256;
+
+template <typename R, bool TO_FIELD>
+__global__ __launch_bounds__(IO_NT) void k_envelope_transpose(const void* __restrict__ src, void* __restrict__ dst,
+                                                               int samples, int envelopes, int te_log2) {
+    __shared__ cx<double> tile[IO_TILE + 32];
+    const int te = 1 << te_log2, ts_log2 = IO_LOG_TILE - te_log2, ts = 1 << ts_log2, row = ts + 1;
+    const long long s0 = (long long)blockIdx.x * ts;
+    const int e0 = blockIdx.y * te;
+    if (TO_FIELD) {
+        const cx<double>* rec = (const cx<double>*)src;
+        for (int k = threadIdx.x; k < IO_TILE; k += IO_NT) {
+            const int s = k >> te_log2, e = k & (te - 1);
+            if (s0 + s < samples && e0 + e < envelopes) tile[e * row + s] = rec[(s0 + s) * envelopes + e0 + e];
+        }
+        __syncthreads();
+        cx<R>* f = (cx<R>*)dst;
+        for (int k = threadIdx.x; k < IO_TILE; k += IO_NT) {
+            const int e = k >> ts_log2, s = k & (ts - 1);
+            if (s0 + s < samples && e0 + e < envelopes) {
+                const cx<double> v = tile[e * row + s];
+                f[(long long)(e0 + e) * samples + s0 + s] = mk<R>((R)v.x, (R)v.y);
+            }
+        }
+    } else {
+        const cx<R>* f = (const cx<R>*)src;
+        for (int k = threadIdx.x; k < IO_TILE; k += IO_NT) {
+            const int e = k >> ts_log2, s = k & (ts - 1);
+            if (s0 + s < samples && e0 + e < envelopes) {
+                const cx<R> v = f[(long long)(e0 + e) * samples + s0 + s];
+                tile[e * row + s] = mk<double>((double)v.x, (double)v.y);
+            }
+        }
+        __syncthreads();
+        cx<double>* rec = (cx<double>*)dst;
+        for (int k = threadIdx.x; k < IO_TILE; k += IO_NT) {
+            const int s = k >> te_log2, e = k & (te - 1);
+            if (s0 + s < samples && e0 + e < envelopes) rec[(s0 + s) * envelopes + e0 + e] = tile[e * row + s];
+        }
+    }
+}
+
+}  // namespace focss_hip

@@ -915,6 +915,32 @@ int filter_entry(focss_ctx* c, int modes, int samples, int taps, const double* h
                           (cx<R>*)(ws + (size_t)samples * sizeof(cx<double>)), (cx<double>*)ws);
 }
 
+// records <-> field: the tile is 2^11 values, (2^11 >> te_log2) samples by
+// 2^te_log2 envelopes, te_log2 = min(ceil(log2 envelopes), 5)
+template <typename R>
+int envelope_transpose(focss_ctx* c, bool to_field, int envelopes, int samples, const void* in, void* out) {
+    int te_log2 = 0;
+    while ((1 << te_log2) < envelopes && te_log2 < 5) ++te_log2;
+    const int ts = IO_TILE >> te_log2;
+    const dim3 grid((unsigned)((samples + ts - 1) / ts), (unsigned)((envelopes + (1 << te_log2) - 1) >> te_log2));
+    if (to_field)
+        hipLaunchKernelGGL((k_envelope_transpose<R, true>), grid, dim3(IO_NT), 0, c->stream, in, out, samples,
+                           envelopes, te_log2);
+    else
+        hipLaunchKernelGGL((k_envelope_transpose<R, false>), grid, dim3(IO_NT), 0, c->stream, in, out, samples,
+                           envelopes, te_log2);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int records_check(focss_ctx* c, int envelopes, int samples, const void* in, void* out, const char* what) {
+    if (!c || !in || !out) return fail(FOCSS_E_ARG, std::string(what) + ": null argument");
+    if (envelopes < 1 || envelopes > (1 << 20) || samples < 1)
+        return fail(FOCSS_E_ARG, std::string(what) + ": envelopes must be in [1, 2^20] and samples >= 1");
+    if (in == out) return fail(FOCSS_E_ARG, std::string(what) + ": in and out must not alias");
+    return 0;
+}
+
 }  // namespace
 
 namespace {
@@ -1571,6 +1597,22 @@ int focss_rrc_sampling(focss_ctx* c, int precision, int modes, int samples, cons
     HIP_TRY(hipSetDevice(c->device));
     FOCSS_DISPATCH((rrc_sampling_impl<double>(c, modes, samples, signal, osf, roll_off, gain, out)),
                    (rrc_sampling_impl<float>(c, modes, samples, signal, osf, roll_off, gain, out)));
+}
+
+int focss_field_from_records(focss_ctx* c, int precision, int envelopes, int samples, const void* records,
+                             void* field) {
+    if (int rc = records_check(c, envelopes, samples, records, field, "field_from_records")) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((envelope_transpose<double>(c, true, envelopes, samples, records, field)),
+                   (envelope_transpose<float>(c, true, envelopes, samples, records, field)));
+}
+
+int focss_field_to_records(focss_ctx* c, int precision, int envelopes, int samples, const void* field,
+                           void* records) {
+    if (int rc = records_check(c, envelopes, samples, field, records, "field_to_records")) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    FOCSS_DISPATCH((envelope_transpose<double>(c, false, envelopes, samples, field, records)),
+                   (envelope_transpose<float>(c, false, envelopes, samples, field, records)));
 }
 
 #undef FOCSS_DISPATCH

@@ -7,10 +7,17 @@
 //                        the equalized field (rx - cubic(tx)) / w, complex<double>)
 //   focss_tasks forward <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
 //                       <max_phase_shift> <max_step> <nf_db> <seed> <center_frequency>
-// <in>/<out> are raw little-endian complex<double> files in the field layout.
+//   focss_tasks save-field <in> <set> <tag> <index> <channels> <modes> <samples>
+//                       (in: the channels' fields one after another)
+//   focss_tasks load-field <set> <tag> <index> <out>   (prints the header)
+//   focss_tasks save-transmission <tx> <rx> <csv> <modes> <samples>
+//   focss_tasks load-transmission <csv> <tx> <rx> <modes> <samples>
+// <in>/<out>/<tx>/<rx> are raw little-endian complex<double> files in the
+// field layout.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -20,6 +27,7 @@
 #include "focss/field.h"
 #include "focss/functions.h"
 #include "focss/runtime.h"
+#include "tasks/field_io.h"
 #include "tasks/systems.h"
 
 double verification(const int& n_samples, const int& n_steps);
@@ -105,6 +113,51 @@ int main(int argc, char** argv) {
         out.write(reinterpret_cast<const char*>(&lw), sizeof(lw));
         focss::Field eq = sino.equalize(rx, tx);
         out.write(reinterpret_cast<const char*>(eq.raw()), (std::streamsize)(eq.size() * sizeof(focss::complex_t)));
+        return 0;
+    }
+    if (cmd == "save-field" && argc == 9) {
+        int channels = std::atoi(argv[6]), modes = std::atoi(argv[7]), samples = std::atoi(argv[8]);
+        focss::Field all = read_field(argv[2], samples, channels * modes, 1.0, 0.0);
+        focss::Vector<focss::Field> data(channels);
+        for (int ch = 0; ch < channels; ++ch) {
+            focss::Field f(focss::Domain{modes, samples, 1.0, 0.0});
+            for (int m = 0; m < modes; ++m)
+                for (int s = 0; s < samples; ++s) f(m, s) = all(ch * modes + m, s);
+            data[ch] = f;
+        }
+        save_field(data, argv[3], argv[4], std::atoi(argv[5]));
+        return 0;
+    }
+    if (cmd == "load-field" && argc == 6) {
+        focss::FieldFileHeader h{};
+        std::ifstream in(focss::field_path(argv[2], argv[3], std::atoi(argv[4])), std::ios::binary);
+        if (!in) {
+            std::fprintf(stderr, "cannot open field file\n");
+            return 2;
+        }
+        focss::Field f;
+        try {
+            f = focss::load_field(in, &h);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+        std::cout << h.channels << ' ' << h.samples << ' ' << h.modes << ' ' << h.flags << std::endl;
+        write_field(argv[5], f);
+        return 0;
+    }
+    if ((cmd == "save-transmission" || cmd == "load-transmission") && argc == 7) {
+        int modes = std::atoi(argv[5]), samples = std::atoi(argv[6]);
+        if (cmd == "save-transmission") {
+            focss::Field tx = read_field(argv[2], samples, modes, 1.0, 0.0);
+            focss::Field rx = read_field(argv[3], samples, modes, 1.0, 0.0);
+            focss::save_transmission(std::string(argv[4]), tx, rx);
+        } else {
+            focss::Field tx(focss::Domain{modes, samples, 1.0, 0.0}), rx(focss::Domain{modes, samples, 1.0, 0.0});
+            focss::load_transmission(std::string(argv[2]), &tx, &rx);
+            write_field(argv[3], tx);
+            write_field(argv[4], rx);
+        }
         return 0;
     }
     std::fprintf(stderr, "bad arguments\n");

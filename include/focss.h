@@ -205,6 +205,20 @@ int focss_rrc_shaping(focss_ctx* ctx, int precision, int modes, int n_symbols, c
 int focss_rrc_sampling(focss_ctx* ctx, int precision, int modes, int samples, const void* signal, int osf,
                        double roll_off, double gain, void* out);
 
+/* ---- on-disk field records ----
+ * save_field / load_field files (src/tasks/wdm_run.cpp:55-77,
+ * src/tasks/wdm_scoring.cpp:79-103) hold four int32 {n_channels, n_samples,
+ * n_modes, flags} and then complex<double> values in sample-major order
+ * [sample][channel][mode].  These calls move values between that order (a
+ * device copy of the file body, `records`) and the field layout,
+ * envelope-major [channel * n_modes + mode][samples] in the given precision
+ * (the rows of load_field's matrix).  envelopes in [1, 2^20]; in and out must
+ * not alias.  Stream-ordered. */
+int focss_field_from_records(focss_ctx* ctx, int precision, int envelopes, int samples, const void* records,
+                             void* field);
+int focss_field_to_records(focss_ctx* ctx, int precision, int envelopes, int samples, const void* field,
+                           void* records);
+
 /* ---- learned few-step DBP (no counterpart in the reference) ----
  * Step j: y = complex FIR (taps, circular, centre taps/2) of x, then
  * z = y exp(i phi), phi(t) = sum_m g_{j,m} |y(t - m + kerr_taps/2)|^2 (a real
