@@ -79,6 +79,7 @@ _sig = {
     "focss_sino_cubic": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P]),
     "focss_sino_train": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "focss_wdm_ppe": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P]),
+    "focss_wdm_ppe_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     "focss_ldbp_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "focss_ldbp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "focss_ldbp_infer": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
@@ -289,6 +290,17 @@ class Context:
         dm = np.ascontiguousarray(Dm, dtype=np.complex128)
         _check(_lib.focss_wdm_ppe(self.handle, precision, c_radius, cm.ctypes.data, d_radius, dm.ctypes.data,
                                   samples, a6.ptr, spm.ptr, xpm.ptr))
+
+    def wdm_ppe_train(self, c_radius, d_radius, tx6, rx6, samples):
+        """WdmPPE::train on complex128 device envelopes [6][samples].
+        Returns (C (2c+1)^2 or None, D (2d+1)^2 or None, intercept, linear weight)."""
+        cm = np.zeros((max(2 * c_radius + 1, 1),) * 2, dtype=np.complex128)
+        dm = np.zeros((max(2 * d_radius + 1, 1),) * 2, dtype=np.complex128)
+        ic = np.zeros(1, dtype=np.complex128)
+        lw = np.zeros(1, dtype=np.complex128)
+        _check(_lib.focss_wdm_ppe_train(self.handle, c_radius, d_radius, samples, tx6.ptr, rx6.ptr, cm.ctypes.data,
+                                        dm.ctypes.data, ic.ctypes.data, lw.ctypes.data))
+        return (cm if c_radius >= 0 else None, dm if d_radius >= 0 else None, ic[0], lw[0])
 
     def add_noise(self, buf, modes, samples, sigma, seed, stream, precision=F64):
         _check(_lib.focss_add_noise(self.handle, precision, modes, samples, sigma, seed, stream, buf.ptr))

@@ -745,6 +745,96 @@ int cholesky_solve(std::vector<cx<double>>& A, int n, std::vector<cx<double>>& b
     return 0;
 }
 
+// the folded weights (one per class {(m,n), (n,m), (-m,-n), (-n,-m)}, m in
+// [-R/2, R/2], n in [|m|, R - |m|]) written into all four places of the
+// (2R+1)^2 matrix, interleaved doubles              sino.cpp:124-153
+void unfold_symmetric(const std::vector<cx<double>>& b, int* index, int radius, double* out) {
+    const int side = 2 * radius + 1;
+    for (int i = 0; i < 2 * side * side; ++i) out[i] = 0;
+    auto put = [&](int m, int n, cx<double> v) {
+        double* e = out + 2 * ((size_t)(radius + m) * side + (radius + n));
+        e[0] = v.x;
+        e[1] = v.y;
+    };
+    const int mb = radius / 2;
+    for (int m = -mb; m <= mb; ++m) {
+        const int am = m < 0 ? -m : m;
+        for (int n = am; n <= radius - am; ++n) {
+            cx<double> v = b[(*index)++];
+            put(m, n, v);
+            if (m == 0 && n == 0) continue;
+            if (m == n) {
+                put(-m, -n, v);
+            } else if (m == -n) {
+                put(n, m, v);
+            } else {
+                put(n, m, v);
+                put(-m, -n, v);
+                put(-n, -m, v);
+            }
+        }
+    }
+}
+
+// The least squares of SINO::train / WdmPPE::train (sino.cpp:105-122,
+// wdm_sino.cpp:84-100) from device features F (column-major rows x nf) and
+// responses y: F^H F, column sums and F^H y on the device, then on the host
+// z-scoring (std with 1/rows, arma stddev norm_type 1), centring, the normal
+// equations by Cholesky, de-standardisation and the intercept.
+int zscore_normal_solve(focss_ctx* c, const cx<double>* F, const cx<double>* y, long long rows, int nf,
+                        std::vector<cx<double>>* weights, cx<double>* intercept) {
+    cx<double>* G = nullptr;
+    cx<double>* sums = nullptr;
+    cx<double>* fy = nullptr;
+    hipError_t e = hipMalloc(&G, (size_t)nf * nf * sizeof(cx<double>));
+    if (e == hipSuccess) e = hipMalloc(&sums, (size_t)(nf + 1) * sizeof(cx<double>));
+    if (e == hipSuccess) e = hipMalloc(&fy, (size_t)nf * sizeof(cx<double>));
+    std::vector<cx<double>> hG((size_t)nf * nf), hs(nf + 1), hfy(nf);
+    if (e == hipSuccess) {
+        dim3 gg((nf + 31) / 32, (nf + 31) / 32);
+        hipLaunchKernelGGL(k_gram, gg, dim3(256), 0, c->stream, F, G, rows, nf);
+        hipLaunchKernelGGL(k_col_sums, dim3(nf + 1), dim3(256), 0, c->stream, F, y, rows, nf, sums, fy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hG.data(), G, hG.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hs.data(), sums, hs.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hfy.data(), fy, hfy.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(G);
+    (void)hipFree(sums);
+    (void)hipFree(fy);
+    if (e != hipSuccess) return fail(FOCSS_E_HIP, std::string("triplet least squares: ") + hipGetErrorString(e));
+    const double inv = 1.0 / (double)rows;
+    std::vector<cx<double>> mu(nf);
+    std::vector<double> sd(nf);
+    for (int j = 0; j < nf; ++j) {
+        mu[j] = hs[j] * inv;
+        double var = hG[(size_t)j * nf + j].x * inv - norm2(mu[j]);
+        sd[j] = std::sqrt(var > 0 ? var : 0.0);
+    }
+    const cx<double> ymean = hs[nf] * inv;
+    std::vector<cx<double>> A((size_t)nf * nf), b(nf);
+    for (int i = 0; i < nf; ++i) {
+        for (int j = 0; j < nf; ++j) {
+            cx<double> v = hG[(size_t)i * nf + j] - conj(mu[i]) * mu[j] * (double)rows;
+            A[(size_t)i * nf + j] = v * (1.0 / (sd[i] * sd[j]));
+        }
+        b[i] = (hfy[i] - conj(mu[i]) * ymean * (double)rows) * (1.0 / sd[i]);
+    }
+    if (int rc = cholesky_solve(A, nf, b)) return rc;
+    cx<double> ic = ymean;
+    for (int j = 0; j < nf; ++j) {
+        b[j] = b[j] * (1.0 / sd[j]);
+        ic = ic - b[j] * mu[j];
+    }
+    *intercept = ic;
+    weights->swap(b);
+    return 0;
+}
+
 }  // namespace
 
 namespace {
@@ -1244,87 +1334,26 @@ int focss_sino_train(focss_ctx* c, int radius, int type, const double* linear_we
     if (rows < nf) return fail(FOCSS_E_ARG, "sino_train: fewer rows than features");
     cx<double>* F = nullptr;
     cx<double>* y = nullptr;
-    cx<double>* G = nullptr;
-    cx<double>* sums = nullptr;
-    cx<double>* fy = nullptr;
     HIP_TRY(hipMalloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
-    HIP_TRY(hipMalloc(&y, (size_t)rows * sizeof(cx<double>)));
-    HIP_TRY(hipMalloc(&G, (size_t)nf * nf * sizeof(cx<double>)));
-    HIP_TRY(hipMalloc(&sums, (size_t)(nf + 1) * sizeof(cx<double>)));
-    HIP_TRY(hipMalloc(&fy, (size_t)nf * sizeof(cx<double>)));
+    if (hipMalloc(&y, (size_t)rows * sizeof(cx<double>)) != hipSuccess) {
+        (void)hipFree(F);
+        return fail(FOCSS_E_HIP, "sino_train: out of device memory");
+    }
     cx<double> lw = mk<double>(linear_weight_in ? linear_weight_in[0] : 0.0, linear_weight_in ? linear_weight_in[1] : 0.0);
     int fblocks = (int)std::min<long long>((rows + 255) / 256, 4096);
     hipLaunchKernelGGL(k_sino_features, dim3(fblocks), dim3(256), 0, c->stream, (const cx<double>*)tx,
                        (const cx<double>*)rx, F, y, radius, type, lw, modes, samples);
-    dim3 gg((nf + 31) / 32, (nf + 31) / 32);
-    hipLaunchKernelGGL(k_gram, gg, dim3(256), 0, c->stream, (const cx<double>*)F, G, rows, nf);
-    hipLaunchKernelGGL(k_col_sums, dim3(nf + 1), dim3(256), 0, c->stream, (const cx<double>*)F, (const cx<double>*)y,
-                       rows, nf, sums, fy);
-    HIP_TRY(hipGetLastError());
-    std::vector<cx<double>> hG((size_t)nf * nf), hs(nf + 1), hfy(nf);
-    HIP_TRY(hipMemcpyAsync(hG.data(), G, hG.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hs.data(), sums, hs.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hfy.data(), fy, hfy.size() * sizeof(cx<double>), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<cx<double>> b;
+    cx<double> ic;
+    int rc = zscore_normal_solve(c, F, y, rows, nf, &b, &ic);
     (void)hipFree(F);
     (void)hipFree(y);
-    (void)hipFree(G);
-    (void)hipFree(sums);
-    (void)hipFree(fy);
-    // z-score (std with 1/rows), centred responses, normal equations
-    // (sino.cpp:105-114 with the least squares as (F^H F) x = F^H y)
-    const double inv = 1.0 / (double)rows;
-    std::vector<cx<double>> mu(nf);
-    std::vector<double> sd(nf);
-    for (int j = 0; j < nf; ++j) {
-        mu[j] = hs[j] * inv;
-        double var = hG[(size_t)j * nf + j].x * inv - norm2(mu[j]);
-        sd[j] = std::sqrt(var > 0 ? var : 0.0);
-    }
-    const cx<double> ymean = hs[nf] * inv;
-    std::vector<cx<double>> A((size_t)nf * nf), b(nf);
-    for (int i = 0; i < nf; ++i) {
-        for (int j = 0; j < nf; ++j) {
-            cx<double> v = hG[(size_t)i * nf + j] - conj(mu[i]) * mu[j] * (double)rows;
-            A[(size_t)i * nf + j] = v * (1.0 / (sd[i] * sd[j]));
-        }
-        b[i] = (hfy[i] - conj(mu[i]) * ymean * (double)rows) * (1.0 / sd[i]);
-    }
-    if (int rc = cholesky_solve(A, nf, b)) return rc;
-    // de-standardise, intercept, unfold                  sino.cpp:122-153
-    cx<double> ic = ymean;
-    for (int j = 0; j < nf; ++j) {
-        b[j] = b[j] * (1.0 / sd[j]);
-        ic = ic - b[j] * mu[j];
-    }
+    if (rc) return rc;
+    // unfold                                           sino.cpp:124-153
     intercept[0] = ic.x;
     intercept[1] = ic.y;
-    const int side = 2 * radius + 1;
-    for (int i = 0; i < 2 * side * side; ++i) weights[i] = 0;
-    auto put = [&](int m, int n, cx<double> v) {
-        double* e = weights + 2 * ((size_t)(radius + m) * side + (radius + n));
-        e[0] = v.x;
-        e[1] = v.y;
-    };
     int index = 0;
-    const int mb = radius / 2;
-    for (int m = -mb; m <= mb; ++m) {
-        const int am = m < 0 ? -m : m;
-        for (int n = am; n <= radius - am; ++n) {
-            cx<double> v = b[index++];
-            put(m, n, v);
-            if (m == 0 && n == 0) continue;
-            if (m == n) {
-                put(-m, -n, v);
-            } else if (m == -n) {
-                put(n, m, v);
-            } else {
-                put(n, m, v);
-                put(-m, -n, v);
-                put(-n, -m, v);
-            }
-        }
-    }
+    unfold_symmetric(b, &index, radius, weights);
     if (type == 0) {
         linear_weight[0] = b[index].x;
         linear_weight[1] = b[index].y;
@@ -1335,6 +1364,58 @@ int focss_sino_train(focss_ctx* c, int radius, int type, const double* linear_we
         linear_weight[0] = lw.x;
         linear_weight[1] = lw.y;
     }
+    return 0;
+}
+
+int focss_wdm_ppe_train(focss_ctx* c, int c_radius, int d_radius, int samples, const void* tx6, const void* rx6,
+                        double* C, double* D, double* intercept, double* linear_weight) {
+    const int r = std::max(c_radius, d_radius);
+    if (!c || !tx6 || !rx6 || !intercept || !linear_weight || c_radius < -1 || d_radius < -1 ||
+        (c_radius >= 0 && !C) || (d_radius >= 0 && !D) || samples <= 2 * std::max(r, 0))
+        return fail(FOCSS_E_ARG, "wdm_ppe_train: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const int n_spm = c_radius < 0 ? 0 : c_radius * (c_radius + 2) / 2 + 1;
+    const int n_xpm = d_radius < 0 ? 0 : 2 * d_radius * d_radius + 2 * d_radius + 1;
+    const int nf = n_spm + n_xpm + 1;
+    const long long rows = 2LL * (samples - 2 * std::max(r, 0));
+    if (rows < nf) return fail(FOCSS_E_ARG, "wdm_ppe_train: fewer rows than features");
+    cx<double>* F = nullptr;
+    cx<double>* y = nullptr;
+    HIP_TRY(hipMalloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
+    if (hipMalloc(&y, (size_t)rows * sizeof(cx<double>)) != hipSuccess) {
+        (void)hipFree(F);
+        return fail(FOCSS_E_HIP, "wdm_ppe_train: out of device memory");
+    }
+    int fblocks = (int)std::min<long long>((rows + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_wdmppe_features, dim3(fblocks), dim3(256), 0, c->stream, (const cx<double>*)tx6,
+                       (const cx<double>*)rx6, F, y, c_radius, d_radius, samples);
+    std::vector<cx<double>> b;
+    cx<double> ic;
+    int rc = zscore_normal_solve(c, F, y, rows, nf, &b, &ic);
+    (void)hipFree(F);
+    (void)hipFree(y);
+    if (rc) return rc;
+    // unfold C (the SINO fold), D (the diamond, m outer), linear weight last
+    //                                                   wdm_sino.cpp:102-137
+    intercept[0] = ic.x;
+    intercept[1] = ic.y;
+    int index = 0;
+    if (c_radius >= 0) unfold_symmetric(b, &index, c_radius, C);
+    if (d_radius >= 0) {
+        const int side = 2 * d_radius + 1;
+        for (int i = 0; i < 2 * side * side; ++i) D[i] = 0;
+        for (int m = -d_radius; m <= d_radius; ++m) {
+            const int bd = d_radius - (m < 0 ? -m : m);
+            for (int n = -bd; n <= bd; ++n) {
+                double* e = D + 2 * ((size_t)(d_radius + m) * side + (d_radius + n));
+                e[0] = b[index].x;
+                e[1] = b[index].y;
+                ++index;
+            }
+        }
+    }
+    linear_weight[0] = b[index].x;
+    linear_weight[1] = b[index].y;
     return 0;
 }
 

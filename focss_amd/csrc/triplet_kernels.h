@@ -12,6 +12,7 @@
 //  k_gram        : G = F^H F over a column-major complex matrix, 32x32 tiles.
 //  k_col_sums    : sum_r F(r, j), sum_r conj(F(r, j)) y(r), sum_r y(r).
 //  k_wdm_ppe     : WdmPPE::spm + WdmPPE::xpm over 6 envelopes (wdm_sino.cpp:256-295).
+//  k_wdmppe_features: the feature rows of WdmPPE::train (wdm_sino.cpp:29-81).
 #pragma once
 #include "cx.h"
 
@@ -178,6 +179,33 @@ __device__ __forceinline__ cx<R> at6(const cx<R>* A6, int samples, int r, int k)
     return A6[(long long)r * samples + k];
 }
 
+// WdmPPE::T (wdm_sino.cpp:155-164): A_{2+pol}(k+m) sum_{p=2,3} A_p(k+n) conj(A_p(k+m+n))
+template <typename R>
+__device__ __forceinline__ cx<R> ppe_spm_term(const cx<R>* A6, int samples, int pol, int k, int m, int n) {
+    cx<R> t = at6(A6, samples, 2, k + n) * conj(at6(A6, samples, 2, k + m + n)) +
+              at6(A6, samples, 3, k + n) * conj(at6(A6, samples, 3, k + m + n));
+    return at6(A6, samples, 2 + pol, k + m) * t;
+}
+
+// WdmPPE::T1(m, n) + WdmPPE::T3(m, -n) (wdm_sino.cpp:166-200): the XPM terms
+// of the channels below (rows 0..1) and above (rows 4..5) the COI
+template <typename R>
+__device__ __forceinline__ cx<R> ppe_xpm_term(const cx<R>* A6, int samples, int pol, int k, int m, int n) {
+    const int a = 2 + pol, o = 3 - pol;      // this / other COI polarisation
+    const int c0 = pol, c1 = 1 - pol;        // channel 0 rows
+    const int e0 = 4 + pol, e1 = 5 - pol;    // channel 2 rows
+    cx<R> t1 = (at6(A6, samples, a, k + m) * at6(A6, samples, c0, k + n) * conj(at6(A6, samples, c0, k + m + n))) *
+                   (R)2 +
+               at6(A6, samples, a, k + m) * at6(A6, samples, c1, k + n) * conj(at6(A6, samples, c1, k + m + n)) +
+               at6(A6, samples, o, k + m) * at6(A6, samples, c0, k + n) * conj(at6(A6, samples, c1, k + m + n));
+    const int nn = -n;
+    cx<R> t3 = (at6(A6, samples, a, k + m) * at6(A6, samples, e0, k + nn) *
+                conj(at6(A6, samples, e0, k + m + nn))) * (R)2 +
+               at6(A6, samples, a, k + m) * at6(A6, samples, e1, k + nn) * conj(at6(A6, samples, e1, k + m + nn)) +
+               at6(A6, samples, o, k + m) * at6(A6, samples, e0, k + nn) * conj(at6(A6, samples, e1, k + m + nn));
+    return t1 + t3;
+}
+
 template <typename R>
 __global__ __launch_bounds__(TRIP_NT) void k_wdm_ppe(const cx<R>* __restrict__ A6, const cx<R>* __restrict__ cdiam,
                                                      int c_radius, const cx<R>* __restrict__ ddiam, int d_radius,
@@ -190,43 +218,62 @@ __global__ __launch_bounds__(TRIP_NT) void k_wdm_ppe(const cx<R>* __restrict__ A
             int idx = 0;
             for (int n = -c_radius; n <= c_radius; ++n) {
                 const int b = c_radius - (n < 0 ? -n : n);
-                for (int m = -b; m <= b; ++m, ++idx) {
-                    cx<R> t = at6(A6, samples, 2, k + n) * conj(at6(A6, samples, 2, k + m + n)) +
-                              at6(A6, samples, 3, k + n) * conj(at6(A6, samples, 3, k + m + n));
-                    acc = acc + cdiam[idx] * (at6(A6, samples, 2 + pol, k + m) * t);
-                }
+                for (int m = -b; m <= b; ++m, ++idx)
+                    acc = acc + cdiam[idx] * ppe_spm_term(A6, samples, pol, k, m, n);
             }
         }
         spm[(long long)pol * samples + k] = acc;
         cx<R> acx = mk<R>(0, 0);
         if (d_radius >= 0 && k >= d_radius && k < samples - d_radius) {
-            const int a = 2 + pol, o = 3 - pol;      // this / other COI polarisation
-            const int c0 = pol, c1 = 1 - pol;        // channel 0 rows
-            const int e0 = 4 + pol, e1 = 5 - pol;    // channel 2 rows
             int idx = 0;
             for (int m = -d_radius; m <= d_radius; ++m) {
                 const int b = d_radius - (m < 0 ? -m : m);
-                for (int n = -b; n <= b; ++n, ++idx) {
-                    // T1(pol, k, m, n)
-                    cx<R> t1 = (at6(A6, samples, a, k + m) * at6(A6, samples, c0, k + n) *
-                                conj(at6(A6, samples, c0, k + m + n))) * (R)2 +
-                               at6(A6, samples, a, k + m) * at6(A6, samples, c1, k + n) *
-                                   conj(at6(A6, samples, c1, k + m + n)) +
-                               at6(A6, samples, o, k + m) * at6(A6, samples, c0, k + n) *
-                                   conj(at6(A6, samples, c1, k + m + n));
-                    // T3(pol, k, m, -n)
-                    const int nn = -n;
-                    cx<R> t3 = (at6(A6, samples, a, k + m) * at6(A6, samples, e0, k + nn) *
-                                conj(at6(A6, samples, e0, k + m + nn))) * (R)2 +
-                               at6(A6, samples, a, k + m) * at6(A6, samples, e1, k + nn) *
-                                   conj(at6(A6, samples, e1, k + m + nn)) +
-                               at6(A6, samples, o, k + m) * at6(A6, samples, e0, k + nn) *
-                                   conj(at6(A6, samples, e1, k + m + nn));
-                    acx = acx + ddiam[idx] * (t1 + t3);
-                }
+                for (int n = -b; n <= b; ++n, ++idx) acx = acx + ddiam[idx] * ppe_xpm_term(A6, samples, pol, k, m, n);
             }
         }
         xpm[(long long)pol * samples + k] = acx;
+    }
+}
+
+// WdmPPE::train features (wdm_sino.cpp:39-81), one thread per row; rows
+// pol-major (row = pol * (samples - 2r) + k - r, r = max(c, d)), F
+// column-major [row + rows * j]: the folded SPM set (the fold of SINO::train),
+// the XPM diamond (m outer), the COI sample; y = rx rows 2..3.
+__global__ void k_wdmppe_features(const cx<double>* __restrict__ tx6, const cx<double>* __restrict__ rx6,
+                                  cx<double>* __restrict__ F, cx<double>* __restrict__ y, int c_radius,
+                                  int d_radius, int samples) {
+    const int r = c_radius > d_radius ? c_radius : d_radius;
+    const int per_pol = samples - 2 * r;
+    const long long rows = 2LL * per_pol;
+    for (long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x; row < rows;
+         row += (long long)gridDim.x * blockDim.x) {
+        const int pol = (int)(row / per_pol);
+        const int k = (int)(row % per_pol) + r;
+        y[row] = rx6[(long long)(2 + pol) * samples + k];
+        int index = 0;
+        const int mb = c_radius / 2;
+        for (int m = -mb; m <= mb; ++m) {
+            const int am = m < 0 ? -m : m;
+            for (int n = am; n <= c_radius - am; ++n) {
+                cx<double> v;
+                if (m == 0 && n == 0) {
+                    v = ppe_spm_term(tx6, samples, pol, k, 0, 0);
+                } else if (m == n) {
+                    v = ppe_spm_term(tx6, samples, pol, k, m, n) + ppe_spm_term(tx6, samples, pol, k, -m, -n);
+                } else if (m == -n) {
+                    v = ppe_spm_term(tx6, samples, pol, k, m, n) + ppe_spm_term(tx6, samples, pol, k, n, m);
+                } else {
+                    v = ppe_spm_term(tx6, samples, pol, k, m, n) + ppe_spm_term(tx6, samples, pol, k, n, m) +
+                        ppe_spm_term(tx6, samples, pol, k, -m, -n) + ppe_spm_term(tx6, samples, pol, k, -n, -m);
+                }
+                F[row + rows * index++] = v;
+            }
+        }
+        for (int m = -d_radius; m <= d_radius; ++m) {
+            const int b = d_radius - (m < 0 ? -m : m);
+            for (int n = -b; n <= b; ++n) F[row + rows * index++] = ppe_xpm_term(tx6, samples, pol, k, m, n);
+        }
+        F[row + rows * index] = tx6[(long long)(2 + pol) * samples + k];
     }
 }
 

@@ -172,6 +172,17 @@ int focss_sino_train(focss_ctx* ctx, int radius, int type, const double* linear_
 int focss_wdm_ppe(focss_ctx* ctx, int precision, int c_radius, const double* C, int d_radius,
                   const double* D, int samples, const void* a6, void* spm, void* xpm);
 
+/* WdmPPE::train (src/focss/equalizer/wdm_sino.cpp:29-141): tx6, rx6 are
+ * complex<double> device envelopes [6][samples] (the rows of load_field's
+ * matrix for three dual-polarisation channels).  Rows pol-major over samples
+ * [r, samples - r), r = max(c, d); features: the folded SPM set of radius c,
+ * T1(m, n) + T3(m, -n) over the d-diamond, the COI sample.  Feature matrix
+ * and F^H F on the device, z-scored normal equations by Cholesky on the
+ * host.  Outputs (host, interleaved doubles): C (2c+1)^2 and D (2d+1)^2
+ * (skipped for radius -1), intercept, linear weight. */
+int focss_wdm_ppe_train(focss_ctx* ctx, int c_radius, int d_radius, int samples, const void* tx6,
+                        const void* rx6, double* C, double* D, double* intercept, double* linear_weight);
+
 /* ---- transmitter / receiver DSP on the device ----
  * Fields are mode-major device buffers [modes][samples] of the given
  * precision, samples a power of two <= 2^20.  Each call is stream-ordered

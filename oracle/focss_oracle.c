@@ -825,20 +825,13 @@ int orc_complex_lstsq(int rows, int cols, const orc_c128* a_in, const orc_c128* 
     return 0;
 }
 
-/* SINO::train: z-score (std with 1/N, arma stddev norm_type 1), centre,
-   least squares, de-standardise, intercept, unfold into the symmetric
-   (2R+1)^2 weights                                sino.cpp:105-153 */
-int orc_sino_train(int radius, int type, orc_c128 linear_weight_in, int modes, int samples,
-                   const orc_c128* tx, const orc_c128* rx, orc_c128* weights,
-                   orc_c128* intercept, orc_c128* linear_weight) {
-    int nf = orc_sino_n_features(radius, type);
-    long rows = (long)modes * (samples - 2 * radius);
-    orc_c128* feat = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)rows * nf);
-    orc_c128* resp = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)rows);
-    orc_c128* raw = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)nf);
+/* z-score the feature columns (std with 1/N, arma stddev norm_type 1),
+   centre the responses, least squares, de-standardise, intercept
+   (sino.cpp:105-122, wdm_sino.cpp:84-100).  feat and resp are overwritten. */
+static int zscore_lstsq(long rows, int nf, orc_c128* feat, orc_c128* resp, orc_c128* raw,
+                        orc_c128* intercept) {
     orc_c128* mean = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)nf);
     double* sd = (double*)malloc(sizeof(double) * (size_t)nf);
-    orc_sino_features(radius, type, linear_weight_in, modes, samples, tx, rx, feat, resp);
     for (int j = 0; j < nf; ++j) {
         orc_c128 mu = {0, 0};
         for (long i = 0; i < rows; ++i) mu = c_add(mu, feat[i + rows * j]);
@@ -860,6 +853,24 @@ int orc_sino_train(int radius, int type, orc_c128 linear_weight_in, int modes, i
     orc_c128 ic = ymean;
     for (int j = 0; j < nf; ++j) ic = c_sub(ic, c_mul(raw[j], mean[j]));
     *intercept = ic;
+    free(sd);
+    free(mean);
+    return rc;
+}
+
+/* SINO::train: z-score (std with 1/N, arma stddev norm_type 1), centre,
+   least squares, de-standardise, intercept, unfold into the symmetric
+   (2R+1)^2 weights                                sino.cpp:105-153 */
+int orc_sino_train(int radius, int type, orc_c128 linear_weight_in, int modes, int samples,
+                   const orc_c128* tx, const orc_c128* rx, orc_c128* weights,
+                   orc_c128* intercept, orc_c128* linear_weight) {
+    int nf = orc_sino_n_features(radius, type);
+    long rows = (long)modes * (samples - 2 * radius);
+    orc_c128* feat = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)rows * nf);
+    orc_c128* resp = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)rows);
+    orc_c128* raw = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)nf);
+    orc_sino_features(radius, type, linear_weight_in, modes, samples, tx, rx, feat, resp);
+    int rc = zscore_lstsq(rows, nf, feat, resp, raw, intercept);
     int w = 2 * radius + 1;
     memset(weights, 0, sizeof(orc_c128) * (size_t)w * w);
 #define W_AT(i, j) weights[(i) * w + (j)]
@@ -892,8 +903,6 @@ int orc_sino_train(int radius, int type, orc_c128 linear_weight_in, int modes, i
     } else {
         *linear_weight = linear_weight_in;
     }
-    free(sd);
-    free(mean);
     free(raw);
     free(resp);
     free(feat);
@@ -976,6 +985,103 @@ void orc_wdmppe_xpm(int d_radius, const orc_c128* D, int samples, const orc_c128
             }
             out2[(long)pol * samples + k] = acc;
         }
+}
+
+int orc_wdmppe_n_features(int c_radius, int d_radius) {
+    int spm = c_radius < 0 ? 0 : c_radius * (c_radius + 2) / 2 + 1;
+    int xpm = d_radius < 0 ? 0 : 2 * d_radius * d_radius + 2 * d_radius + 1;
+    return spm + xpm + 1;
+}
+
+/* WdmPPE::train features and responses            wdm_sino.cpp:29-81.
+   Rows pol-major: row = pol * (samples - 2r) + (k - r), r = max(c, d);
+   features column-major rows x n_features: the folded SPM set (as SINO,
+   on the COI pair), XPM T1(m, n) + T3(m, -n) over the d-diamond (m outer),
+   then the COI sample itself.  responses: rx rows 2..3. */
+void orc_wdmppe_features(int c_radius, int d_radius, int samples, const orc_c128* tx6,
+                         const orc_c128* rx6, orc_c128* features, orc_c128* responses) {
+    int r = c_radius > d_radius ? c_radius : d_radius;
+    long per_pol = samples - 2 * r;
+    long rows = 2 * per_pol;
+    for (int pol = 0; pol < 2; ++pol)
+        for (int k = r; k < samples - r; ++k) {
+            long row = pol * per_pol + (k - r);
+            responses[row] = rx6[(long)(2 + pol) * samples + k];
+            int index = 0;
+            int mb = c_radius / 2;
+            for (int m = -mb; m <= mb; ++m)
+                for (int n = abs(m); n <= c_radius - abs(m); ++n) {
+                    orc_c128 v;
+                    if (m == 0 && n == 0) {
+                        v = wdm_T(tx6, samples, pol, k, 0, 0);
+                    } else if (m == n) {
+                        v = c_add(wdm_T(tx6, samples, pol, k, m, n), wdm_T(tx6, samples, pol, k, -m, -n));
+                    } else if (m == -n) {
+                        v = c_add(wdm_T(tx6, samples, pol, k, m, n), wdm_T(tx6, samples, pol, k, n, m));
+                    } else {
+                        v = c_add(c_add(wdm_T(tx6, samples, pol, k, m, n), wdm_T(tx6, samples, pol, k, n, m)),
+                                  c_add(wdm_T(tx6, samples, pol, k, -m, -n), wdm_T(tx6, samples, pol, k, -n, -m)));
+                    }
+                    features[row + rows * index++] = v;
+                }
+            for (int m = -d_radius; m <= d_radius; ++m) {
+                int b = d_radius - abs(m);
+                for (int n = -b; n <= b; ++n)
+                    features[row + rows * index++] =
+                        c_add(wdm_T1(tx6, samples, pol, k, m, n), wdm_T3(tx6, samples, pol, k, m, -n));
+            }
+            features[row + rows * index] = tx6[(long)(2 + pol) * samples + k];
+        }
+}
+
+/* WdmPPE::train: z-score, least squares, de-standardise, unfold C (the SINO
+   fold) and D (the diamond), linear weight last     wdm_sino.cpp:84-141.
+   C (2c+1)^2 and D (2d+1)^2 row-major, written when the radius is >= 0. */
+int orc_wdmppe_train(int c_radius, int d_radius, int samples, const orc_c128* tx6, const orc_c128* rx6,
+                     orc_c128* C, orc_c128* D, orc_c128* intercept, orc_c128* linear_weight) {
+    int r = c_radius > d_radius ? c_radius : d_radius;
+    int nf = orc_wdmppe_n_features(c_radius, d_radius);
+    long rows = 2L * (samples - 2 * r);
+    if (rows < nf) return -1;
+    orc_c128* feat = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)rows * nf);
+    orc_c128* resp = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)rows);
+    orc_c128* raw = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)nf);
+    orc_wdmppe_features(c_radius, d_radius, samples, tx6, rx6, feat, resp);
+    int rc = zscore_lstsq(rows, nf, feat, resp, raw, intercept);
+    int index = 0;
+    if (c_radius >= 0) {
+        int w = 2 * c_radius + 1, R = c_radius;
+        memset(C, 0, sizeof(orc_c128) * (size_t)w * w);
+        int mb = c_radius / 2;
+        for (int m = -mb; m <= mb; ++m)
+            for (int n = abs(m); n <= c_radius - abs(m); ++n) {
+                orc_c128 v = raw[index++];
+                C[(R + m) * w + R + n] = v;
+                if (m == 0 && n == 0) continue;
+                if (m == n) {
+                    C[(R - m) * w + R - n] = v;
+                } else if (m == -n) {
+                    C[(R + n) * w + R + m] = v;
+                } else {
+                    C[(R + n) * w + R + m] = v;
+                    C[(R - m) * w + R - n] = v;
+                    C[(R - n) * w + R - m] = v;
+                }
+            }
+    }
+    if (d_radius >= 0) {
+        int w = 2 * d_radius + 1;
+        memset(D, 0, sizeof(orc_c128) * (size_t)w * w);
+        for (int m = -d_radius; m <= d_radius; ++m) {
+            int b = d_radius - abs(m);
+            for (int n = -b; n <= b; ++n) D[(d_radius + m) * w + d_radius + n] = raw[index++];
+        }
+    }
+    *linear_weight = raw[index];
+    free(raw);
+    free(resp);
+    free(feat);
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */

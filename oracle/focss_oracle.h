@@ -125,6 +125,11 @@ void orc_wdmppe_spm(int c_radius, const orc_c128* C, int samples, const orc_c128
                     orc_c128* out2);
 void orc_wdmppe_xpm(int d_radius, const orc_c128* D, int samples, const orc_c128* A6,
                     orc_c128* out2);
+int orc_wdmppe_n_features(int c_radius, int d_radius);
+void orc_wdmppe_features(int c_radius, int d_radius, int samples, const orc_c128* tx6,
+                         const orc_c128* rx6, orc_c128* features, orc_c128* responses);
+int orc_wdmppe_train(int c_radius, int d_radius, int samples, const orc_c128* tx6, const orc_c128* rx6,
+                     orc_c128* C, orc_c128* D, orc_c128* intercept, orc_c128* linear_weight);
 
 /* ---- learned few-step DBP (no reference counterpart; judged by plain DBP) ----
  * fir [steps][taps] complex, kerr [steps][ktaps] real intensity taps of the

@@ -110,6 +110,9 @@ _sig = {
     "orc_complex_lstsq": (C.c_int, [C.c_int, C.c_int, _P, _P, _P]),
     "orc_wdmppe_spm": (None, [C.c_int, _P, C.c_int, _P, _P]),
     "orc_wdmppe_xpm": (None, [C.c_int, _P, C.c_int, _P, _P]),
+    "orc_wdmppe_n_features": (C.c_int, [C.c_int, C.c_int]),
+    "orc_wdmppe_features": (None, [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "orc_wdmppe_train": (C.c_int, [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_ldbp_forward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "orc_ldbp_backward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
 }
@@ -406,6 +409,35 @@ def wdmppe_xpm(d_radius, Dm, A6):
     out = np.zeros(2 * samples, dtype=np.complex128)
     _lib.orc_wdmppe_xpm(d_radius, _ptr(Dm), samples, _ptr(A6), _ptr(out))
     return out.reshape(2, samples)
+
+
+def wdmppe_features(c_radius, d_radius, tx6, rx6):
+    """(features [rows][n_features], responses [rows]); rows pol-major."""
+    tx6, rx6 = _c(tx6), _c(rx6)
+    samples = tx6.size // 6
+    r = max(c_radius, d_radius)
+    rows = 2 * (samples - 2 * r)
+    nf = _lib.orc_wdmppe_n_features(c_radius, d_radius)
+    f = np.zeros(rows * nf, dtype=np.complex128)
+    y = np.zeros(rows, dtype=np.complex128)
+    _lib.orc_wdmppe_features(c_radius, d_radius, samples, _ptr(tx6), _ptr(rx6), _ptr(f), _ptr(y))
+    return f.reshape(nf, rows).T, y
+
+
+def wdmppe_train(c_radius, d_radius, tx6, rx6):
+    """WdmPPE::train: (C (2c+1)^2, D (2d+1)^2, intercept, linear weight)."""
+    tx6, rx6 = _c(tx6), _c(rx6)
+    samples = tx6.size // 6
+    Cm = np.zeros(max(2 * c_radius + 1, 1) ** 2, dtype=np.complex128)
+    Dm = np.zeros(max(2 * d_radius + 1, 1) ** 2, dtype=np.complex128)
+    ic = np.zeros(1, dtype=np.complex128)
+    lw = np.zeros(1, dtype=np.complex128)
+    rc = _lib.orc_wdmppe_train(c_radius, d_radius, samples, _ptr(tx6), _ptr(rx6), _ptr(Cm), _ptr(Dm), _ptr(ic),
+                               _ptr(lw))
+    if rc:
+        raise ValueError("orc_wdmppe_train failed")
+    return (Cm.reshape(2 * c_radius + 1, -1) if c_radius >= 0 else None,
+            Dm.reshape(2 * d_radius + 1, -1) if d_radius >= 0 else None, ic[0], lw[0])
 
 
 # ---------------- learned DBP ----------------

@@ -378,6 +378,92 @@ def test_wdmppe_delta_weights_closed_form():
     assert np.allclose(xpm[1, k], -0.25j * ref1[k], rtol=1e-12)
 
 
+def _ppe_terms_brute(A, pol, k, m, n):
+    """T(m, n) and T1(m, n) + T3(m, -n) written out from wdm_sino.cpp:155-200."""
+    a0, a1, a2, a3, a4, a5 = A
+    t = A[2 + pol, k + m] * (a2[k + n] * np.conj(a2[k + m + n]) + a3[k + n] * np.conj(a3[k + m + n]))
+    me, ot = (a2, a3) if pol == 0 else (a3, a2)
+    c0, c1 = (a0, a1) if pol == 0 else (a1, a0)
+    e0, e1 = (a4, a5) if pol == 0 else (a5, a4)
+    t1 = (2 * me[k + m] * c0[k + n] * np.conj(c0[k + m + n]) + me[k + m] * c1[k + n] * np.conj(c1[k + m + n])
+          + ot[k + m] * c0[k + n] * np.conj(c1[k + m + n]))
+    nn = -n
+    t3 = (2 * me[k + m] * e0[k + nn] * np.conj(e0[k + m + nn]) + me[k + m] * e1[k + nn] * np.conj(e1[k + m + nn])
+          + ot[k + m] * e0[k + nn] * np.conj(e1[k + m + nn]))
+    return t, t1 + t3
+
+
+@pytest.mark.parametrize("m,n", [(1, -2), (-2, 1), (0, 3), (2, 0)])
+def test_wdmppe_single_weight_brute_force(m, n):
+    """One nonzero C(m, n) or D(m, n) off the centre: the off-diagonal indices
+    (and the -n of T3) against the definitions written out in numpy."""
+    r, samples = 3, 24
+    A = crandn(rng(40 + m * 7 + n), 6 * samples).reshape(6, samples)
+    W = np.zeros((2 * r + 1, 2 * r + 1), dtype=np.complex128)
+    W[r + m, r + n] = 0.7 - 0.4j
+    spm, xpm = O.wdmppe_spm(r, W, A), O.wdmppe_xpm(r, W, A)
+    for pol in range(2):
+        for k in range(r, samples - r):
+            t, tx = _ppe_terms_brute(A, pol, k, m, n)
+            assert spm[pol, k] == pytest.approx(W[r + m, r + n] * t, rel=1e-13)
+            assert xpm[pol, k] == pytest.approx(W[r + m, r + n] * tx, rel=1e-13)
+
+
+def _planted_ppe(c, d, samples, seed):
+    """tx6 random; rx rows 2..3 = lw tx + spm(C) + xpm(D) + intercept with C
+    carrying the fold symmetry C(m,n) = C(n,m) = C(-m,-n) = C(-n,-m)."""
+    r = rng(seed)
+    tx = crandn(r, 6 * samples).reshape(6, samples)
+    Cm = Dm = None
+    rx = crandn(r, 6 * samples).reshape(6, samples)
+    lw, ic = 0.9 - 0.2j, 0.03 + 0.01j
+    core = lw * tx[2:4] + ic
+    if c >= 0:
+        Cm = np.zeros((2 * c + 1, 2 * c + 1), dtype=np.complex128)
+        for m in range(-c, c + 1):
+            for n in range(-c, c + 1):
+                if abs(m) + abs(n) <= c and Cm[c + m, c + n] == 0:
+                    v = complex(r.standard_normal(), r.standard_normal()) * 0.1
+                    for a, b in ((m, n), (n, m), (-m, -n), (-n, -m)):
+                        Cm[c + a, c + b] = v
+        core = core + O.wdmppe_spm(c, Cm, tx)
+    if d >= 0:
+        Dm = np.zeros((2 * d + 1, 2 * d + 1), dtype=np.complex128)
+        for m in range(-d, d + 1):
+            for n in range(-d, d + 1):
+                if abs(m) + abs(n) <= d:
+                    Dm[d + m, d + n] = complex(r.standard_normal(), r.standard_normal()) * 0.1
+        core = core + O.wdmppe_xpm(d, Dm, tx)
+    rad = max(c, d)
+    rx[2:4, rad:samples - rad] = core[:, rad:samples - rad]
+    return tx, rx, Cm, Dm, ic, lw
+
+
+@pytest.mark.parametrize("c,d", [(3, 2), (4, 0), (2, -1), (-1, 2), (0, 0)])
+def test_wdmppe_train_recovers_planted_model(c, d):
+    """WdmPPE::train (wdm_sino.cpp:29-141) on noiseless data generated by
+    spm/xpm of a planted model returns that model: pins the feature
+    definitions, the fold, the column order and the unfold together."""
+    tx, rx, Cm, Dm, ic, lw = _planted_ppe(c, d, 160, 50 + c * 10 + d)
+    C2, D2, ic2, lw2 = O.wdmppe_train(c, d, tx, rx)
+    if c >= 0:
+        assert np.max(np.abs(C2 - Cm)) < 1e-9
+    if d >= 0:
+        assert np.max(np.abs(D2 - Dm)) < 1e-9
+    assert abs(ic2 - ic) < 1e-9 and abs(lw2 - lw) < 1e-9
+
+
+def test_wdmppe_features_shape_and_rank():
+    c, d, samples = 4, 2, 90
+    tx = crandn(rng(60), 6 * samples).reshape(6, samples)
+    F, y = O.wdmppe_features(c, d, tx, tx)
+    nf = c * (c + 2) // 2 + 1 + 2 * d * d + 2 * d + 1 + 1
+    assert F.shape == (2 * (samples - 2 * c), nf)
+    assert np.linalg.matrix_rank(F) == nf
+    np.testing.assert_array_equal(y, np.concatenate([tx[2, c:samples - c], tx[3, c:samples - c]]))
+    np.testing.assert_array_equal(F[:, -1], y)
+
+
 # ---------------------------------------------------------------- learned DBP
 def test_ldbp_identity_and_gradients_fd():
     steps, taps, n = 3, 5, 64
