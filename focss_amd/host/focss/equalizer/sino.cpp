@@ -40,25 +40,12 @@ SINO& SINO::set_linear_weight(const complex_t& w) {
     return *this;
 }
 
-namespace {
-struct DeviceField {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    DeviceField(const Field& f) : bytes((size_t)f.size() * sizeof(complex_t)) {
-        check(focss_device_alloc(runtime(), bytes, &ptr), "focss_device_alloc");
-        check(focss_memcpy_h2d(runtime(), ptr, f.raw(), bytes), "focss_memcpy_h2d");
-    }
-    explicit DeviceField(size_t b) : bytes(b) { check(focss_device_alloc(runtime(), bytes, &ptr), "focss_device_alloc"); }
-    ~DeviceField() {
-        if (ptr) focss_device_free(runtime(), ptr);
-    }
-};
-}  // namespace
 
 // sino.cpp:42-159 with the feature / Gram work on the device
 void SINO::train(const Field& tx, const Field& rx) {
     assert(tx.domain() == rx.domain());
-    DeviceField dtx(tx), drx(rx);
+    DeviceBuffer dtx(tx.raw(), (size_t)tx.size() * sizeof(complex_t));
+    DeviceBuffer drx(rx.raw(), (size_t)rx.size() * sizeof(complex_t));
     int side = 2 * radius_ + 1;
     weights_.assign((size_t)side * side, complex_t(0, 0));
     double lw_in[2] = {linear_weight_.real(), linear_weight_.imag()};
@@ -83,12 +70,12 @@ Field SINO::linear(const Field& f) const { return linear_weight_ * f; }
 // sino.cpp:203-220 on the device
 Field SINO::cubic(const Field& f) const {
     assert(trained_);
-    DeviceField din(f), dout((size_t)f.size() * sizeof(complex_t));
+    DeviceBuffer din(f.raw(), (size_t)f.size() * sizeof(complex_t)), dout((size_t)f.size() * sizeof(complex_t));
     check(focss_sino_cubic(runtime(), FOCSS_F64, radius_, reinterpret_cast<const double*>(weights_.data()), f.modes(),
                            f.samples(), din.ptr, dout.ptr),
           "focss_sino_cubic");
     Field out(f.domain());
-    check(focss_memcpy_d2h(runtime(), out.raw(), dout.ptr, dout.bytes), "focss_memcpy_d2h");
+    dout.download(out.raw());
     return out;
 }
 

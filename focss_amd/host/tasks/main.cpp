@@ -5,6 +5,10 @@
 //   focss_tasks sino    <tx> <rx> <out> <samples> <modes> <radius> <type> <lw_re> <lw_im>
 //                       (out: weights (2R+1)^2, intercept, linear weight, then
 //                        the equalized field (rx - cubic(tx)) / w, complex<double>)
+//   focss_tasks wdmppe  <tx> <rx> <out> <samples> <c_radius> <d_radius>
+//                       (tx, rx: 6 envelope rows; out: C (2c+1)^2 if c >= 0,
+//                        D (2d+1)^2 if d >= 0, intercept, linear weight, then
+//                        the equalized COI (rx_coi - cubic(tx)) / w, 2 rows)
 //   focss_tasks forward <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
 //                       <max_phase_shift> <max_step> <nf_db> <seed> <center_frequency>
 //   focss_tasks save-field <in> <set> <tag> <index> <channels> <modes> <samples>
@@ -24,6 +28,7 @@
 #include <vector>
 
 #include "focss/equalizer/sino.h"
+#include "focss/equalizer/wdm_sino.h"
 #include "focss/field.h"
 #include "focss/functions.h"
 #include "focss/runtime.h"
@@ -113,6 +118,25 @@ int main(int argc, char** argv) {
         out.write(reinterpret_cast<const char*>(&lw), sizeof(lw));
         focss::Field eq = sino.equalize(rx, tx);
         out.write(reinterpret_cast<const char*>(eq.raw()), (std::streamsize)(eq.size() * sizeof(focss::complex_t)));
+        return 0;
+    }
+    if (cmd == "wdmppe" && argc == 8) {
+        int samples = std::atoi(argv[5]);
+        focss::Field tx = read_field(argv[2], samples, 6, 1e-11, 0.0);
+        focss::Field rx = read_field(argv[3], samples, 6, 1e-11, 0.0);
+        focss::WdmPPE ppe(std::atoi(argv[6]), std::atoi(argv[7]));
+        ppe.train(tx, rx);
+        std::ofstream out(argv[4], std::ios::binary);
+        auto put = [&](const focss::complex_t* p, size_t n) {
+            out.write(reinterpret_cast<const char*>(p), (std::streamsize)(n * sizeof(focss::complex_t)));
+        };
+        put(ppe.c_weights().data(), ppe.c_weights().size());
+        put(ppe.d_weights().data(), ppe.d_weights().size());
+        focss::complex_t ic = ppe.get_intercept(), lw = ppe.get_linear_weight();
+        put(&ic, 1);
+        put(&lw, 1);
+        focss::Field eq = ppe.equalize(tx, rx);
+        put(eq.raw(), (size_t)eq.size());
         return 0;
     }
     if (cmd == "save-field" && argc == 9) {

@@ -103,3 +103,34 @@ def test_sino_class(tmp_path):
     assert abs(lwg - lwo) < 1e-9 and abs(icg - ico) < 1e-9
     eq_ref = (rx - O.sino_cubic(Wo.ravel(), radius, tx)) / lwo
     assert np.linalg.norm(eq - eq_ref) / np.linalg.norm(eq_ref) < 1e-9
+
+
+@pytest.mark.parametrize("c,d", [(4, 2), (3, -1)])
+def test_wdmppe_class(tmp_path, c, d):
+    """focss::WdmPPE train + equalize (6 envelope rows) against the oracle's
+    WdmPPE::train and spm / xpm."""
+    n = 700
+    r = np.random.default_rng(43 + c)
+    tx = (r.standard_normal((6, n)) + 1j * r.standard_normal((6, n))) * 0.5
+    rx = (r.standard_normal((6, n)) + 1j * r.standard_normal((6, n))) * 0.5
+    Dm = (r.standard_normal((2 * d + 1, 2 * d + 1)) + 0j) * 0.05 if d >= 0 else None
+    core = 0.93 * tx[2:4] + O.wdmppe_spm(c, np.eye(2 * c + 1) * 0.05, tx)
+    if d >= 0:
+        core = core + O.wdmppe_xpm(d, Dm, tx)
+    rx[2:4] = core + 0.01 * (r.standard_normal((2, n)) + 1j * r.standard_normal((2, n)))
+    ftx, frx, fout = tmp_path / "tx.bin", tmp_path / "rx.bin", tmp_path / "w.bin"
+    tx.tofile(ftx)
+    rx.tofile(frx)
+    run(["wdmppe", ftx, frx, fout, n, c, d], tmp_path)
+    out = np.fromfile(fout, dtype=np.complex128)
+    cs, ds = (2 * c + 1) ** 2, (2 * d + 1) ** 2 if d >= 0 else 0
+    Cg, Dg = out[:cs].reshape(2 * c + 1, -1), out[cs:cs + ds]
+    icg, lwg, eq = out[cs + ds], out[cs + ds + 1], out[cs + ds + 2:].reshape(2, n)
+    Co, Do, ico, lwo = O.wdmppe_train(c, d, tx, rx)
+    assert np.max(np.abs(Cg - Co)) < 1e-9
+    if d >= 0:
+        assert np.max(np.abs(Dg.reshape(Do.shape) - Do)) < 1e-9
+    assert abs(lwg - lwo) < 1e-9 and abs(icg - ico) < 1e-9
+    cubic = O.wdmppe_spm(c, Co, tx) + (O.wdmppe_xpm(d, Do, tx) if d >= 0 else 0)
+    eq_ref = (rx[2:4] - cubic) / lwo
+    assert np.linalg.norm(eq - eq_ref) / np.linalg.norm(eq_ref) < 1e-12
