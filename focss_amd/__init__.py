@@ -80,13 +80,14 @@ _sig = {
     "focss_sino_train": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "focss_wdm_ppe": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, _P, _P]),
     "focss_wdm_ppe_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
-    "focss_ldbp_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
-    "focss_ldbp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
-    "focss_ldbp_infer": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
-    "focss_ldbp_gradients": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
-                                       C.c_size_t, _P, _P]),
-    "focss_ldbp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int,
-                                        C.c_float, _P, _P, _P, _P, C.c_size_t, _P]),
+    "focss_ldbp_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "focss_ldbp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
+                                     _P]),
+    "focss_ldbp_infer": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+    "focss_ldbp_gradients": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P,
+                                       _P, C.c_size_t, _P, _P]),
+    "focss_ldbp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P,
+                                        C.c_int, C.c_float, _P, _P, _P, _P, C.c_size_t, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -311,13 +312,16 @@ class LearnedDBP:
     taps, nonlinear phase from the intensity filtered by `kerr_taps` real
     taps; one tap = the Kerr phase theta |y|^2).  Parameters are float32 on
     the device: FIR taps [steps][taps] complex64, then g [steps][kerr_taps].
-    `kerr` is theta [steps] or g [steps][kerr_taps]."""
+    `kerr` is theta [steps] or g [steps][kerr_taps].  Fields are complex64
+    [batch][modes][samples]; modes=2 is the Manakov model (one FIR per step on
+    both polarisations, phase from the summed intensity)."""
 
-    def __init__(self, ctx, steps, taps, samples, batch, fir, kerr):
+    def __init__(self, ctx, steps, taps, samples, batch, fir, kerr, modes=1):
         kerr = np.asarray(kerr, dtype=np.float64)
         if kerr.ndim == 1:
             kerr = kerr.reshape(steps, 1)
         self.ctx, self.steps, self.taps, self.samples, self.batch = ctx, steps, taps, samples, batch
+        self.modes = modes
         self.kerr_taps = kerr.shape[1]
         self.n_params = 2 * steps * taps + steps * self.kerr_taps
         host = self.pack(fir, kerr)
@@ -329,7 +333,7 @@ class LearnedDBP:
         self.m.upload(zeros)
         self.v.upload(zeros)
         self.t = 0
-        self.ws_bytes = _lib.focss_ldbp_workspace_bytes(steps, samples, batch)
+        self.ws_bytes = _lib.focss_ldbp_workspace_bytes(steps, modes, samples, batch)
         self.ws = ctx.alloc(self.ws_bytes)
         self.grad = ctx.alloc(8 * (self.n_params + 1))
         self.lr_scale = None
@@ -354,7 +358,8 @@ class LearnedDBP:
     def forward(self, inp, out=None, target=None, batch=None):
         loss = C.c_double(0)
         b = batch or self.batch
-        _check(_lib.focss_ldbp_forward(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples, b,
+        _check(_lib.focss_ldbp_forward(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
+                                       self.samples, b,
                                        self.params.ptr, inp.ptr, out.ptr if out else None,
                                        target.ptr if target else None, C.byref(loss) if target else None))
         return loss.value
@@ -363,14 +368,16 @@ class LearnedDBP:
         """Inference with the FIR on FP16 matrix-core products (FP32
         accumulation); stream-ordered."""
         b = batch or self.batch
-        _check(_lib.focss_ldbp_infer(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples, b,
+        _check(_lib.focss_ldbp_infer(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
+                                     self.samples, b,
                                      self.params.ptr, inp.ptr, out.ptr))
 
     def gradients(self, inp, target):
         """(loss, Wirtinger grad of the taps [steps][taps] complex, dL/dg [steps][kerr_taps])."""
         self.grad.upload(np.zeros(self.n_params + 1, dtype=np.float64))
         gp = self.grad.ptr.value
-        _check(_lib.focss_ldbp_gradients(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples,
+        _check(_lib.focss_ldbp_gradients(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
+                                         self.samples,
                                          self.batch, self.params.ptr, inp.ptr, target.ptr, self.ws.ptr,
                                          self.ws_bytes, C.c_void_p(gp + 8), C.c_void_p(gp)))
         g = self.grad.download(np.empty(self.n_params + 1, dtype=np.float64))
@@ -381,7 +388,8 @@ class LearnedDBP:
     def train_step(self, inp, target, lr):
         self.t += 1
         loss = C.c_double(0)
-        _check(_lib.focss_ldbp_train_step(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.samples,
+        _check(_lib.focss_ldbp_train_step(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
+                                          self.samples,
                                           self.batch, self.params.ptr, self.m.ptr, self.v.ptr, self.t, lr,
                                           self.lr_scale.ptr if self.lr_scale else None, inp.ptr, target.ptr,
                                           self.ws.ptr, self.ws_bytes, C.byref(loss)))

@@ -1037,48 +1037,57 @@ namespace {
 
 using LdbpKernel = void (*)(LdbpArgs);
 
-template <int K>
-bool ldbp_kernels_k(int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
+template <int K, int M>
+bool ldbp_kernels_km(int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
     switch (ktaps) {
-        case 1: *fwd = &k_ldbp_fwd<K, 1>; *bwd = &k_ldbp_bwd<K, 1>; return true;
-        case 5: *fwd = &k_ldbp_fwd<K, 5>; *bwd = &k_ldbp_bwd<K, 5>; return true;
-        case 9: *fwd = &k_ldbp_fwd<K, 9>; *bwd = &k_ldbp_bwd<K, 9>; return true;
-        case 15: *fwd = &k_ldbp_fwd<K, 15>; *bwd = &k_ldbp_bwd<K, 15>; return true;
+        case 1: *fwd = &k_ldbp_fwd<K, 1, M>; *bwd = &k_ldbp_bwd<K, 1, M>; return true;
+        case 5: *fwd = &k_ldbp_fwd<K, 5, M>; *bwd = &k_ldbp_bwd<K, 5, M>; return true;
+        case 9: *fwd = &k_ldbp_fwd<K, 9, M>; *bwd = &k_ldbp_bwd<K, 9, M>; return true;
+        case 15: *fwd = &k_ldbp_fwd<K, 15, M>; *bwd = &k_ldbp_bwd<K, 15, M>; return true;
         default: return false;
     }
 }
 
-bool ldbp_kernels(int taps, int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
-    if (taps == 15) return ldbp_kernels_k<15>(ktaps, fwd, bwd);
-    if (taps == 23) return ldbp_kernels_k<23>(ktaps, fwd, bwd);
-    if (taps == 31) return ldbp_kernels_k<31>(ktaps, fwd, bwd);
-    if (taps == 47) return ldbp_kernels_k<47>(ktaps, fwd, bwd);
-    if (taps == 63) return ldbp_kernels_k<63>(ktaps, fwd, bwd);
-    if (taps == 5) return ldbp_kernels_k<5>(ktaps, fwd, bwd);
+template <int K>
+bool ldbp_kernels_k(int ktaps, int modes, LdbpKernel* fwd, LdbpKernel* bwd) {
+    if (modes == 1) return ldbp_kernels_km<K, 1>(ktaps, fwd, bwd);
+    if (modes == 2) return ldbp_kernels_km<K, 2>(ktaps, fwd, bwd);
     return false;
 }
 
-int ldbp_check(int steps, int taps, int ktaps, int samples, int batch) {
+bool ldbp_kernels(int taps, int ktaps, int modes, LdbpKernel* fwd, LdbpKernel* bwd) {
+    if (taps == 15) return ldbp_kernels_k<15>(ktaps, modes, fwd, bwd);
+    if (taps == 23) return ldbp_kernels_k<23>(ktaps, modes, fwd, bwd);
+    if (taps == 31) return ldbp_kernels_k<31>(ktaps, modes, fwd, bwd);
+    if (taps == 47) return ldbp_kernels_k<47>(ktaps, modes, fwd, bwd);
+    if (taps == 63) return ldbp_kernels_k<63>(ktaps, modes, fwd, bwd);
+    if (taps == 5) return ldbp_kernels_k<5>(ktaps, modes, fwd, bwd);
+    return false;
+}
+
+int ldbp_check(int steps, int taps, int ktaps, int modes, int samples, int batch) {
+    if (modes != 1 && modes != 2) return fail(FOCSS_E_ARG, "learned DBP: modes must be 1 or 2");
     LdbpKernel f, b;
-    if (!ldbp_kernels(taps, ktaps, &f, &b))
+    if (!ldbp_kernels(taps, ktaps, modes, &f, &b))
         return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15, 23, 31, 47 or 63 and kerr_taps 1, 5, 9 or 15");
     if (steps < 1 || batch < 1) return fail(FOCSS_E_ARG, "learned DBP: steps and batch must be >= 1");
     int l = ilog2_int(samples);
-    if (l < 0 || samples < LDBP_MIN_N || samples > LDBP_MAX_N)
-        return fail(FOCSS_E_ARG, "learned DBP: samples must be a power of two in [1024, 16384]");
+    if (l < 0 || samples < LDBP_MIN_N || (long long)samples * modes > LDBP_MAX_N)
+        return fail(FOCSS_E_ARG, "learned DBP: samples must be a power of two in [1024, 16384 / modes]");
     return 0;
 }
 
-// forward: N / 16 threads, LDS = N + N/16 complex64; backward: 1024 threads,
-// LDS = N complex64
-int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
-    hipLaunchKernelGGL(k, dim3(a.B), dim3(a.N / LDBP_R), (size_t)(a.N + a.N / 16) * sizeof(cx<float>), c->stream, a);
+// forward: N / 16 threads, LDS = M (N + N/16) complex64; backward: 1024
+// threads, LDS = M N complex64
+int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
+    hipLaunchKernelGGL(k, dim3(a.B), dim3(a.N / LDBP_R), (size_t)modes * (a.N + a.N / 16) * sizeof(cx<float>),
+                       c->stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
-    hipLaunchKernelGGL(k, dim3(a.B), dim3(LDBP_NT), (size_t)a.N * sizeof(cx<float>), c->stream, a);
+int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
+    hipLaunchKernelGGL(k, dim3(a.B), dim3(LDBP_NT), (size_t)modes * a.N * sizeof(cx<float>), c->stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1086,19 +1095,25 @@ int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a) {
 using LdbpInferKernel = void (*)(LdbpArgs, const ldbp_h8*);
 using LdbpAfragKernel = void (*)(const cx<float>*, int, ldbp_h8*);
 
-template <int K>
-bool ldbp_infer_k(int ktaps, LdbpInferKernel* k, LdbpAfragKernel* f, int* nks) {
-    *f = &k_ldbp_afrag<K>;
-    *nks = InferShape<K>::NKS;
+template <int K, int M>
+bool ldbp_infer_km(int ktaps, LdbpInferKernel* k) {
     switch (ktaps) {
-        case 1: *k = &k_ldbp_infer<K, 1>; return true;
-        case 5: *k = &k_ldbp_infer<K, 5>; return true;
-        case 9: *k = &k_ldbp_infer<K, 9>; return true;
-        case 15: *k = &k_ldbp_infer<K, 15>; return true;
+        case 1: *k = &k_ldbp_infer<K, 1, M>; return true;
+        case 5: *k = &k_ldbp_infer<K, 5, M>; return true;
+        case 9: *k = &k_ldbp_infer<K, 9, M>; return true;
+        case 15: *k = &k_ldbp_infer<K, 15, M>; return true;
         default: return false;
     }
 }
 
+template <int K>
+bool ldbp_infer_k(int ktaps, int modes, LdbpInferKernel* k, LdbpAfragKernel* f, int* nks) {
+    *f = &k_ldbp_afrag<K>;
+    *nks = InferShape<K>::NKS;
+    if (modes == 1) return ldbp_infer_km<K, 1>(ktaps, k);
+    if (modes == 2) return ldbp_infer_km<K, 2>(ktaps, k);
+    return false;
+}
 
 }  // namespace
 
@@ -1451,20 +1466,20 @@ int focss_wdm_ppe(focss_ctx* c, int precision, int c_radius, const double* C, in
     return 0;
 }
 
-size_t focss_ldbp_workspace_bytes(int steps, int samples, int batch) {
-    size_t acts = 2ull * (size_t)batch * steps * samples * sizeof(cx<float>);
-    size_t out = (size_t)batch * samples * sizeof(cx<float>);
+size_t focss_ldbp_workspace_bytes(int steps, int modes, int samples, int batch) {
+    size_t acts = 2ull * (size_t)batch * steps * modes * samples * sizeof(cx<float>);
+    size_t out = (size_t)batch * modes * samples * sizeof(cx<float>);
     return acts + 2 * out + 256;  // activations, final output, step scratch
 }
 
-int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch,
+int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                        const float* params, const void* in, void* out, const void* target, double* loss) {
     if (!c || !params || !in || (!out && !target) || (target && !loss))
         return fail(FOCSS_E_ARG, "learned DBP: null argument");
-    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
+    if (int rc = ldbp_check(steps, taps, kerr_taps, modes, samples, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     LdbpKernel fwd, bwd;
-    ldbp_kernels(taps, kerr_taps, &fwd, &bwd);
+    ldbp_kernels(taps, kerr_taps, modes, &fwd, &bwd);
     double* dloss = nullptr;
     if (target) {
         HIP_TRY(hipMalloc(&dloss, sizeof(double)));
@@ -1482,7 +1497,7 @@ int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int sam
     a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    int rc = ldbp_launch_fwd(c, fwd, a);
+    int rc = ldbp_launch_fwd(c, fwd, a, modes);
     if (dloss) {
         hipError_t e = hipSuccess;
         if (rc == 0) e = hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream);
@@ -1493,21 +1508,21 @@ int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int sam
     return rc;
 }
 
-int focss_ldbp_infer(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch, const float* params,
-                     const void* in, void* out) {
+int focss_ldbp_infer(focss_ctx* c, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
+                     const float* params, const void* in, void* out) {
     if (!c || !params || !in || !out) return fail(FOCSS_E_ARG, "learned DBP: null argument");
-    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
+    if (int rc = ldbp_check(steps, taps, kerr_taps, modes, samples, batch)) return rc;
     LdbpInferKernel k = nullptr;
     LdbpAfragKernel f = nullptr;
     int nks = 0;
     bool ok = false;
     switch (taps) {
-        case 5: ok = ldbp_infer_k<5>(kerr_taps, &k, &f, &nks); break;
-        case 15: ok = ldbp_infer_k<15>(kerr_taps, &k, &f, &nks); break;
-        case 23: ok = ldbp_infer_k<23>(kerr_taps, &k, &f, &nks); break;
-        case 31: ok = ldbp_infer_k<31>(kerr_taps, &k, &f, &nks); break;
-        case 47: ok = ldbp_infer_k<47>(kerr_taps, &k, &f, &nks); break;
-        case 63: ok = ldbp_infer_k<63>(kerr_taps, &k, &f, &nks); break;
+        case 5: ok = ldbp_infer_k<5>(kerr_taps, modes, &k, &f, &nks); break;
+        case 15: ok = ldbp_infer_k<15>(kerr_taps, modes, &k, &f, &nks); break;
+        case 23: ok = ldbp_infer_k<23>(kerr_taps, modes, &k, &f, &nks); break;
+        case 31: ok = ldbp_infer_k<31>(kerr_taps, modes, &k, &f, &nks); break;
+        case 47: ok = ldbp_infer_k<47>(kerr_taps, modes, &k, &f, &nks); break;
+        case 63: ok = ldbp_infer_k<63>(kerr_taps, modes, &k, &f, &nks); break;
         default: break;
     }
     if (!ok) return fail(FOCSS_E_UNSUPPORTED, "learned DBP: no inference kernel for these taps");
@@ -1536,27 +1551,27 @@ int focss_ldbp_infer(focss_ctx* c, int steps, int taps, int kerr_taps, int sampl
     a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    const size_t lds = (size_t)samples * (sizeof(cx_h) + sizeof(float));
+    const size_t lds = (size_t)modes * samples * (sizeof(cx_h) + sizeof(float));
     hipLaunchKernelGGL(k, dim3(batch), dim3(LI_NT), lds, c->stream, a, (const ldbp_h8*)afrag);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch,
+int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                          const float* params, const void* in, const void* target, void* workspace,
                          size_t workspace_bytes, double* grad, double* loss) {
     if (!c || !params || !in || !target || !workspace || !grad) return fail(FOCSS_E_ARG, "learned DBP: null argument");
-    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
-    if (workspace_bytes < focss_ldbp_workspace_bytes(steps, samples, batch))
+    if (int rc = ldbp_check(steps, taps, kerr_taps, modes, samples, batch)) return rc;
+    if (workspace_bytes < focss_ldbp_workspace_bytes(steps, modes, samples, batch))
         return fail(FOCSS_E_ARG, "learned DBP: workspace too small");
     HIP_TRY(hipSetDevice(c->device));
     LdbpKernel fwd, bwd;
-    ldbp_kernels(taps, kerr_taps, &fwd, &bwd);
-    size_t act = (size_t)batch * steps * samples;
+    ldbp_kernels(taps, kerr_taps, modes, &fwd, &bwd);
+    size_t act = (size_t)batch * steps * modes * samples;
     cx<float>* xs = (cx<float>*)workspace;
     cx<float>* ys = xs + act;
     cx<float>* out = ys + act;
-    cx<float>* zs = out + (size_t)batch * samples;
+    cx<float>* zs = out + (size_t)batch * modes * samples;
     LdbpArgs a = {};
     a.zs = zs;
     a.in = (const cx<float>*)in;
@@ -1573,23 +1588,24 @@ int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int s
     a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    if (int rc = ldbp_launch_fwd(c, fwd, a)) return rc;
-    if (int rc = ldbp_launch_bwd(c, bwd, a)) return rc;
+    if (int rc = ldbp_launch_fwd(c, fwd, a, modes)) return rc;
+    if (int rc = ldbp_launch_bwd(c, bwd, a, modes)) return rc;
     return 0;
 }
 
-int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int samples, int batch, float* params,
-                          float* adam_m, float* adam_v, int t, float lr, const float* lr_scale, const void* in,
-                          const void* target, void* workspace, size_t workspace_bytes, double* loss_host) {
+int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
+                          float* params, float* adam_m, float* adam_v, int t, float lr, const float* lr_scale,
+                          const void* in, const void* target, void* workspace, size_t workspace_bytes,
+                          double* loss_host) {
     if (!c || !params || !adam_m || !adam_v || t < 1) return fail(FOCSS_E_ARG, "learned DBP: bad argument");
-    if (int rc = ldbp_check(steps, taps, kerr_taps, samples, batch)) return rc;
+    if (int rc = ldbp_check(steps, taps, kerr_taps, modes, samples, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     const int np = 2 * steps * taps + steps * kerr_taps;
     // gradient and loss accumulators live at the head of a small buffer
     double* acc = nullptr;
     HIP_TRY(hipMalloc(&acc, (size_t)(np + 1) * sizeof(double)));
     HIP_TRY(hipMemsetAsync(acc, 0, (size_t)(np + 1) * sizeof(double), c->stream));
-    int rc = focss_ldbp_gradients(c, steps, taps, kerr_taps, samples, batch, params, in, target, workspace,
+    int rc = focss_ldbp_gradients(c, steps, taps, kerr_taps, modes, samples, batch, params, in, target, workspace,
                                   workspace_bytes, acc + 1, acc);
     if (rc == 0) {
         hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), dim3(256), 0, c->stream, params, (const double*)(acc + 1),

@@ -7,13 +7,17 @@
 // KP = 1 is the plain Kerr step exp(i theta_j |y|^2); a wider KP low-pass
 // filters the intensity that drives the phase, which lets a long step
 // account for the dispersive walk-off inside it.
-// Loss = sum over the batch of (1/(N B)) sum_t |z_J(t) - target(t)|^2.
+// Loss = sum over the batch of (1/(M N B)) sum_{mode, t} |z_J(t) - target(t)|^2.
 // Gradients are Wirtinger dL/d(conj h) and real dL/dg:
 //     q = -Im(conj(G_z) z),   dL/dphi = 2 q,
 //     dL/dg_m = sum_t 2 q(t) p(t - m + P),   dL/dp(s) = sum_m g_m 2 q(s + m - P),
 //     G_y = G_z e^{-i phi} + (dL/dp) y,
 //     G_h_k = sum_t G_y(t) conj(x(t - k + c)),   G_x(s) = sum_k conj(h_k) G_y(s + k - c).
-// One workgroup owns one waveform (N <= 16384 complex64 in LDS) and runs
+// With M = 2 modes (Manakov) the FIR acts on each polarisation and p(t) is
+// the intensity summed over both, so the phase is common to them; the loss
+// averages over every mode and sample, and
+//     q = -sum_modes Im(conj(G_z) z),   G_h_k sums over the modes.
+// One workgroup owns one waveform (M N <= 16384 complex64 in LDS) and runs
 // every step on its CU.
 #pragma once
 #include "cx.h"
@@ -28,16 +32,16 @@ constexpr int LDBP_R = 16;          // forward pass: consecutive outputs per thr
 constexpr int LDBP_KC = 8;          // forward pass: FIR taps per register window
 
 struct LdbpArgs {
-    const cx<float>* in;      // [B][N]
-    cx<float>* out;           // [B][N] (may be null)
-    const cx<float>* target;  // [B][N] (may be null)
+    const cx<float>* in;      // [B][M][N]
+    cx<float>* out;           // [B][M][N] (may be null)
+    const cx<float>* target;  // [B][M][N] (may be null)
     const cx<float>* fir;     // [J][K]
     const float* kerr;        // [J][KP]
-    cx<float>* xs;            // [B][J][N] FIR inputs (training) or null
-    cx<float>* ys;            // [B][J][N] FIR outputs (training) or null
+    cx<float>* xs;            // [B][J][M][N] FIR inputs (training) or null
+    cx<float>* ys;            // [B][J][M][N] FIR outputs (training) or null
     double* loss;             // [1] accumulated, may be null
     double* grad;             // [J*K*2 + J*KP] accumulated: Re/Im of G_h, then dL/dg
-    cx<float>* zs;            // [B][N] backward scratch (per-sample q and phi)
+    cx<float>* zs;            // [B][M][N] backward scratch (per-sample q and phi, input gradient)
     int J, K, KP, N, B;
 };
 
@@ -63,61 +67,78 @@ __device__ __forceinline__ double ldbp_block_sum(double v, double* red, int nt) 
     return s;  // valid in thread 0
 }
 
-// Forward pass, N / 16 threads: thread i owns outputs 16 i .. 16 i + 15.
-template <int K, int KP>
+// Forward pass, N / 16 threads: thread i owns outputs 16 i .. 16 i + 15 of
+// every mode.  LDS holds the M modes, mode stride N + N / 16 (p16 layout).
+template <int K, int KP, int M>
 __global__ __launch_bounds__(LDBP_MAX_N / LDBP_R) void k_ldbp_fwd(LdbpArgs a) {
     constexpr int R = LDBP_R, KC = LDBP_KC, P = KP / 2, c = K / 2;
     constexpr int NW = LDBP_MAX_N / LDBP_R / 64;
     extern __shared__ unsigned char smem_raw[];
-    cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // p16-indexed, N + N/16 elements
+    cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [M][p16(N)]
     __shared__ float edge[2][NW][P > 0 ? P : 1];              // first / last P intensities of each wave
     __shared__ double red[NW];
-    const int N = a.N, mask = N - 1;
+    const int N = a.N, mask = N - 1, S = N + N / 16;
     const int nt = blockDim.x, nw = nt >> 6;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int b = blockIdx.x;
     const int t0 = threadIdx.x * R;
-    const cx<float>* in = a.in + (long long)b * N;
-    for (int t = threadIdx.x; t < N; t += nt) buf[p16(t)] = in[t];
+    const cx<float>* in = a.in + (long long)b * M * N;
+#pragma unroll
+    for (int md = 0; md < M; ++md)
+        for (int t = threadIdx.x; t < N; t += nt) buf[md * S + p16(t)] = in[(long long)md * N + t];
     __syncthreads();
     for (int j = 0; j < a.J; ++j) {
         const float* hj = reinterpret_cast<const float*>(a.fir + (long long)j * K);
         if (a.xs) {
-            cx<float>* xs = a.xs + ((long long)b * a.J + j) * N + t0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) xs[r] = buf[p16(t0 + r)];
+            for (int md = 0; md < M; ++md) {
+                cx<float>* xs = a.xs + (((long long)b * a.J + j) * M + md) * N + t0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) xs[r] = buf[md * S + p16(t0 + r)];
+            }
         }
-        // FIR, KC taps per register window of R + KC - 1 inputs
-        cx<float> y[R];
+        // FIR of each mode, KC taps per register window of R + KC - 1 inputs
+        cx<float> y[M][R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) y[r] = mk<float>(0, 0);
+        for (int md = 0; md < M; ++md) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) y[md][r] = mk<float>(0, 0);
+            const cx<float>* bm = buf + md * S;
 #pragma unroll 1
-        for (int k0 = 0; k0 < K; k0 += KC) {
-            cx<float> w[R + KC - 1];
-            const int base = t0 + c - k0 - (KC - 1);
+            for (int k0 = 0; k0 < K; k0 += KC) {
+                cx<float> w[R + KC - 1];
+                const int base = t0 + c - k0 - (KC - 1);
 #pragma unroll
-            for (int u = 0; u < R + KC - 1; ++u) w[u] = buf[p16((base + u) & mask)];
+                for (int u = 0; u < R + KC - 1; ++u) w[u] = bm[p16((base + u) & mask)];
 #pragma unroll
-            for (int kk = 0; kk < KC; ++kk) {
-                const bool live = k0 + kk < K;
-                const float hr = live ? sload(hj + 2 * (k0 + kk)) : 0.0f;
-                const float hi = live ? sload(hj + 2 * (k0 + kk) + 1) : 0.0f;
+                for (int kk = 0; kk < KC; ++kk) {
+                    const bool live = k0 + kk < K;
+                    const float hr = live ? sload(hj + 2 * (k0 + kk)) : 0.0f;
+                    const float hi = live ? sload(hj + 2 * (k0 + kk) + 1) : 0.0f;
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const cx<float> x = w[r - kk + KC - 1];
-                    y[r].x = fmaf(hr, x.x, fmaf(-hi, x.y, y[r].x));
-                    y[r].y = fmaf(hr, x.y, fmaf(hi, x.x, y[r].y));
+                    for (int r = 0; r < R; ++r) {
+                        const cx<float> x = w[r - kk + KC - 1];
+                        y[md][r].x = fmaf(hr, x.x, fmaf(-hi, x.y, y[md][r].x));
+                        y[md][r].y = fmaf(hr, x.y, fmaf(hi, x.x, y[md][r].y));
+                    }
                 }
             }
         }
         if (a.ys) {
-            cx<float>* ys = a.ys + ((long long)b * a.J + j) * N + t0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) ys[r] = y[r];
+            for (int md = 0; md < M; ++md) {
+                cx<float>* ys = a.ys + (((long long)b * a.J + j) * M + md) * N + t0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) ys[r] = y[md][r];
+            }
         }
         float p[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) p[r] = norm2(y[r]);
+        for (int r = 0; r < R; ++r) {
+            p[r] = norm2(y[0][r]);
+#pragma unroll
+            for (int md = 1; md < M; ++md) p[r] += norm2(y[md][r]);
+        }
         if constexpr (P > 0) {
             if (lane == 0) {
 #pragma unroll
@@ -155,26 +176,32 @@ __global__ __launch_bounds__(LDBP_MAX_N / LDBP_R) void k_ldbp_fwd(LdbpArgs a) {
             for (int m = 0; m < KP; ++m) phi = fmaf(g[m], pe[r - m + 2 * P], phi);
             float s, co;
             sincos_fast(phi, &s, &co);
-            buf[p16(t0 + r)] = y[r] * mk<float>(co, s);
+#pragma unroll
+            for (int md = 0; md < M; ++md) buf[md * S + p16(t0 + r)] = y[md][r] * mk<float>(co, s);
         }
         __syncthreads();
     }
     double part = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const long long t = (long long)b * N + t0 + r;
-        const cx<float> z = buf[p16(t0 + r)];
-        if (a.out) a.out[t] = z;
-        if (a.target) part += (double)norm2(z - a.target[t]);
+    for (int md = 0; md < M; ++md) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const long long t = ((long long)b * M + md) * N + t0 + r;
+            const cx<float> z = buf[md * S + p16(t0 + r)];
+            if (a.out) a.out[t] = z;
+            if (a.target) part += (double)norm2(z - a.target[t]);
+        }
     }
     if (a.loss) {
         double s = ldbp_block_sum(part, red, nt);
-        if (threadIdx.x == 0) atomicAdd(a.loss, s / ((double)N * a.B));
+        if (threadIdx.x == 0) atomicAdd(a.loss, s / ((double)N * M * a.B));
     }
 }
 
-// needs the forward pass run with xs / ys stored; out holds z_J
-template <int K, int KP>
+// needs the forward pass run with xs / ys stored; out holds z_J.  LDS holds
+// the gradient of every mode [M][N]; zs ([B][M][N]) stages the per-sample q
+// and phi (first N) and the input gradient.
+template <int K, int KP, int M>
 __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     extern __shared__ unsigned char smem_raw[];
     cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);
@@ -184,14 +211,14 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     constexpr int P = KP / 2;
     const int N = a.N, mask = N - 1, c = K / 2;
     const int b = blockIdx.x;
-    const float scale = 1.0f / ((float)N * (float)a.B);
-    cx<float>* zs = a.zs + (long long)b * N;
-    for (int t = threadIdx.x; t < N; t += LDBP_NT)
-        g[t] = (a.out[(long long)b * N + t] - a.target[(long long)b * N + t]) * scale;
+    const float scale = 1.0f / ((float)N * (float)M * (float)a.B);
+    const long long wb = (long long)b * M * N;  // this waveform's element offset
+    cx<float>* zs = a.zs + wb;
+    for (int t = threadIdx.x; t < M * N; t += LDBP_NT) g[t] = (a.out[wb + t] - a.target[wb + t]) * scale;
     __syncthreads();
     for (int j = a.J - 1; j >= 0; --j) {
-        const cx<float>* x = a.xs + ((long long)b * a.J + j) * N;
-        const cx<float>* y = a.ys + ((long long)b * a.J + j) * N;
+        const cx<float>* x = a.xs + ((long long)b * a.J + j) * M * N;  // [M][N]
+        const cx<float>* y = a.ys + ((long long)b * a.J + j) * M * N;
         const float* gk = a.kerr + j * KP;
         // nonlinear phase backward, pass 1: q and phi per sample (staged in
         // zs), intensity-tap gradients
@@ -199,17 +226,25 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
 #pragma unroll
         for (int m = 0; m < KP; ++m) dg[m] = 0;
         for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-            float pn[KP];  // p(t - m + P)
+            float pn[KP];  // p(t - m + P), summed over the modes
 #pragma unroll
-            for (int m = 0; m < KP; ++m) pn[m] = norm2(y[(t - m + P) & mask]);
+            for (int m = 0; m < KP; ++m) {
+                pn[m] = norm2(y[(t - m + P) & mask]);
+#pragma unroll
+                for (int md = 1; md < M; ++md) pn[m] += norm2(y[md * N + ((t - m + P) & mask)]);
+            }
             float phi = 0;
 #pragma unroll
             for (int m = 0; m < KP; ++m) phi = fmaf(gk[m], pn[m], phi);
             float s, co;
             sincos_fast(phi, &s, &co);
-            const cx<float> zv = y[t] * mk<float>(co, s);
-            const cx<float> gz = g[t];
-            const float q = -(gz.x * zv.y - gz.y * zv.x);  // -Im(conj(gz) z)
+            float q = 0;  // -sum_modes Im(conj(gz) z)
+#pragma unroll
+            for (int md = 0; md < M; ++md) {
+                const cx<float> zv = y[md * N + t] * mk<float>(co, s);
+                const cx<float> gz = g[md * N + t];
+                q -= gz.x * zv.y - gz.y * zv.x;
+            }
 #pragma unroll
             for (int m = 0; m < KP; ++m) dg[m] += 2.0 * (double)q * (double)pn[m];
             zs[t] = mk<float>(q, phi);
@@ -223,7 +258,8 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
             const float phi = zs[t].y;
             float s, co;
             sincos_fast(phi, &s, &co);
-            g[t] = g[t] * mk<float>(co, -s) + y[t] * dp;
+#pragma unroll
+            for (int md = 0; md < M; ++md) g[md * N + t] = g[md * N + t] * mk<float>(co, -s) + y[md * N + t] * dp;
         }
         __syncthreads();
         double* grk = a.grad + (long long)a.J * K * 2 + (long long)j * KP;
@@ -232,8 +268,9 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
             double st = ldbp_block_sum(dg[m], red, LDBP_NT);
             if (threadIdx.x == 0) atomicAdd(&grk[m], st);
         }
-        // FIR tap gradients G_h_k = sum_t G_y(t) conj(x(t - k + c)), KC taps per
-        // sweep; per-thread partial sums in float, block sums in double
+        // FIR tap gradients G_h_k = sum_modes sum_t G_y(t) conj(x(t - k + c)),
+        // KC taps per sweep; per-thread partial sums in float, block sums in
+        // double
         double* gr = a.grad + (long long)j * K * 2;
         for (int k0 = 0; k0 < K; k0 += KC) {
             float dre[KC], dim[KC];
@@ -242,14 +279,17 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                 dre[q] = 0;
                 dim[q] = 0;
             }
-            for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-                cx<float> gy = g[t];
 #pragma unroll
-                for (int q = 0; q < KC; ++q) {
-                    if (k0 + q < K) {
-                        cx<float> v = gy * conj(x[(t - (k0 + q) + c) & mask]);
-                        dre[q] += v.x;
-                        dim[q] += v.y;
+            for (int md = 0; md < M; ++md) {
+                for (int t = threadIdx.x; t < N; t += LDBP_NT) {
+                    cx<float> gy = g[md * N + t];
+#pragma unroll
+                    for (int q = 0; q < KC; ++q) {
+                        if (k0 + q < K) {
+                            cx<float> v = gy * conj(x[md * N + ((t - (k0 + q) + c) & mask)]);
+                            dre[q] += v.x;
+                            dim[q] += v.y;
+                        }
                     }
                 }
             }
@@ -261,17 +301,22 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                 if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q) + 1], si);
             }
         }
-        // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c), staged in zs
+        // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c) of every
+        // mode, staged in zs
         if (threadIdx.x < K) h[threadIdx.x] = conj(a.fir[j * K + threadIdx.x]);
         __syncthreads();
-        for (int s = threadIdx.x; s < N; s += LDBP_NT) {
-            cx<float> acc = mk<float>(0, 0);
+#pragma unroll
+        for (int md = 0; md < M; ++md) {
+            const cx<float>* gm = g + md * N;
+            for (int s = threadIdx.x; s < N; s += LDBP_NT) {
+                cx<float> acc = mk<float>(0, 0);
 #pragma unroll 16
-            for (int k = 0; k < K; ++k) acc = acc + h[k] * g[(s + k - c) & mask];
-            zs[s] = acc;
+                for (int k = 0; k < K; ++k) acc = acc + h[k] * gm[(s + k - c) & mask];
+                zs[md * N + s] = acc;
+            }
         }
         __syncthreads();
-        for (int s = threadIdx.x; s < N; s += LDBP_NT) g[s] = zs[s];
+        for (int s = threadIdx.x; s < M * N; s += LDBP_NT) g[s] = zs[s];
         __syncthreads();
     }
 }
@@ -337,24 +382,29 @@ __global__ void k_ldbp_afrag(const cx<float>* __restrict__ fir, int J, ldbp_h8* 
     afrag[i] = A;
 }
 
-template <int K, int KP>
+// M modes: LDS holds the field [M][16][nb] (FP16) and each mode's
+// intensities [M][16][nb]; the M N / 512 tiles (at most 32: M N <= 16384)
+// are dealt over the 16 waves, and the nonlinear phase sums the modes.
+template <int K, int KP, int M>
 __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8* __restrict__ afrag) {
     using IS = InferShape<K>;
     constexpr int c = IS::c, DELTA = IS::DELTA, NKS = IS::NKS, P = KP / 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int N = a.N, nb = N >> 4, lnb = __builtin_ctz(N >> 4);
-    cx_h* xs = reinterpret_cast<cx_h*>(smem_raw);                                 // [16][nb]
-    float* pb = reinterpret_cast<float*>(smem_raw + (size_t)N * sizeof(cx_h));   // [16][nb]
+    cx_h* xs = reinterpret_cast<cx_h*>(smem_raw);                                     // [M][16][nb]
+    float* pb = reinterpret_cast<float*>(smem_raw + (size_t)M * N * sizeof(cx_h));   // [M][16][nb]
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int hl = lane >> 5, col = lane & 31;
-    const int n_tiles = N / 512;
+    const int mode_tiles = N / 512, n_tiles = M * mode_tiles;
     const int b = blockIdx.x;
-    const cx<float>* in = a.in + (long long)b * N;
-    for (int p = threadIdx.x; p < N; p += LI_NT) {
-        const int t = ((p & (nb - 1)) << 4) + (p >> lnb);  // sample stored at position p
-        const cx<float> v = in[t];
-        xs[p] = cx_h{(_Float16)v.x, (_Float16)v.y};
-    }
+    const cx<float>* in = a.in + (long long)b * M * N;
+#pragma unroll
+    for (int md = 0; md < M; ++md)
+        for (int p = threadIdx.x; p < N; p += LI_NT) {
+            const int t = ((p & (nb - 1)) << 4) + (p >> lnb);  // sample stored at position p
+            const cx<float> v = in[(long long)md * N + t];
+            xs[md * N + p] = cx_h{(_Float16)v.x, (_Float16)v.y};
+        }
     __syncthreads();
     for (int j = 0; j < a.J; ++j) {
         // FIR on the matrix cores
@@ -367,7 +417,8 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             acc[it] = ldbp_f16{};
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
-                const int blk = tile * 32 + col;
+                const cx_h* xm = xs + (tile / mode_tiles) * N;
+                const int blk = (tile % mode_tiles) * 32 + col;
                 const int off = c - (K - 1) - DELTA + 4 * hl;  // window start relative to 16 blk
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) {
@@ -375,7 +426,7 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int o = off + 8 * s + v;  // sample 16 blk + o = row o & 15 of block blk + (o >> 4)
-                        const cx_h xv = xs[((o & 15) << lnb) + ((blk + (o >> 4)) & (nb - 1))];
+                        const cx_h xv = xm[((o & 15) << lnb) + ((blk + (o >> 4)) & (nb - 1))];
                         bf[2 * v] = xv.x;
                         bf[2 * v + 1] = xv.y;
                     }
@@ -389,15 +440,16 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
         for (int it = 0; it < 2; ++it) {
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
-                const int blk = tile * 32 + col;
+                const int mo = (tile / mode_tiles) * N;
+                const int blk = (tile % mode_tiles) * 32 + col;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int jj = 0; jj < 2; ++jj) {
                         const int row = 4 * i + 2 * hl + jj;
                         const float re = acc[it][4 * i + 2 * jj], im = acc[it][4 * i + 2 * jj + 1];
-                        xs[(row << lnb) + blk] = cx_h{(_Float16)re, (_Float16)im};
-                        pb[(row << lnb) + blk] = re * re + im * im;
+                        xs[mo + (row << lnb) + blk] = cx_h{(_Float16)re, (_Float16)im};
+                        pb[mo + (row << lnb) + blk] = re * re + im * im;
                     }
             }
         }
@@ -408,15 +460,24 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
 #pragma unroll
         for (int mm = 0; mm < KP; ++mm) g[mm] = sload(a.kerr + j * KP + mm);
         for (int blk = threadIdx.x; blk < nb; blk += LI_NT) {
-            float pe[16 + 2 * P];  // p(16 blk - P + u)
+            float pe[16 + 2 * P];  // p(16 blk - P + u), summed over the modes
 #pragma unroll
-            for (int r = 0; r < 16; ++r) pe[P + r] = pb[(r << lnb) + blk];
+            for (int r = 0; r < 16; ++r) {
+                pe[P + r] = pb[(r << lnb) + blk];
+#pragma unroll
+                for (int md = 1; md < M; ++md) pe[P + r] += pb[md * N + (r << lnb) + blk];
+            }
             if constexpr (P > 0) {
                 const int bp = (blk + nb - 1) & (nb - 1), bn = (blk + 1) & (nb - 1);
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
                     pe[i] = pb[((16 - P + i) << lnb) + bp];
                     pe[P + 16 + i] = pb[(i << lnb) + bn];
+#pragma unroll
+                    for (int md = 1; md < M; ++md) {
+                        pe[i] += pb[md * N + ((16 - P + i) << lnb) + bp];
+                        pe[P + 16 + i] += pb[md * N + (i << lnb) + bn];
+                    }
                 }
             }
 #pragma unroll
@@ -426,12 +487,15 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
                 for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
                 float sn, cs;
                 sincos_fast(phi, &sn, &cs);
-                const cx_h yh = xs[(r << lnb) + blk];
-                const cx<float> z = mk<float>((float)yh.x, (float)yh.y) * mk<float>(cs, sn);
-                if (last)
-                    a.out[(long long)b * N + 16 * blk + r] = z;
-                else
-                    xs[(r << lnb) + blk] = cx_h{(_Float16)z.x, (_Float16)z.y};
+#pragma unroll
+                for (int md = 0; md < M; ++md) {
+                    const cx_h yh = xs[md * N + (r << lnb) + blk];
+                    const cx<float> z = mk<float>((float)yh.x, (float)yh.y) * mk<float>(cs, sn);
+                    if (last)
+                        a.out[((long long)b * M + md) * N + 16 * blk + r] = z;
+                    else
+                        xs[md * N + (r << lnb) + blk] = cx_h{(_Float16)z.x, (_Float16)z.y};
+                }
             }
         }
         // the new field is visible before the next step's reads

@@ -236,35 +236,38 @@ int focss_field_to_records(focss_ctx* ctx, int precision, int envelopes, int sam
  * filter on the intensity; kerr_taps = 1 is the plain Kerr step
  * exp(i theta_j |y|^2)).  params (device, float): the FIR taps [steps][taps]
  * as interleaved complex64, followed by g [steps][kerr_taps].  Fields are
- * complex64 device buffers [batch][samples], samples a power of two in
- * [1024, 16384], taps 5, 15, 23, 31, 47 or 63, kerr_taps 1, 5, 9 or
- * 15.  Loss =
- * sum_b (1/(samples batch)) sum_t |z - target|^2.  Judged against plain DBP
- * (focss_link_run); callers are learned-equaliser training and inference. */
-size_t focss_ldbp_workspace_bytes(int steps, int samples, int batch);
+ * complex64 device buffers [batch][modes][samples]; modes 2 is the Manakov
+ * (dual-polarisation) model, one FIR per step applied to each polarisation
+ * and the phase driven by the intensity summed over both.  samples is a
+ * power of two in [1024, 16384 / modes], taps 5, 15, 23, 31, 47 or 63,
+ * kerr_taps 1, 5, 9 or 15.  Loss =
+ * (1/(modes samples batch)) sum_{b, mode, t} |z - target|^2.  Judged against
+ * plain DBP (focss_link_run); callers are learned-equaliser training and
+ * inference. */
+size_t focss_ldbp_workspace_bytes(int steps, int modes, int samples, int batch);
 /* Inference (out) and/or loss against target (*loss, host double; the call
  * then synchronises).  Without target the call is stream-ordered and
  * allocates nothing. */
-int focss_ldbp_forward(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+int focss_ldbp_forward(focss_ctx* ctx, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                        const float* params, const void* in, void* out, const void* target, double* loss);
 /* Inference on the matrix cores: the same model with the FIR of every step
  * as FP16 MFMA products accumulated in FP32 and the field held in FP16
  * between steps (about 3e-3 relative deviation from the FP32 path after 80
  * steps).  Stream-ordered; the context keeps a buffer for the per-step
  * matrix operands (grown, with a synchronisation, when a call needs more). */
-int focss_ldbp_infer(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+int focss_ldbp_infer(focss_ctx* ctx, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                      const float* params, const void* in, void* out);
 /* Accumulates into grad (device, double [2 steps taps + steps kerr_taps],
  * zeroed by the caller) the Wirtinger gradient dL/d(conj h) (re, im) and
  * dL/dg, and the loss into *loss (device double).  Stream-ordered, no host
  * sync. */
-int focss_ldbp_gradients(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+int focss_ldbp_gradients(focss_ctx* ctx, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                          const float* params, const void* in, const void* target, void* workspace,
                          size_t workspace_bytes, double* grad, double* loss);
 /* One training step: gradients then Adam (t = step number from 1, optional
  * per-parameter learning-rate scale).  *loss_host receives the batch loss
  * before the update. */
-int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int kerr_taps, int samples, int batch,
+int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                           float* params, float* adam_m, float* adam_v, int t, float lr, const float* lr_scale,
                           const void* in, const void* target, void* workspace, size_t workspace_bytes,
                           double* loss_host);

@@ -1090,7 +1090,9 @@ int orc_wdmppe_train(int c_radius, int d_radius, int samples, const orc_c128* tx
 /*           p(t)   = |y(t)|^2                                          */
 /*           phi(t) = sum_m g_{j,m} p((t - m + P) mod N),  P = KP/2     */
 /*           z(t)   = y(t) exp(i phi(t))                                */
-/*   loss   = (1/N) sum_t |z_J(t) - target(t)|^2                        */
+/*   loss   = (1/(M N)) sum_{mode, t} |z_J(t) - target(t)|^2            */
+/* With M = 2 modes (Manakov) the FIR acts on each mode and p(t) is the */
+/* intensity summed over the modes, so the phase is common to both.     */
 /* KP = 1 is the plain Kerr step exp(i theta_j |y|^2).  Gradients are   */
 /* Wirtinger dL/d(conj h) for the FIR taps and dL/dg for the real       */
 /* intensity taps:  q = -Im(conj(G_z) z),  dL/dphi = 2 q,                */
@@ -1109,90 +1111,111 @@ static void ldbp_fir(int taps, const orc_c128* h, int n, const orc_c128* x, orc_
     }
 }
 
-/* phi(t) of one step from y */
-static void ldbp_phase(int ktaps, const double* g, int n, const orc_c128* y, double* phi) {
+/* phi(t) of one step from y [modes][n]: the intensity is summed over modes */
+static void ldbp_phase(int ktaps, const double* g, int modes, int n, const orc_c128* y, double* phi) {
     int P = ktaps / 2;
     for (int t = 0; t < n; ++t) {
         double f = 0;
-        for (int m = 0; m < ktaps; ++m) f += g[m] * c_norm(y[wrap(t - m + P, n)]);
+        for (int m = 0; m < ktaps; ++m) {
+            double p = 0;
+            for (int md = 0; md < modes; ++md) p += c_norm(y[(long)md * n + wrap(t - m + P, n)]);
+            f += g[m] * p;
+        }
         phi[t] = f;
     }
 }
 
-double orc_ldbp_forward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr, int n,
+double orc_ldbp_forward(int steps, int taps, int ktaps, int modes, const orc_c128* fir, const double* kerr, int n,
                         const orc_c128* in, orc_c128* out, const orc_c128* target) {
-    orc_c128* x = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
-    orc_c128* y = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
+    long mn = (long)modes * n;
+    orc_c128* x = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)mn);
+    orc_c128* y = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)mn);
     double* phi = (double*)malloc(sizeof(double) * (size_t)n);
-    memcpy(x, in, sizeof(orc_c128) * (size_t)n);
+    memcpy(x, in, sizeof(orc_c128) * (size_t)mn);
     for (int j = 0; j < steps; ++j) {
-        ldbp_fir(taps, fir + (long)j * taps, n, x, y);
-        ldbp_phase(ktaps, kerr + (long)j * ktaps, n, y, phi);
-        for (int t = 0; t < n; ++t) x[t] = c_mul(y[t], i_exp(phi[t]));
+        for (int md = 0; md < modes; ++md) ldbp_fir(taps, fir + (long)j * taps, n, x + (long)md * n, y + (long)md * n);
+        ldbp_phase(ktaps, kerr + (long)j * ktaps, modes, n, y, phi);
+        for (long i = 0; i < mn; ++i) x[i] = c_mul(y[i], i_exp(phi[i % n]));
     }
     double loss = 0;
     if (target)
-        for (int t = 0; t < n; ++t) loss += c_norm(c_sub(x[t], target[t]));
-    if (out) memcpy(out, x, sizeof(orc_c128) * (size_t)n);
+        for (long i = 0; i < mn; ++i) loss += c_norm(c_sub(x[i], target[i]));
+    if (out) memcpy(out, x, sizeof(orc_c128) * (size_t)mn);
     free(phi);
     free(y);
     free(x);
-    return loss / n;
+    return loss / mn;
 }
 
-double orc_ldbp_backward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr, int n,
+double orc_ldbp_backward(int steps, int taps, int ktaps, int modes, const orc_c128* fir, const double* kerr, int n,
                          const orc_c128* in, const orc_c128* target, orc_c128* grad_fir, double* grad_kerr) {
     /* forward with the per-step FIR inputs and outputs kept */
-    orc_c128* xs = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n * (steps + 1));
-    orc_c128* ys = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n * steps);
+    long mn = (long)modes * n;
+    orc_c128* xs = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)mn * (steps + 1));
+    orc_c128* ys = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)mn * steps);
     double* phi = (double*)malloc(sizeof(double) * (size_t)n);
     double* q = (double*)malloc(sizeof(double) * (size_t)n);
-    memcpy(xs, in, sizeof(orc_c128) * (size_t)n);
+    memcpy(xs, in, sizeof(orc_c128) * (size_t)mn);
     for (int j = 0; j < steps; ++j) {
-        orc_c128* x = xs + (long)j * n;
-        orc_c128* y = ys + (long)j * n;
-        ldbp_fir(taps, fir + (long)j * taps, n, x, y);
-        ldbp_phase(ktaps, kerr + (long)j * ktaps, n, y, phi);
-        for (int t = 0; t < n; ++t) xs[(long)(j + 1) * n + t] = c_mul(y[t], i_exp(phi[t]));
+        orc_c128* x = xs + (long)j * mn;
+        orc_c128* y = ys + (long)j * mn;
+        for (int md = 0; md < modes; ++md) ldbp_fir(taps, fir + (long)j * taps, n, x + (long)md * n, y + (long)md * n);
+        ldbp_phase(ktaps, kerr + (long)j * ktaps, modes, n, y, phi);
+        for (long i = 0; i < mn; ++i) xs[(long)(j + 1) * mn + i] = c_mul(y[i], i_exp(phi[i % n]));
     }
     double loss = 0;
-    orc_c128* g = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
-    orc_c128* gy = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
-    for (int t = 0; t < n; ++t) {
-        orc_c128 e = c_sub(xs[(long)steps * n + t], target[t]);
+    orc_c128* g = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)mn);
+    orc_c128* gy = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)mn);
+    for (long i = 0; i < mn; ++i) {
+        orc_c128 e = c_sub(xs[(long)steps * mn + i], target[i]);
         loss += c_norm(e);
-        g[t] = c_scale(e, 1.0 / n);
+        g[i] = c_scale(e, 1.0 / mn);
     }
     int c = taps / 2, P = ktaps / 2;
     for (int j = steps - 1; j >= 0; --j) {
-        const orc_c128* x = xs + (long)j * n;
-        const orc_c128* y = ys + (long)j * n;
-        const orc_c128* z = xs + (long)(j + 1) * n;
+        const orc_c128* x = xs + (long)j * mn;
+        const orc_c128* y = ys + (long)j * mn;
+        const orc_c128* z = xs + (long)(j + 1) * mn;
         const double* gk = kerr + (long)j * ktaps;
-        ldbp_phase(ktaps, gk, n, y, phi);
-        for (int t = 0; t < n; ++t) q[t] = -c_mul(c_conj(g[t]), z[t]).im;
+        ldbp_phase(ktaps, gk, modes, n, y, phi);
+        /* q = -sum_modes Im(conj(G_z) z): the phase is common to the modes */
+        for (int t = 0; t < n; ++t) {
+            q[t] = 0;
+            for (int md = 0; md < modes; ++md) q[t] -= c_mul(c_conj(g[(long)md * n + t]), z[(long)md * n + t]).im;
+        }
         for (int m = 0; m < ktaps; ++m) {
             double acc = 0;
-            for (int t = 0; t < n; ++t) acc += 2.0 * q[t] * c_norm(y[wrap(t - m + P, n)]);
+            for (int t = 0; t < n; ++t) {
+                double p = 0;
+                for (int md = 0; md < modes; ++md) p += c_norm(y[(long)md * n + wrap(t - m + P, n)]);
+                acc += 2.0 * q[t] * p;
+            }
             grad_kerr[(long)j * ktaps + m] = acc;
         }
         for (int s = 0; s < n; ++s) {
             double dp = 0;
             for (int m = 0; m < ktaps; ++m) dp += gk[m] * 2.0 * q[wrap(s + m - P, n)];
-            gy[s] = c_add(c_mul(g[s], i_exp(-phi[s])), c_scale(y[s], dp));
+            for (int md = 0; md < modes; ++md) {
+                long i = (long)md * n + s;
+                gy[i] = c_add(c_mul(g[i], i_exp(-phi[s])), c_scale(y[i], dp));
+            }
         }
-        /* FIR: G_h_k = sum_t G_y(t) conj(x(t-k+c));  G_x(s) = sum_k conj(h_k) G_y(s+k-c) */
+        /* FIR: G_h_k = sum_modes sum_t G_y(t) conj(x(t-k+c));  G_x(s) = sum_k conj(h_k) G_y(s+k-c) */
         const orc_c128* h = fir + (long)j * taps;
         for (int k = 0; k < taps; ++k) {
             orc_c128 acc = {0, 0};
-            for (int t = 0; t < n; ++t) acc = c_add(acc, c_mul(gy[t], c_conj(x[wrap(t - k + c, n)])));
+            for (int md = 0; md < modes; ++md)
+                for (int t = 0; t < n; ++t)
+                    acc = c_add(acc, c_mul(gy[(long)md * n + t], c_conj(x[(long)md * n + wrap(t - k + c, n)])));
             grad_fir[(long)j * taps + k] = acc;
         }
-        for (int s = 0; s < n; ++s) {
-            orc_c128 acc = {0, 0};
-            for (int k = 0; k < taps; ++k) acc = c_add(acc, c_mul(c_conj(h[k]), gy[wrap(s + k - c, n)]));
-            g[s] = acc;
-        }
+        for (int md = 0; md < modes; ++md)
+            for (int s = 0; s < n; ++s) {
+                orc_c128 acc = {0, 0};
+                for (int k = 0; k < taps; ++k)
+                    acc = c_add(acc, c_mul(c_conj(h[k]), gy[(long)md * n + wrap(s + k - c, n)]));
+                g[(long)md * n + s] = acc;
+            }
     }
     free(gy);
     free(g);
@@ -1200,5 +1223,5 @@ double orc_ldbp_backward(int steps, int taps, int ktaps, const orc_c128* fir, co
     free(phi);
     free(ys);
     free(xs);
-    return loss / n;
+    return loss / mn;
 }

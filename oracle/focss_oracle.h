@@ -134,10 +134,10 @@ int orc_wdmppe_train(int c_radius, int d_radius, int samples, const orc_c128* tx
 /* ---- learned few-step DBP (no reference counterpart; judged by plain DBP) ----
  * fir [steps][taps] complex, kerr [steps][ktaps] real intensity taps of the
  * nonlinear phase (ktaps = 1: exp(i theta |y|^2)). */
-double orc_ldbp_forward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr,
+double orc_ldbp_forward(int steps, int taps, int ktaps, int modes, const orc_c128* fir, const double* kerr,
                         int samples, const orc_c128* in, orc_c128* out,
                         const orc_c128* target);
-double orc_ldbp_backward(int steps, int taps, int ktaps, const orc_c128* fir, const double* kerr,
+double orc_ldbp_backward(int steps, int taps, int ktaps, int modes, const orc_c128* fir, const double* kerr,
                          int samples, const orc_c128* in, const orc_c128* target,
                          orc_c128* grad_fir, double* grad_kerr);
 

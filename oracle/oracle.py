@@ -113,8 +113,8 @@ _sig = {
     "orc_wdmppe_n_features": (C.c_int, [C.c_int, C.c_int]),
     "orc_wdmppe_features": (None, [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
     "orc_wdmppe_train": (C.c_int, [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
-    "orc_ldbp_forward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
-    "orc_ldbp_backward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
+    "orc_ldbp_forward": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
+    "orc_ldbp_backward": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -447,19 +447,20 @@ def _kerr(kerr):
     return k.reshape(k.shape[0], 1) if k.ndim == 1 else k
 
 
-def ldbp_forward(fir, kerr, x, target=None):
+def ldbp_forward(fir, kerr, x, target=None, modes=1):
+    """x, target: [modes][n] flattened (mode-major)."""
     fir = _c(fir)
     k = _kerr(kerr)
     x = _c(x)
     steps, taps = fir.shape
     out = np.zeros_like(x)
     tgt = _c(target) if target is not None else None
-    loss = _lib.orc_ldbp_forward(steps, taps, k.shape[1], _ptr(fir), _ptr(k), x.size, _ptr(x), _ptr(out),
-                                 _ptr(tgt) if tgt is not None else None)
+    loss = _lib.orc_ldbp_forward(steps, taps, k.shape[1], modes, _ptr(fir), _ptr(k), x.size // modes, _ptr(x),
+                                 _ptr(out), _ptr(tgt) if tgt is not None else None)
     return out, loss
 
 
-def ldbp_backward(fir, kerr, x, target):
+def ldbp_backward(fir, kerr, x, target, modes=1):
     """Returns (loss, dL/d conj(fir), dL/d kerr) with kerr's shape."""
     fir = _c(fir)
     shape = np.shape(kerr)
@@ -468,6 +469,6 @@ def ldbp_backward(fir, kerr, x, target):
     steps, taps = fir.shape
     gf = np.zeros_like(fir)
     gk = np.zeros_like(k)
-    loss = _lib.orc_ldbp_backward(steps, taps, k.shape[1], _ptr(fir), _ptr(k), x.size, _ptr(x),
+    loss = _lib.orc_ldbp_backward(steps, taps, k.shape[1], modes, _ptr(fir), _ptr(k), x.size // modes, _ptr(x),
                                   _ptr(target), _ptr(gf), _ptr(gk))
     return loss, gf, gk.reshape(shape)

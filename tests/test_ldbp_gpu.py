@@ -127,3 +127,83 @@ def test_ldbp_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
         err = np.linalg.norm(out[sl] - o) / np.linalg.norm(o)
         print(taps, ktaps, n, steps, b, "rel L2 %.2e (bound %.1e)" % (err, tol))
         assert err < tol
+
+
+@pytest.mark.parametrize("taps,ktaps,n,batch", [(15, 9, 4096, 2), (31, 1, 8192, 1), (5, 5, 1024, 3)])
+def test_ldbp_two_modes_forward_and_gradients(ctx, taps, ktaps, n, batch):
+    """Manakov model (modes=2, fields [batch][2][n]): FP32 kernels against the
+    oracle (pinned for two modes in test_oracle.py).  Small phases, as in the
+    tight one-mode case: the summed intensity doubles them, so theta is
+    scaled by 1/8."""
+    import focss_amd as F
+    steps = 6
+    print(f"ldbp two modes taps={taps} ktaps={ktaps} n={n} batch={batch}", flush=True)
+    fir, theta, x, tgt = case(steps, taps, 2 * n, batch, seed=7, ktaps=ktaps)
+    theta = (theta * 0.125).astype(np.float32).astype(np.float64)
+    model = F.LearnedDBP(ctx, steps, taps, n, batch, fir, theta, modes=2)
+    bx, bt, bo = ctx.alloc(x.nbytes), ctx.alloc(tgt.nbytes), ctx.alloc(x.nbytes)
+    bx.upload(x)
+    bt.upload(tgt)
+    loss = model.forward(bx, bo, bt)
+    out = bo.download(np.empty_like(x))
+    loss_ref, gf_ref, gt_ref = 0.0, np.zeros_like(fir), np.zeros_like(theta)
+    for b in range(batch):
+        sl = slice(b * 2 * n, (b + 1) * 2 * n)
+        xb, tb = x[sl].astype(np.complex128), tgt[sl].astype(np.complex128)
+        o, _ = O.ldbp_forward(fir, theta, xb, tb, modes=2)
+        assert np.linalg.norm(out[sl] - o) / np.linalg.norm(o) < 5e-5
+        lb, gf, gt = O.ldbp_backward(fir, theta, xb, tb, modes=2)
+        loss_ref += lb / batch
+        gf_ref += gf / batch
+        gt_ref += gt / batch
+    assert loss == pytest.approx(loss_ref, rel=1e-5)
+    l2, gf, gt = model.gradients(bx, bt)
+    assert l2 == pytest.approx(loss_ref, rel=1e-5)
+    assert np.linalg.norm(gf - gf_ref) / np.linalg.norm(gf_ref) < 2e-4
+    gt = gt.reshape(np.shape(gt_ref))
+    assert np.linalg.norm(gt - gt_ref) / np.linalg.norm(gt_ref) < 2e-4
+
+
+@pytest.mark.parametrize("taps,ktaps,n,batch,steps", [(31, 9, 8192, 2, 6), (15, 1, 2048, 3, 6), (31, 9, 4096, 1, 60)])
+def test_ldbp_two_modes_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
+    """Manakov matrix-core inference against the oracle (6 steps) or the FP32
+    kernels (60 steps), bound 6e-4 sqrt(steps) as for one mode."""
+    import focss_amd as F
+    print(f"ldbp two modes infer taps={taps} ktaps={ktaps} n={n} batch={batch} steps={steps}", flush=True)
+    r = np.random.default_rng(43)
+    fir = np.zeros((steps, taps), dtype=np.complex128)
+    fir[:, taps // 2] = 1.0
+    fir += 0.03 * (r.standard_normal((steps, taps)) + 1j * r.standard_normal((steps, taps))) / np.sqrt(taps / 15)
+    g = 0.025 / ktaps * r.standard_normal((steps, ktaps))
+    fir = fir.astype(np.complex64).astype(np.complex128)
+    g = g.astype(np.float32).astype(np.float64)
+    m = 2 * batch * n
+    x = ((r.standard_normal(m) + 1j * r.standard_normal(m)) * 0.5).astype(np.complex64)
+    model = F.LearnedDBP(ctx, steps, taps, n, batch, fir, g, modes=2)
+    bx, bo, bf = ctx.alloc(x.nbytes), ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
+    bx.upload(x)
+    model.infer(bx, bo)
+    model.forward(bx, bf)
+    out = bo.download(np.empty_like(x))
+    out32 = bf.download(np.empty_like(x))
+    tol = 6e-4 * np.sqrt(steps)
+    for b in range(batch):
+        sl = slice(b * 2 * n, (b + 1) * 2 * n)
+        o = O.ldbp_forward(fir, g, x[sl].astype(np.complex128), modes=2)[0] if steps <= 6 else out32[sl]
+        err = np.linalg.norm(out[sl] - o) / np.linalg.norm(o)
+        print(f"  waveform {b}: rel L2 {err:.2e} (bound {tol:.1e})", flush=True)
+        assert err < tol
+
+
+def test_ldbp_modes_guard(ctx):
+    import focss_amd as F
+    fir = np.zeros((2, 5), dtype=np.complex128)
+    fir[:, 2] = 1
+    with pytest.raises(F.FocssError, match="samples"):
+        m = F.LearnedDBP(ctx, 2, 5, 16384, 1, fir, np.zeros(2), modes=2)
+        b = ctx.alloc(2 * 16384 * 8)
+        m.forward(b, b)
+    with pytest.raises(F.FocssError, match="modes"):
+        m = F.LearnedDBP(ctx, 2, 5, 1024, 1, fir, np.zeros(2), modes=3)
+        b = ctx.alloc(3 * 1024 * 8)
+        m.forward(b, b)

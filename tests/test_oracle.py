@@ -540,3 +540,50 @@ def test_ldbp_filtered_kerr_closed_form_and_fd():
             km[j, m] -= eps
             d = (O.ldbp_forward(fir, kp, x, target)[1] - O.ldbp_forward(fir, km, x, target)[1]) / (2 * eps)
             assert d == pytest.approx(gk[j, m], rel=1e-6, abs=1e-9)
+
+
+def test_ldbp_two_modes_closed_form_decoupling_and_fd():
+    """Dual polarisation (Manakov): one FIR per step on each mode and one
+    phase from the intensity summed over the modes.  Identity FIR and one
+    Kerr tap give x_m exp(i theta (|x_0|^2 + |x_1|^2)); with mode 1 zero the
+    model reduces to the one-mode model; gradients match central finite
+    differences of the loss, which averages over both modes."""
+    r = rng(29)
+    n, steps, taps, ktaps = 40, 2, 5, 3
+    x = crandn(r, 2 * n) * 0.6
+    target = crandn(r, 2 * n) * 0.6
+    ident = np.zeros((1, taps), dtype=np.complex128)
+    ident[0, taps // 2] = 1.0
+    out, _ = O.ldbp_forward(ident, np.array([0.37]), x, modes=2)
+    p = np.abs(x[:n]) ** 2 + np.abs(x[n:]) ** 2
+    ref = x * np.exp(1j * 0.37 * np.concatenate([p, p]))
+    assert np.allclose(out, ref, rtol=0, atol=1e-14)
+    fir = np.zeros((steps, taps), dtype=np.complex128)
+    fir[:, taps // 2] = 1.0
+    fir = fir + 0.1 * crandn(r, steps * taps).reshape(steps, taps)
+    kerr = 0.2 * r.standard_normal((steps, ktaps))
+    x0 = np.concatenate([x[:n], np.zeros(n)])
+    one, _ = O.ldbp_forward(fir, kerr, x[:n])
+    two, _ = O.ldbp_forward(fir, kerr, x0, modes=2)
+    assert np.allclose(two[:n], one, rtol=0, atol=1e-14) and np.all(two[n:] == 0)
+    loss, gf, gk = O.ldbp_backward(fir, kerr, x, target, modes=2)
+    assert loss == pytest.approx(O.ldbp_forward(fir, kerr, x, target, modes=2)[1], rel=1e-14)
+    eps = 1e-6
+
+    def lossf(f, k):
+        return O.ldbp_forward(f, k, x, target, modes=2)[1]
+
+    for (j, k) in [(0, 0), (1, 2), (1, 4)]:
+        for part in (1.0, 1j):
+            fp, fm = fir.copy(), fir.copy()
+            fp[j, k] += eps * part
+            fm[j, k] -= eps * part
+            d = (lossf(fp, kerr) - lossf(fm, kerr)) / (2 * eps)
+            ana = 2 * (gf[j, k].real if part == 1.0 else gf[j, k].imag)
+            assert d == pytest.approx(ana, rel=1e-6, abs=1e-9)
+    for j in range(steps):
+        for m in range(ktaps):
+            kp, km = kerr.copy(), kerr.copy()
+            kp[j, m] += eps
+            km[j, m] -= eps
+            assert (lossf(fir, kp) - lossf(fir, km)) / (2 * eps) == pytest.approx(gk[j, m], rel=1e-6, abs=1e-9)
