@@ -133,7 +133,7 @@ def _signal_psd(k):
     return p
 
 
-def init_params(n, steps_per_span=2, taps=63, floor=0.01, kerr_taps=1):
+def init_params(n, steps_per_span=2, taps=63, floor=0.01, kerr_taps=1, n_spans=N_SPANS, kappa=1.0):
     """Physics initialisation.  FIR j is a weighted least-squares fit of the
     BACKWARD linear operator of its h = L / steps_per_span (loss gain
     e^{alpha h/2}, dispersion e^{-i beta2 h w^2/2}, the span's drop_power on
@@ -141,8 +141,10 @@ def init_params(n, steps_per_span=2, taps=63, floor=0.01, kerr_taps=1):
     fitted to the running error of the chain so far (target = ideal response
     of steps 0..j divided by the fitted response of steps 0..j-1), so the
     per-step truncation errors do not pile up along the 40 steps.
-    theta_j = -kappa gamma L_eff(h) for the field at the start of the step,
-    placed on the centre intensity tap (the others start at 0)."""
+    theta_j = -kappa gamma L_eff(h) for the field at the start of the step
+    (kappa = 8/9 for the Manakov model driven by the summed intensity of both
+    polarisations), placed on the centre intensity tap (the others start
+    at 0)."""
     h = SPAN_LEN / steps_per_span
     dt = 1.0 / (BAUD * 2)
     k = np.fft.fftfreq(n, d=dt) * 2 * np.pi
@@ -154,7 +156,7 @@ def init_params(n, steps_per_span=2, taps=63, floor=0.01, kerr_taps=1):
     firs, thetas = [], []
     ideal = np.ones(n, dtype=np.complex128)
     chain = np.ones(n, dtype=np.complex128)
-    for span in range(N_SPANS):
+    for span in range(n_spans):
         for j in range(steps_per_span):
             Hd = np.exp(ALPHA * h / 2) * np.exp(1j * BETA2 / 2 * (-h) * k ** 2)
             if j == 0:
@@ -165,7 +167,7 @@ def init_params(n, steps_per_span=2, taps=63, floor=0.01, kerr_taps=1):
             chain = chain * (A @ coef)
             firs.append(coef)
             g = np.zeros(kerr_taps)
-            g[kerr_taps // 2] = -GAMMA * l_eff
+            g[kerr_taps // 2] = -kappa * GAMMA * l_eff
             thetas.append(g)
     return np.array(firs), np.array(thetas)
 
