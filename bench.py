@@ -4,7 +4,7 @@
 20 x 80 km SSMF with the ADAPTIVE grid (max nonlinear phase 1e-3 rad,
 max step 1 km), EDFA gain 16 dB and seeded ASE at NF 5 dB, on one MI355X.
 
-One bench step = the whole 20-span link for the batch (default 16 independent
+One bench step = the whole 20-span link for the batch (default 32 independent
 realisations).
 Metric: GSample-steps/s = sum over waveforms of samples x split steps / s.
 Prints one JSON line.  FP64 (complex128) is the headline; the FP32 path
@@ -113,6 +113,31 @@ def run_gpu(F, ctx, fields, dt, precision, steps, warmup, batch):
                 finite=bool(np.all(np.isfinite(out))), output_power=float(np.mean(np.abs(out) ** 2)))
 
 
+def f32_error_per_span(F, ctx, field, dt, grid):
+    """Relative L2 error of the FP32 path against the FP64 path after each
+    span, one realisation carried span by span through both precisions with
+    identical ASE (seed per span).  The FP64 device path is the reference:
+    the tests hold it to the CPU oracle within 1e-10."""
+    gain = 10 ** (ALPHA_DB_KM * SPAN_LEN * 1e-3 / 10)
+    sigma = np.sqrt(PLANCK * F_C * 10 ** (NF_DB / 10) * (gain - 1) / 2 / dt / 2)
+    b64, b32 = ctx.alloc(field.size * 16), ctx.alloc(field.size * 8)
+    b64.upload(field.astype(np.complex128))
+    b32.upload(field.astype(np.complex64))
+    off = np.zeros(1, dtype=np.int64)
+    errs = []
+    for s in range(SPANS):
+        link = F.make_link(F.Fiber(**fiber_dict()), grid, F.FORWARD, 1, N_SAMPLES, dt, center_frequency=F_C,
+                           post_scale=np.sqrt(gain), noise_sigma=sigma, noise_seed=SEED + 1000 + s)
+        ctx.link_run([link], b64, off, precision=F.F64)
+        ctx.link_run([link], b32, off, precision=F.F32)
+        y64 = b64.download(np.empty(field.size, dtype=np.complex128))
+        y32 = b32.download(np.empty(field.size, dtype=np.complex64)).astype(np.complex128)
+        errs.append(float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64)))
+    b64.free()
+    b32.free()
+    return errs
+
+
 def cpu_baseline(field, dt, km):
     """The CPU FP64 oracle (a restatement of the reference path, single
     thread) on the first `km` km of span 1 of the same field."""
@@ -155,11 +180,16 @@ def main():
         fields.append(f)
     field = fields[0]
     res = {}
+    span_err = None
     with F.Context(0) as ctx:
         if args.precision in ("f64", "both"):
             res["f64"] = run_gpu(F, ctx, fields, dt, F.F64, args.steps, args.warmup, args.batch)
         if args.precision in ("f32", "both"):
             res["f32"] = run_gpu(F, ctx, fields, dt, F.F32, args.steps, args.warmup, args.batch)
+        if args.precision == "both":
+            span_err = {"adaptive": f32_error_per_span(F, ctx, field, dt, F.Grid(F.ADAPTIVE, 0, PHI_MAX, H_MAX)),
+                        "uniform_100_steps_per_span": f32_error_per_span(F, ctx, field, dt,
+                                                                         F.Grid(F.UNIFORM, 100, 0.0, 0.0))}
     head = res.get("f64") or res.get("f32")
     head_key = "f64" if "f64" in res else "f32"
     traffic = load_traffic(head_key)
@@ -198,6 +228,12 @@ def main():
                        "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
                        "time_pass_ms": round(r["time_pass_ms"], 5),
                        "spec_pass_ms": round(r["spec_pass_ms"], 5)}
+        if span_err:
+            # the ADAPTIVE schedule follows each precision's own peak power, so
+            # the two runs take different steps; the UNIFORM run isolates the
+            # arithmetic
+            line["f32"]["rel_l2_vs_f64_per_span"] = {k: [float(f"{e:.3g}") for e in v]
+                                                     for k, v in span_err.items()}
     if not args.no_learned:
         # the metric's second half: Q of learned few-step DBP vs plain FP64 DBP
         from focss_amd.learned import parser as learned_parser, run as learned_run
