@@ -138,21 +138,37 @@ def f32_error_per_span(F, ctx, field, dt, grid):
     return errs
 
 
-def cpu_baseline(field, dt, km):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(field, dt, km, runs=3):
     """The CPU FP64 oracle (a restatement of the reference path, single
-    thread) on the first `km` km of span 1 of the same field."""
+    thread) on the first `km` km of span 1 of the same field; the median of
+    `runs` runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     fd = fiber_dict()
     fd["length"] = km * 1e3
     fib = O.fiber(**fd)
     grd = O.grid(O.ADAPTIVE, 0, PHI_MAX, H_MAX)
-    t0 = time.perf_counter()
-    _, st = O.ssfm_run(fib, grd, 1.0, field, dt)
-    t = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, st = O.ssfm_run(fib, grd, 1.0, field, dt)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
     return dict(value=N_SAMPLES * st / t / 1e9, unit="GSample-steps/s", cores=1, kind="port",
                 sample=f"FP64 oracle (own Stockham FFT, 1 thread), first {km:g} km of span 1 "
-                       f"({st} adaptive steps, 2^18 samples), {t:.1f} s")
+                       f"({st} adaptive steps, 2^18 samples), median of {runs} runs {t:.2f} s; "
+                       f"host {cpu_model()}, {os.cpu_count()} logical CPUs")
 
 
 def main():
