@@ -24,6 +24,12 @@ Report: Q (dB, mean over test realisations and both polarisations, RRC
 matched filter and constant-phase alignment) of every method, and
 waveforms/s (one waveform = both polarisations of one centre channel) on
 batches of `rate_batch` waveforms.
+Full-length link (long_link): the trained model on the centre channel of a
+link with 2^15 symbols per channel (2^20 samples per polarisation, 2^16 at
+2 sps), by overlap-save blocks of 8192 samples with 2048 samples of context
+on each side (the model's impulse response spans at most 15 taps x 120
+steps = 1800 samples either way), against plain FP64 DBP at 32 steps per
+span on the whole waveform.
 """
 import argparse
 import json
@@ -36,11 +42,11 @@ from .learned import ALPHA, BETA2, F_C, GAMMA, NF_DB, PLANCK, SPAN_LEN, WAVELENG
 from .manakov import BAUD, KAPPA, OSF, ROLL_OFF, SPACING
 
 
-def simulate(F, ctx, n_real, seed, args):
+def simulate(F, ctx, n_real, seed, args, n_sym=None):
     """Returns (rx [n_real][2][n2] centre channel at 2 sps, tx [n_real][2][n2]
     transmitted centre channel at 2 sps, symbols [n_real][2][n_sym])."""
     from .transmitter import rrc_shaping, wdm_signal
-    n_sym = args.symbols
+    n_sym = n_sym or args.symbols
     n = n_sym * OSF
     coi = args.channels // 2
     fib = F.Fiber(ALPHA, BETA2, GAMMA, SPAN_LEN, WAVELENGTH, KAPPA)
@@ -107,6 +113,25 @@ def run_links(F, ctx, links, rx, precision, reps=0):
         t = (time.perf_counter() - t0) / reps
     buf.free()
     return out, t
+
+
+def overlap_blocks(x, n_block, guard):
+    """Overlap-save blocks of a periodic waveform x [2][L]: block b holds the
+    samples b hop - guard .. b hop - guard + n_block (circular), hop =
+    n_block - 2 guard; returns [L / hop][2][n_block]."""
+    length = x.shape[1]
+    hop = n_block - 2 * guard
+    if length % hop:
+        raise ValueError("the waveform length must be a multiple of the hop")
+    idx = (np.arange(length // hop)[:, None] * hop - guard + np.arange(n_block)[None, :]) % length
+    return np.ascontiguousarray(x[:, idx].transpose(1, 0, 2))
+
+
+def stitch_blocks(blocks, guard):
+    """The interiors [guard, guard + hop) of the blocks, concatenated: [2][L]."""
+    nb, modes, n_block = blocks.shape
+    hop = n_block - 2 * guard
+    return blocks[:, :, guard:guard + hop].transpose(1, 0, 2).reshape(modes, nb * hop)
 
 
 def q_mean(out, syms):
@@ -190,6 +215,8 @@ def run(args, log=print):
                 fn(r_in, r_out)
             ctx.synchronize()
             rates[name] = rb * 5 / (time.perf_counter() - t0)
+        if args.long_symbols:
+            rep["long_link"] = long_link(F, ctx, args, model, steps, n2, log)
         rep.update({
             "steps_per_span": spp, "taps": taps, "kerr_taps": kt, "iters": args.iters, "batch": batch,
             "train_seconds": round(t_train, 2), "loss_first": losses[0] if losses else None,
@@ -200,6 +227,54 @@ def run(args, log=print):
         for b in (te_in, te_out, xin, tgt, r_in, r_out):
             b.free()
         return rep
+
+
+def long_link(F, ctx, args, model, steps, n_block, log):
+    """The trained model on the centre channel of one full-length link
+    (long_symbols per channel, 2^20 samples per polarisation at 2^15):
+    overlap-save blocks of n_block samples with `guard` samples of context on
+    each side, all blocks in one matrix-core batch, against plain FP64 DBP at
+    32 steps per span on the whole waveform.  Rates per long waveform on
+    batches of `long_rate_batch` copies; block assembly runs on the host and
+    is not timed."""
+    rx, _, sy = simulate(F, ctx, 1, args.seed + 2, args, n_sym=args.long_symbols)
+    length = rx.shape[2]
+    out, _ = run_links(F, ctx, dbp_links(F, args, 1, length, 32), rx, F.F64)
+    res = {"samples_per_polarisation": length, "block": n_block, "guard": args.guard,
+           "q_db": {"plain_dbp_32_steps_per_span_f64": q_mean(out, sy)}}
+    blocks = overlap_blocks(rx[0], n_block, args.guard)
+    nb = blocks.shape[0]
+    m = F.LearnedDBP(ctx, steps, args.taps, n_block, nb, *model.get(), modes=2)
+    b_in, b_out = ctx.alloc(blocks.size * 8), ctx.alloc(blocks.size * 8)
+    b_in.upload(blocks.astype(np.complex64).ravel())
+    m.infer(b_in, b_out)
+    y = b_out.download(np.empty(blocks.size, dtype=np.complex64)).reshape(blocks.shape).astype(np.complex128)
+    res["q_db"]["learned_fp16_mfma_overlap_save"] = q_mean(stitch_blocks(y, args.guard)[None], sy)
+    b_in.free()
+    b_out.free()
+    # rates: waveforms of `length` samples per polarisation
+    rb = args.long_rate_batch
+    _, t = run_links(F, ctx, dbp_links(F, args, rb, length, 32), np.repeat(rx, rb, axis=0), F.F64, reps=2)
+    rates = {"plain_dbp_32_f64": rb / t}
+    mb = F.LearnedDBP(ctx, steps, args.taps, n_block, rb * nb, *model.get(), modes=2)
+    r_in, r_out = ctx.alloc(rb * blocks.size * 8), ctx.alloc(rb * blocks.size * 8)
+    r_in.upload(np.tile(blocks.astype(np.complex64).ravel(), rb))
+    mb.infer(r_in, r_out)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        mb.infer(r_in, r_out)
+    ctx.synchronize()
+    rates["learned_fp16_mfma_overlap_save"] = rb * 5 / (time.perf_counter() - t0)
+    r_in.free()
+    r_out.free()
+    res["waveforms_per_s"] = rates
+    res["speedup_learned_vs_dbp32_f64"] = rates["learned_fp16_mfma_overlap_save"] / rates["plain_dbp_32_f64"]
+    res["q_gap_learned_vs_dbp32_db"] = (res["q_db"]["learned_fp16_mfma_overlap_save"] -
+                                        res["q_db"]["plain_dbp_32_steps_per_span_f64"])
+    log(f"long link 2^{int(np.log2(length))} per pol: DBP32 Q {res['q_db']['plain_dbp_32_steps_per_span_f64']:.3f}, "
+        f"learned Q {res['q_db']['learned_fp16_mfma_overlap_save']:.3f}")
+    return res
 
 
 def parser():
@@ -219,6 +294,10 @@ def parser():
     ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--theta-lr-mult", type=float, default=10.0)
     ap.add_argument("--rate-batch", type=int, default=512)
+    ap.add_argument("--long-symbols", type=int, default=1 << 15,
+                    help="symbols per channel of the full-length link (0: skip it)")
+    ap.add_argument("--guard", type=int, default=2048, help="overlap-save context on each side of a block")
+    ap.add_argument("--long-rate-batch", type=int, default=32)
     return ap
 
 

@@ -30,7 +30,7 @@ def test_manakov_learned_dbp_small():
     from focss_amd.manakov_learned import parser, run
     args = parser().parse_args(["--channels", "3", "--symbols", "512", "--spans", "10", "--train", "8",
                                 "--test", "4", "--iters", "300", "--batch", "8", "--rate-batch", "8",
-                                "--power-dbm", "2"])
+                                "--power-dbm", "2", "--long-symbols", "0"])
     rep = run(args, log=print)
     q = rep["q_db"]
     print(rep)
