@@ -333,10 +333,22 @@ class LearnedDBP:
         self.m.upload(zeros)
         self.v.upload(zeros)
         self.t = 0
+        # training buffers (activations of every step) on the first
+        # gradients / train_step call; inference needs none
         self.ws_bytes = _lib.focss_ldbp_workspace_bytes(steps, modes, samples, batch)
-        self.ws = ctx.alloc(self.ws_bytes)
-        self.grad = ctx.alloc(8 * (self.n_params + 1))
+        self.ws = None
+        self.grad = None
         self.lr_scale = None
+
+    def _training_buffers(self):
+        if self.ws is None:
+            self.ws = self.ctx.alloc(self.ws_bytes)
+            self.grad = self.ctx.alloc(8 * (self.n_params + 1))
+
+    def free(self):
+        for b in (self.params, self.m, self.v, self.ws, self.grad, self.lr_scale):
+            if b is not None:
+                b.free()
 
     def pack(self, fir, kerr):
         fir = np.ascontiguousarray(fir, dtype=np.complex64).reshape(self.steps, self.taps)
@@ -374,6 +386,7 @@ class LearnedDBP:
 
     def gradients(self, inp, target):
         """(loss, Wirtinger grad of the taps [steps][taps] complex, dL/dg [steps][kerr_taps])."""
+        self._training_buffers()
         self.grad.upload(np.zeros(self.n_params + 1, dtype=np.float64))
         gp = self.grad.ptr.value
         _check(_lib.focss_ldbp_gradients(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
@@ -386,6 +399,7 @@ class LearnedDBP:
                 g[1 + nf:].reshape(self.steps, self.kerr_taps))
 
     def train_step(self, inp, target, lr):
+        self._training_buffers()
         self.t += 1
         loss = C.c_double(0)
         _check(_lib.focss_ldbp_train_step(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,

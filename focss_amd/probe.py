@@ -1,10 +1,21 @@
-"""Short measurement run for hardware-counter profiles (rocprofv3 --pmc).
+"""Short measurement runs for rocprofv3 (kernel traces and --pmc counters).
 
-Runs the first `pairs` split steps of the benchmark workload (a batch of
-5-channel WDM fields, 2^18 samples, adaptive 80 km spans) with one kernel
-launch per pass, then one plain 16-byte copy kernel over `copy_mib` MiB,
-whose counters calibrate HBM bytes.  Usage:
+--workload ssfm (default): the first `pairs` split steps of the benchmark
+workload (a batch of 5-channel WDM fields, 2^18 samples, adaptive 80 km
+spans) with one kernel launch per pass, then one plain 16-byte copy kernel
+over `copy_mib` MiB, whose counters calibrate HBM bytes.
+--workload side: the other hot kernels at their measured sizes, `reps`
+launches each --
+  learned DBP matrix-core inference, one polarisation: 6 steps/span x 20
+    spans, 31 taps, 9 intensity taps, 512 waveforms of 2^14 samples;
+  the same, two polarisations: 4 steps/span x 30 spans, 512 waveforms of
+    2 x 8192 samples;
+  the save_field record reorder of 11 channels x 2 polarisations x 2^20
+    samples, both directions, FP64;
+  WdmPPE training, c = 30, d = 2, 2^14 samples (features, Gram, sums).
+Usage:
     python -m focss_amd.probe [--batch 32] [--pairs 12] [--precision f64|f32] [--variant V]
+    python -m focss_amd.probe --workload side [--reps 5]
 """
 import argparse
 import os
@@ -22,7 +33,11 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
     ap.add_argument("--variant", type=int, default=-1, help="tile variant (default: the library's choice)")
+    ap.add_argument("--workload", choices=["ssfm", "side"], default="ssfm")
+    ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
+    if args.workload == "side":
+        return side(args)
     sys.path.insert(0, ROOT)
     import bench
     import focss_amd as F
@@ -52,5 +67,59 @@ def main():
               f"field bytes {host.nbytes}, copy bytes {nbytes}", flush=True)
 
 
+def side(args):
+    sys.path.insert(0, ROOT)
+    import time
+    import focss_amd as F
+    rng = np.random.default_rng(1)
+
+    def crandn(*shape):
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    with F.Context(0) as ctx:
+        for steps, n, modes, label in ((120, 1 << 14, 1, "1 pol"), (120, 8192, 2, "2 pol")):
+            taps, kt, batch = 31, 9, 512
+            fir = np.zeros((steps, taps), dtype=np.complex128)
+            fir[:, taps // 2] = 1
+            fir += 0.03 * crandn(steps, taps)
+            g = 0.01 * rng.standard_normal((steps, kt))
+            m = F.LearnedDBP(ctx, steps, taps, n, batch, fir, g, modes=modes)
+            x = (0.5 * crandn(batch * modes * n)).astype(np.complex64)
+            bi, bo = ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
+            bi.upload(x)
+            m.infer(bi, bo)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.reps):
+                m.infer(bi, bo)
+            ctx.synchronize()
+            t = (time.perf_counter() - t0) / args.reps
+            print(f"probe side: learned inference {label}, {steps} steps, {batch} x {modes} x {n}: "
+                  f"{t * 1e3:.3f} ms per launch", flush=True)
+            for b in (bi, bo):
+                b.free()
+            m.free()
+        env, n = 22, 1 << 20
+        rec = crandn(n, env)
+        rb, fb = ctx.alloc(rec.nbytes), ctx.alloc(rec.nbytes)
+        rb.upload(rec)
+        for _ in range(args.reps):
+            ctx.field_from_records(rb, fb, env, n)
+            ctx.field_to_records(fb, rb, env, n)
+        ctx.synchronize()
+        print(f"probe side: record reorder {env} x {n}, {args.reps} launches each way", flush=True)
+        rb.free()
+        fb.free()
+        n = 1 << 14
+        a6 = crandn(6, n) * 0.5
+        ta, tr = ctx.alloc(a6.nbytes), ctx.alloc(a6.nbytes)
+        ta.upload(a6)
+        tr.upload(a6 + 0.01 * crandn(6, n))
+        for _ in range(args.reps):
+            ctx.wdm_ppe_train(30, 2, ta, tr, n)
+        print(f"probe side: WdmPPE training c=30 d=2 on 2^14 samples, {args.reps} runs", flush=True)
+    return 0
+
+
 if __name__ == "__main__":
-    main()
+    sys.exit(main())
