@@ -790,9 +790,15 @@ int zscore_normal_solve(focss_ctx* c, const cx<double>* F, const cx<double>* y, 
     if (e == hipSuccess) e = hipMalloc(&sums, (size_t)(nf + 1) * sizeof(cx<double>));
     if (e == hipSuccess) e = hipMalloc(&fy, (size_t)nf * sizeof(cx<double>));
     std::vector<cx<double>> hG((size_t)nf * nf), hs(nf + 1), hfy(nf);
+    if (e == hipSuccess) e = hipMemsetAsync(G, 0, (size_t)nf * nf * sizeof(cx<double>), c->stream);
     if (e == hipSuccess) {
-        dim3 gg((nf + 31) / 32, (nf + 31) / 32);
-        hipLaunchKernelGGL(k_gram, gg, dim3(256), 0, c->stream, F, G, rows, nf);
+        // upper 32 x 32 tiles, split over row slices of at least 8 chunks so
+        // that about 1024 workgroups run
+        const int side = (nf + GM_T - 1) / GM_T;
+        const int pairs = side * (side + 1) / 2;
+        const long long max_split = std::max<long long>(1, rows / (8 * GM_KC));
+        const int split = (int)std::min<long long>(std::max(1, 1024 / pairs), max_split);
+        hipLaunchKernelGGL(k_gram_mfma, dim3(pairs, split), dim3(256), 0, c->stream, F, G, rows, nf, side);
         hipLaunchKernelGGL(k_col_sums, dim3(nf + 1), dim3(256), 0, c->stream, F, y, rows, nf, sums, fy);
         e = hipGetLastError();
     }
@@ -807,6 +813,9 @@ int zscore_normal_solve(focss_ctx* c, const cx<double>* F, const cx<double>* y, 
     (void)hipFree(sums);
     (void)hipFree(fy);
     if (e != hipSuccess) return fail(FOCSS_E_HIP, std::string("triplet least squares: ") + hipGetErrorString(e));
+    // the lower triangle from the upper tiles (G is Hermitian)
+    for (int i = 1; i < nf; ++i)
+        for (int j = 0; j < i; ++j) hG[(size_t)i * nf + j] = conj(hG[(size_t)j * nf + i]);
     const double inv = 1.0 / (double)rows;
     std::vector<cx<double>> mu(nf);
     std::vector<double> sd(nf);

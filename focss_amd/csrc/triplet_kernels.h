@@ -9,7 +9,8 @@
 //                  loads).
 //  k_sino_features: the folded symmetric feature rows of SINO::train
 //                  (sino.cpp:67-103), column-major rows x n_features.
-//  k_gram        : G = F^H F over a column-major complex matrix, 32x32 tiles.
+//  k_gram_mfma   : G = F^H F over a column-major complex matrix on the FP64
+//                  matrix cores, upper 32x32 tiles split over row slices.
 //  k_col_sums    : sum_r F(r, j), sum_r conj(F(r, j)) y(r), sum_r y(r).
 //  k_wdm_ppe     : WdmPPE::spm + WdmPPE::xpm over 6 envelopes (wdm_sino.cpp:256-295).
 //  k_wdmppe_features: the feature rows of WdmPPE::train (wdm_sino.cpp:29-81).
@@ -112,34 +113,92 @@ __global__ void k_sino_features(const cx<double>* __restrict__ tx, const cx<doub
     }
 }
 
-// G(i, j) = sum_r conj(F(r, i)) F(r, j); 32x32 output tile per workgroup,
-// rows consumed in chunks of 32 through LDS
-__global__ __launch_bounds__(256) void k_gram(const cx<double>* __restrict__ F, cx<double>* __restrict__ G,
-                                              long long rows, int nf) {
-    __shared__ cx<double> a[32][33];
-    __shared__ cx<double> b[32][33];
-    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
-    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // ty in [0, 8)
-    cx<double> acc[4];
-    for (int q = 0; q < 4; ++q) acc[q] = mk<double>(0, 0);
-    for (long long r0 = 0; r0 < rows; r0 += 32) {
-        for (int q = 0; q < 4; ++q) {
-            int c = ty + 8 * q;  // column within tile
-            long long r = r0 + tx;
-            a[c][tx] = (r < rows && i0 + c < nf) ? F[r + rows * (i0 + c)] : mk<double>(0, 0);
-            b[c][tx] = (r < rows && j0 + c < nf) ? F[r + rows * (j0 + c)] : mk<double>(0, 0);
+// G += F^H F on the FP64 matrix cores (v_mfma_f64_16x16x4f64), upper tiles
+// only (G is Hermitian; the caller mirrors the lower triangle).  With F = A +
+// i B: Re G = A^T A + B^T B, Im G = A^T B - B^T A.  A workgroup owns one
+// 32 x 32 complex tile (ti <= tj, blockIdx.x over the upper pairs) and the
+// rows of one K slice (blockIdx.y of gridDim.y); wave w computes the real
+// 32 x 32 block P^T Q of (P, Q) = (A, A), (B, B), (A, B), (B, A) as 2 x 2
+// MFMA tiles from row chunks staged in LDS, and adds it with its sign into
+// G (FP64 atomics, G zeroed by the caller).
+// MFMA lane layout (FP64 16x16x4): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator v holds D[4 v + l / 16][l % 16] (one row per
+// register, unlike the 4-row groups of the FP16 16x16 instructions).
+constexpr int GM_T = 32;   // complex columns per tile side
+constexpr int GM_KC = 16;  // rows per LDS chunk
+typedef double gm_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gram_mfma(const cx<double>* __restrict__ F, cx<double>* __restrict__ G,
+                                                   long long rows, int nf, int tiles_side) {
+    __shared__ cx<double> fi[GM_KC][GM_T + 1];
+    __shared__ cx<double> fj[GM_KC][GM_T + 1];
+    // upper-triangular pair (ti, tj), ti <= tj, from the linear index
+    int p = blockIdx.x, ti = 0;
+    while (p >= tiles_side - ti) {
+        p -= tiles_side - ti;
+        ++ti;
+    }
+    const int tj = ti + p;
+    const int i0 = ti * GM_T, j0 = tj * GM_T;
+    const long long per = (rows + gridDim.y - 1) / gridDim.y;
+    const long long r_begin = blockIdx.y * per;
+    const long long r_end = r_begin + per < rows ? r_begin + per : rows;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool p_im = wave == 1 || wave == 3;  // P = B for waves 1 (B,B) and 3 (B,A)
+    const bool q_im = wave == 1 || wave == 2;  // Q = B for waves 1 (B,B) and 2 (A,B)
+    gm_d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = gm_d4{0, 0, 0, 0};
+    for (long long r0 = r_begin; r0 < r_end; r0 += GM_KC) {
+        // 2 x 512 complex values: consecutive threads take consecutive rows of one column
+        for (int e = threadIdx.x; e < 2 * GM_KC * GM_T; e += 256) {
+            const int which = e / (GM_KC * GM_T), f = e % (GM_KC * GM_T);
+            const int c = f / GM_KC, rr = f % GM_KC;
+            const long long r = r0 + rr;
+            const int col = (which ? j0 : i0) + c;
+            const cx<double> v = (r < r_end && col < nf) ? F[r + rows * col] : mk<double>(0, 0);
+            if (which)
+                fj[rr][c] = v;
+            else
+                fi[rr][c] = v;
         }
         __syncthreads();
-        for (int rr = 0; rr < 32; ++rr) {
-            cx<double> bv = b[tx][rr];
-            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + conj(a[ty + 8 * q][rr]) * bv;
+#pragma unroll
+        for (int ks = 0; ks < GM_KC / 4; ++ks) {
+            const int k = ks * 4 + (lane >> 4);
+            double av[2], bv[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const cx<double> x = fi[k][t * 16 + (lane & 15)];
+                const cx<double> y = fj[k][t * 16 + (lane & 15)];
+                av[t] = p_im ? x.y : x.x;
+                bv[t] = q_im ? y.y : y.x;
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
     }
-    for (int q = 0; q < 4; ++q) {
-        int i = i0 + ty + 8 * q, j = j0 + tx;
-        if (i < nf && j < nf) G[(long long)i * nf + j] = acc[q];
-    }
+    // waves 0, 1 add to Re G; wave 2 adds, wave 3 subtracts, Im G
+    const double sign = wave == 3 ? -1.0 : 1.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + a * 16 + 4 * v + (lane >> 4);
+                const int j = j0 + b * 16 + (lane & 15);
+                if (i < nf && j < nf) {
+                    double* g = reinterpret_cast<double*>(G + (long long)i * nf + j);
+                    atomicAdd(g + (wave >= 2 ? 1 : 0), sign * acc[a][b][v]);
+                }
+            }
 }
 
 // per column j: sum_r F(r, j) and sum_r conj(F(r, j)) y(r); column nf holds sum_r y(r)
