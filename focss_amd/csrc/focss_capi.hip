@@ -250,6 +250,10 @@ struct focss_ctx {
     int* d_done = nullptr;
     int* h_done = nullptr;  // pinned, [2]
     hipEvent_t ev_done[2] = {nullptr, nullptr};
+    // LDS-resident length classes run on a second stream beside the two-pass
+    // passes (they fill only one workgroup per waveform)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t cap_waves = 0;
     int tile_variant[2][2][21] = {};  // [pass: 0 time, 1 spectral][precision][log2 samples]
     GraphCache graph;
@@ -375,12 +379,12 @@ void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
 }
 
 template <typename R>
-void launch_resident(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
+void launch_resident(hipStream_t s, const LengthClass& lc, PassArgs a, Tables<R> tb) {
     const Registry<R>& r = reg<R>();
     a.waves += lc.first;
     a.log_n = lc.log_n;
     a.tiles_per_wave = 1;
-    hipLaunchKernelGGL(r.resident[lc.log_n][lc.modes - 1], dim3(lc.count), dim3(NT), 0, c->stream, a, tb);
+    hipLaunchKernelGGL(r.resident[lc.log_n][lc.modes - 1], dim3(lc.count), dim3(NT), 0, s, a, tb);
 }
 
 int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
@@ -489,13 +493,14 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         // kernel at a time, each bracketed by events on the context stream
         std::vector<hipEvent_t> ev(2 * (size_t)profile_pairs + 3);
         for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
         HIP_TRY(hipMemcpyAsync(c->d_statics, st.data(), n * sizeof(WaveStatic), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(c->d_states, init.data(), 2 * n * sizeof(WaveState), hipMemcpyHostToDevice,
                                c->stream));
         HIP_TRY(hipMemcpyAsync(c->d_red, red0.data(), 3 * n * sizeof(Reduction), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(c->d_waves, order.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipEventRecord(ev[0], c->stream));
-        for (const LengthClass& lc : resident) launch_resident<R>(c, lc, a, tb);
+        for (const LengthClass& lc : resident) launch_resident<R>(c->stream, lc, a, tb);
         HIP_TRY(hipEventRecord(ev[1], c->stream));
         for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
         HIP_TRY(hipEventRecord(ev[2], c->stream));
@@ -561,14 +566,25 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
             c->graph.key = key;
         }
     }
+    // a previous call's side-stream work (one that returned early) is done
+    // before this call's set-up overwrites the shared state
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     HIP_TRY(hipMemcpyAsync(c->d_statics, st.data(), n * sizeof(WaveStatic), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_states, init.data(), 2 * n * sizeof(WaveState), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_red, red0.data(), 3 * n * sizeof(Reduction), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_waves, order.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     *c->h_done = 0;
     HIP_TRY(hipMemcpyAsync(c->d_done, c->h_done, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    for (const LengthClass& lc : resident) launch_resident<R>(c, lc, a, tb);
-    HIP_TRY(hipGetLastError());
+    if (!resident.empty()) {
+        // resident classes on the side stream, after the set-up above; each
+        // kernel touches only its own waveforms' state, and the done counter
+        // is atomic
+        HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        for (const LengthClass& lc : resident) launch_resident<R>(c->side, lc, a, tb);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    }
     if (!two_pass.empty()) {
         // entry time pass (counter 0), then replays of the graph
         for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
@@ -601,6 +617,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                 return fail(FOCSS_E_NUMERIC, "link program did not terminate within the split-step limit");
         }
     }
+    if (!resident.empty()) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (total_steps) {
         std::vector<WaveState> fin(2 * (size_t)n);
         HIP_TRY(hipMemcpyAsync(fin.data(), c->d_states, 2 * n * sizeof(WaveState), hipMemcpyDeviceToHost,
@@ -1172,6 +1189,9 @@ int focss_ctx_create(int device, focss_ctx** out) {
     HIP_TRY(hipHostMalloc(&c->h_done, 2 * sizeof(int), hipHostMallocDefault));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_done[0], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_done[1], hipEventDisableTiming));
+    HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     *out = c;
     return 0;
 }
@@ -1180,6 +1200,7 @@ int focss_ctx_destroy(focss_ctx* c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     drop_graph(c);
     for (int i = 0; i < 2; ++i) {
         if (c->bank[i]) (void)hipFree(c->bank[i]);
@@ -1198,6 +1219,9 @@ int focss_ctx_destroy(focss_ctx* c) {
     if (c->h_done) (void)hipHostFree(c->h_done);
     for (int i = 0; i < 2; ++i)
         if (c->ev_done[i]) (void)hipEventDestroy(c->ev_done[i]);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1233,6 +1257,7 @@ int focss_memcpy_d2h(focss_ctx* c, void* dst, const void* src, size_t bytes) {
 int focss_synchronize(focss_ctx* c) {
     if (!c) return fail(FOCSS_E_ARG, "null ctx");
     HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->side));
     return 0;
 }
 
