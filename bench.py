@@ -183,8 +183,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the link batch would shard "
-                                   "waveform-wise across GPUs (see DESIGN.md), --gpus > 1 is not run"}))
+        print(json.dumps({"error": "one GPU only: the link batch would shard "
+                                   "waveform-wise across GPUs; --gpus > 1 is not run"}))
         return 2
 
     import focss_amd as F

@@ -30,8 +30,8 @@ def build_hip():
 
 
 def build_host():
-    """libfocss.so from every host/*.cpp except the driver, then the
-    focss_tasks driver executable."""
+    """libfocss.so from every host/*.cpp except tasks/main.cpp, then the
+    focss_tasks command-line program from tasks/main.cpp."""
     host = os.path.join(HERE, "host")
     srcs = []
     for d, _, files in os.walk(host):

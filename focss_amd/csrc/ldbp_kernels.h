@@ -1,5 +1,5 @@
 // Learned few-step digital backpropagation (no counterpart in the
-// reference; judged against plain DBP).  Step j of J:
+// reference; measured against plain DBP).  Step j of J:
 //     y(t)   = sum_k h_{j,k} x((t - k + c) mod N),  c = K / 2       (complex FIR)
 //     p(t)   = |y(t)|^2
 //     phi(t) = sum_m g_{j,m} p((t - m + P) mod N),  P = KP / 2      (filtered intensity)

@@ -6,7 +6,7 @@
    16QAM at 32 GBd, RRC 0.1, simulated at 4 samples/symbol over 20 x 80 km
    SSMF (ADAPTIVE grid, EDFA 16 dB + ASE NF 5 dB, launch 0..4 dBm) by the
    device link program, then decimated to 2 samples/symbol (2^14 samples).
-2. Plain DBP at 16 steps per span (FP64, device link program) is the judge.
+2. Plain DBP at 16 steps per span (FP64, device link program) is the yardstick.
 3. Learned DBP: steps_per_span x 20 steps of (complex FIR, nonlinear phase
    from the intensity through a short real filter -- kerr_taps = 1 is the
    plain Kerr phase theta |y|^2), initialised from the physics (init_params)
@@ -193,7 +193,7 @@ def run(args, log=print):
         rx_tr, tx_tr, _, _ = make_dataset(F, ctx, args.train, n, seed=args.seed)
         rx_te, tx_te, sy_te, p_te = make_dataset(F, ctx, args.test, n, seed=args.seed + 1)
         log(f"datasets: {args.train} + {args.test} waveforms of {n} samples, {time.perf_counter() - t0:.1f} s")
-        # judge: plain DBP at 16 and 2 steps per span (FP64)
+        # yardstick: plain DBP at 16 and 2 steps per span (FP64)
         dbp16 = plain_dbp(F, ctx, rx_te, 16)
         dbp2 = plain_dbp(F, ctx, rx_te, 2)
         q16, q2 = q_of(dbp16, sy_te), q_of(dbp2, sy_te)

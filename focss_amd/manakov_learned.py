@@ -12,7 +12,7 @@ one periodic block of the two-mode learned kernels.  Realisations differ in
 their symbols and ASE.
 Receiver: the centre channel of every realisation and polarisation is
 selected and decimated on the device (wdm_select, decimate).
-Judge: plain DBP (backward_propagation, systems.cpp:73-91) in FP64 at 32
+Yardstick: plain DBP (backward_propagation, systems.cpp:73-91) in FP64 at 32
 steps per span; also 4 and 1 steps per span and dispersion compensation.
 Learned: steps_per_span x 30 steps, each a complex FIR applied to both
 polarisations and a nonlinear phase from the intensity summed over them

@@ -241,7 +241,7 @@ int focss_field_to_records(focss_ctx* ctx, int precision, int envelopes, int sam
  * and the phase driven by the intensity summed over both.  samples is a
  * power of two in [1024, 16384 / modes], taps 5, 15, 23, 31, 47 or 63,
  * kerr_taps 1, 5, 9 or 15.  Loss =
- * (1/(modes samples batch)) sum_{b, mode, t} |z - target|^2.  Judged against
+ * (1/(modes samples batch)) sum_{b, mode, t} |z - target|^2.  Measured against
  * plain DBP (focss_link_run); callers are learned-equaliser training and
  * inference. */
 size_t focss_ldbp_workspace_bytes(int steps, int modes, int samples, int batch);

@@ -1,7 +1,8 @@
 /*
  * focss oracle -- CPU FP64 restatement of the reference's SSFM / DBP /
  * triplet-equalizer hot path.  TEST INFRASTRUCTURE ONLY: the tests, the smoke
- * check and the bench's CPU-baseline leg use it as the checker; the shipped
+ * check and the bench's CPU-baseline leg use it as the reference results are
+ * checked against; the shipped
  * library never links or calls it.
  *
  * Every function cites the reference lines (euav/focss) it restates.  The

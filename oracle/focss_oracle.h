@@ -2,8 +2,9 @@
  * focss oracle -- CPU FP64 restatement of the reference's split-step Fourier
  * hot path, written from the reference's algorithm, used ONLY as test
  * infrastructure: the parity tests, the smoke check and the bench's CPU
- * baseline leg call it as the checker.  Nothing in the shipped library links
- * it, and the GPU path never falls back to it.
+ * baseline leg use it as the reference results are checked against.
+ * Nothing in the shipped library links it, and the GPU path never falls
+ * back to it.
  *
  * Conventions restated from the reference (file:line of euav/focss):
  *   - field layout: mode-major interleaved complex<double>, element (mode,
@@ -131,7 +132,7 @@ void orc_wdmppe_features(int c_radius, int d_radius, int samples, const orc_c128
 int orc_wdmppe_train(int c_radius, int d_radius, int samples, const orc_c128* tx6, const orc_c128* rx6,
                      orc_c128* C, orc_c128* D, orc_c128* intercept, orc_c128* linear_weight);
 
-/* ---- learned few-step DBP (no reference counterpart; judged by plain DBP) ----
+/* ---- learned few-step DBP (no reference counterpart; measured against plain DBP) ----
  * fir [steps][taps] complex, kerr [steps][ktaps] real intensity taps of the
  * nonlinear phase (ktaps = 1: exp(i theta |y|^2)). */
 double orc_ldbp_forward(int steps, int taps, int ktaps, int modes, const orc_c128* fir, const double* kerr,

@@ -1,7 +1,8 @@
 """ctypes view of the CPU FP64 oracle (oracle/focss_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
-by bench.py's cpu_baseline leg, always as the checker.  The product library
+by bench.py's cpu_baseline leg, always as the reference that results are
+checked against.  The product library
 (focss_amd) never imports it.
 
 Arrays are numpy complex128, mode-major: element (mode, sample) at

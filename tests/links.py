@@ -1,5 +1,5 @@
-"""Shared link configurations and seeded waveforms for the tests (the
-checker side: builds inputs with the oracle's generators)."""
+"""Shared link configurations and seeded waveforms for the tests (inputs
+built with the oracle's generators)."""
 import numpy as np
 
 import oracle as O
