@@ -265,6 +265,8 @@ struct focss_ctx {
     int n_cu = 256;          // compute units of the device
     void* ldbp_frag = nullptr;  // learned-DBP inference: matrix-core operands of every step
     size_t ldbp_frag_bytes = 0;
+    double* ldbp_acc = nullptr;  // learned-DBP training: loss and gradient accumulators
+    size_t ldbp_acc_bytes = 0;
     void* dsp_tmp = nullptr;    // transmitter/receiver DSP work space
     size_t dsp_tmp_bytes = 0;
 };
@@ -1207,6 +1209,7 @@ int focss_ctx_destroy(focss_ctx* c) {
         if (c->fine[i]) (void)hipFree(c->fine[i]);
     }
     if (c->ldbp_frag) (void)hipFree(c->ldbp_frag);
+    if (c->ldbp_acc) (void)hipFree(c->ldbp_acc);
     if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
     if (c->d_statics) {
         (void)hipFree(c->d_statics);
@@ -1635,10 +1638,19 @@ int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int 
     if (int rc = ldbp_check(steps, taps, kerr_taps, modes, samples, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     const int np = 2 * steps * taps + steps * kerr_taps;
-    // gradient and loss accumulators live at the head of a small buffer
-    double* acc = nullptr;
-    HIP_TRY(hipMalloc(&acc, (size_t)(np + 1) * sizeof(double)));
-    HIP_TRY(hipMemsetAsync(acc, 0, (size_t)(np + 1) * sizeof(double), c->stream));
+    // loss and gradient accumulators in the context's buffer (grown, with a
+    // synchronisation, when a call needs more)
+    const size_t acc_bytes = (size_t)(np + 1) * sizeof(double);
+    if (c->ldbp_acc_bytes < acc_bytes) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->ldbp_acc) (void)hipFree(c->ldbp_acc);
+        c->ldbp_acc = nullptr;
+        c->ldbp_acc_bytes = 0;
+        HIP_TRY(hipMalloc(&c->ldbp_acc, acc_bytes));
+        c->ldbp_acc_bytes = acc_bytes;
+    }
+    double* acc = c->ldbp_acc;
+    HIP_TRY(hipMemsetAsync(acc, 0, acc_bytes, c->stream));
     int rc = focss_ldbp_gradients(c, steps, taps, kerr_taps, modes, samples, batch, params, in, target, workspace,
                                   workspace_bytes, acc + 1, acc);
     if (rc == 0) {
@@ -1651,7 +1663,6 @@ int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int 
         HIP_TRY(hipMemcpyAsync(loss_host, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
-    (void)hipFree(acc);
     return rc;
 }
 

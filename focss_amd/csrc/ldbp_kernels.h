@@ -54,6 +54,12 @@ __device__ __forceinline__ float sload(const float* p) {
     return *(const __attribute__((address_space(4))) float*)p;
 }
 
+__device__ __forceinline__ double ldbp_wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ double ldbp_block_sum(double v, double* red, int nt) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -205,9 +211,12 @@ template <int K, int KP, int M>
 __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     extern __shared__ unsigned char smem_raw[];
     cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);
-    __shared__ double red[LDBP_NT / 64];
+    // per-wave sums of the step's parameter gradients: Re/Im of the FIR taps,
+    // then the intensity taps; one barrier per step combines them
+    __shared__ double gsum[LDBP_NT / 64][2 * K + KP];
     __shared__ cx<float> h[K];
     constexpr int KC = 8;  // taps per gradient sweep (register budget)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     constexpr int P = KP / 2;
     const int N = a.N, mask = N - 1, c = K / 2;
     const int b = blockIdx.x;
@@ -262,16 +271,14 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
             for (int md = 0; md < M; ++md) g[md * N + t] = g[md * N + t] * mk<float>(co, -s) + y[md * N + t] * dp;
         }
         __syncthreads();
-        double* grk = a.grad + (long long)a.J * K * 2 + (long long)j * KP;
-#pragma unroll 1
+#pragma unroll
         for (int m = 0; m < KP; ++m) {
-            double st = ldbp_block_sum(dg[m], red, LDBP_NT);
-            if (threadIdx.x == 0) atomicAdd(&grk[m], st);
+            const double st = ldbp_wave_sum(dg[m]);
+            if (lane == 0) gsum[wid][2 * K + m] = st;
         }
         // FIR tap gradients G_h_k = sum_modes sum_t G_y(t) conj(x(t - k + c)),
-        // KC taps per sweep; per-thread partial sums in float, block sums in
+        // KC taps per sweep; per-thread partial sums in float, wave sums in
         // double
-        double* gr = a.grad + (long long)j * K * 2;
         for (int k0 = 0; k0 < K; k0 += KC) {
             float dre[KC], dim[KC];
 #pragma unroll
@@ -293,13 +300,26 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                     }
                 }
             }
-#pragma unroll 1
-            for (int q = 0; q < KC && k0 + q < K; ++q) {
-                double sr = ldbp_block_sum((double)dre[q], red, LDBP_NT);
-                if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q)], sr);
-                double si = ldbp_block_sum((double)dim[q], red, LDBP_NT);
-                if (threadIdx.x == 0) atomicAdd(&gr[2 * (k0 + q) + 1], si);
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                if (k0 + q < K) {
+                    const double sr = ldbp_wave_sum((double)dre[q]);
+                    const double si = ldbp_wave_sum((double)dim[q]);
+                    if (lane == 0) {
+                        gsum[wid][2 * (k0 + q)] = sr;
+                        gsum[wid][2 * (k0 + q) + 1] = si;
+                    }
+                }
             }
+        }
+        // the waves' sums are visible; one atomic per parameter of the step
+        __syncthreads();
+        for (int i = threadIdx.x; i < 2 * K + KP; i += LDBP_NT) {
+            double st = 0;
+            for (int w = 0; w < LDBP_NT / 64; ++w) st += gsum[w][i];
+            double* dst = i < 2 * K ? a.grad + (long long)j * K * 2 + i
+                                    : a.grad + (long long)a.J * K * 2 + (long long)j * KP + (i - 2 * K);
+            atomicAdd(dst, st);
         }
         // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c) of every
         // mode, staged in zs
