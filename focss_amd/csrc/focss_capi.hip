@@ -1105,8 +1105,8 @@ int ldbp_check(int steps, int taps, int ktaps, int modes, int samples, int batch
     return 0;
 }
 
-// forward: N / 16 threads, LDS = M (N + N/16) complex64; backward: 1024
-// threads, LDS = M N complex64
+// forward: N / 16 threads; backward: 1024 threads; both hold every mode in
+// LDS in the p16 layout, M (N + N/16) complex64
 int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
     hipLaunchKernelGGL(k, dim3(a.B), dim3(a.N / LDBP_R), (size_t)modes * (a.N + a.N / 16) * sizeof(cx<float>),
                        c->stream, a);
@@ -1115,7 +1115,8 @@ int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
 }
 
 int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
-    hipLaunchKernelGGL(k, dim3(a.B), dim3(LDBP_NT), (size_t)modes * a.N * sizeof(cx<float>), c->stream, a);
+    hipLaunchKernelGGL(k, dim3(a.B), dim3(LDBP_NT), (size_t)modes * (a.N + a.N / 16) * sizeof(cx<float>),
+                       c->stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -205,30 +205,38 @@ __global__ __launch_bounds__(LDBP_MAX_N / LDBP_R) void k_ldbp_fwd(LdbpArgs a) {
 }
 
 // needs the forward pass run with xs / ys stored; out holds z_J.  LDS holds
-// the gradient of every mode [M][N]; zs ([B][M][N]) stages the per-sample q
-// and phi (first N) and the input gradient.
+// the gradient of every mode, p16 layout with mode stride N + N / 16; zs
+// ([B][M][N]) stages the per-sample q and phi (first N).  The two FIR-shaped
+// contractions -- the tap gradients and the input gradient -- run on work
+// units of 16 consecutive samples of one mode (M N / 16 <= 1024 units, at
+// most one per thread) with register windows, as the forward FIR does.
 template <int K, int KP, int M>
 __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
     extern __shared__ unsigned char smem_raw[];
-    cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);
+    cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);  // [M][p16(N)]
     // per-wave sums of the step's parameter gradients: Re/Im of the FIR taps,
     // then the intensity taps; one barrier per step combines them
     __shared__ double gsum[LDBP_NT / 64][2 * K + KP];
-    __shared__ cx<float> h[K];
-    constexpr int KC = 8;  // taps per gradient sweep (register budget)
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int R = LDBP_R, KC = 4;  // taps per register window (register budget at 1024 threads)
     constexpr int P = KP / 2;
-    const int N = a.N, mask = N - 1, c = K / 2;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int N = a.N, mask = N - 1, c = K / 2, S = N + N / 16, lg = __builtin_ctz(N);
     const int b = blockIdx.x;
     const float scale = 1.0f / ((float)N * (float)M * (float)a.B);
     const long long wb = (long long)b * M * N;  // this waveform's element offset
     cx<float>* zs = a.zs + wb;
-    for (int t = threadIdx.x; t < M * N; t += LDBP_NT) g[t] = (a.out[wb + t] - a.target[wb + t]) * scale;
+    // this thread's work unit: mode um, samples u0 .. u0 + 15
+    const int units = M * N / R;
+    const bool has_unit = threadIdx.x < units;
+    const int um = (threadIdx.x * R) >> lg, u0 = (threadIdx.x * R) & mask;
+    for (int i = threadIdx.x; i < M * N; i += LDBP_NT)
+        g[(i >> lg) * S + p16(i & mask)] = (a.out[wb + i] - a.target[wb + i]) * scale;
     __syncthreads();
     for (int j = a.J - 1; j >= 0; --j) {
         const cx<float>* x = a.xs + ((long long)b * a.J + j) * M * N;  // [M][N]
         const cx<float>* y = a.ys + ((long long)b * a.J + j) * M * N;
         const float* gk = a.kerr + j * KP;
+        const float* hj = reinterpret_cast<const float*>(a.fir + (long long)j * K);
         // nonlinear phase backward, pass 1: q and phi per sample (staged in
         // zs), intensity-tap gradients
         double dg[KP];
@@ -251,12 +259,17 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
 #pragma unroll
             for (int md = 0; md < M; ++md) {
                 const cx<float> zv = y[md * N + t] * mk<float>(co, s);
-                const cx<float> gz = g[md * N + t];
+                const cx<float> gz = g[md * S + p16(t)];
                 q -= gz.x * zv.y - gz.y * zv.x;
             }
 #pragma unroll
             for (int m = 0; m < KP; ++m) dg[m] += 2.0 * (double)q * (double)pn[m];
             zs[t] = mk<float>(q, phi);
+        }
+#pragma unroll
+        for (int m = 0; m < KP; ++m) {
+            const double st = ldbp_wave_sum(dg[m]);
+            if (lane == 0) gsum[wid][2 * K + m] = st;
         }
         __syncthreads();
         // pass 2: G_y = G_z e^{-i phi} + (dL/dp) y, in place
@@ -268,17 +281,22 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
             float s, co;
             sincos_fast(phi, &s, &co);
 #pragma unroll
-            for (int md = 0; md < M; ++md) g[md * N + t] = g[md * N + t] * mk<float>(co, -s) + y[md * N + t] * dp;
+            for (int md = 0; md < M; ++md) {
+                cx<float>& gv = g[md * S + p16(t)];
+                gv = gv * mk<float>(co, -s) + y[md * N + t] * dp;
+            }
         }
         __syncthreads();
-#pragma unroll
-        for (int m = 0; m < KP; ++m) {
-            const double st = ldbp_wave_sum(dg[m]);
-            if (lane == 0) gsum[wid][2 * K + m] = st;
-        }
-        // FIR tap gradients G_h_k = sum_modes sum_t G_y(t) conj(x(t - k + c)),
-        // KC taps per sweep; per-thread partial sums in float, wave sums in
+        // FIR tap gradients G_h_k = sum_modes sum_t G_y(t) conj(x(t - k + c)):
+        // KC taps per sweep, the unit's 16 G_y values and a window of
+        // 16 + KC - 1 inputs in registers; partial sums in float, wave sums in
         // double
+        cx<float> gy[R];
+        if (has_unit) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) gy[r] = g[um * S + p16(u0 + r)];
+        }
+#pragma unroll 1
         for (int k0 = 0; k0 < K; k0 += KC) {
             float dre[KC], dim[KC];
 #pragma unroll
@@ -286,17 +304,18 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                 dre[q] = 0;
                 dim[q] = 0;
             }
+            if (has_unit) {
+                cx<float> w[R + KC - 1];  // x(u0 + c - k0 - (KC - 1) + v)
+                const int base = u0 + c - k0 - (KC - 1);
 #pragma unroll
-            for (int md = 0; md < M; ++md) {
-                for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-                    cx<float> gy = g[md * N + t];
+                for (int v = 0; v < R + KC - 1; ++v) w[v] = x[um * N + ((base + v) & mask)];
 #pragma unroll
-                    for (int q = 0; q < KC; ++q) {
-                        if (k0 + q < K) {
-                            cx<float> v = gy * conj(x[md * N + ((t - (k0 + q) + c) & mask)]);
-                            dre[q] += v.x;
-                            dim[q] += v.y;
-                        }
+                for (int q = 0; q < KC; ++q) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const cx<float> xv = w[r - q + KC - 1];  // x(u0 + r - (k0 + q) + c)
+                        dre[q] = fmaf(gy[r].x, xv.x, fmaf(gy[r].y, xv.y, dre[q]));
+                        dim[q] = fmaf(gy[r].y, xv.x, fmaf(-gy[r].x, xv.y, dim[q]));
                     }
                 }
             }
@@ -312,7 +331,35 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                 }
             }
         }
-        // the waves' sums are visible; one atomic per parameter of the step
+        // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c) of the unit's
+        // 16 samples, KC taps per window of 16 + KC - 1 values of G_y
+        cx<float> gx[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) gx[r] = mk<float>(0, 0);
+        if (has_unit) {
+            const cx<float>* gm = g + um * S;
+#pragma unroll 1
+            for (int k0 = 0; k0 < K; k0 += KC) {
+                cx<float> w[R + KC - 1];  // G_y(u0 + k0 - c + v)
+                const int base = u0 + k0 - c;
+#pragma unroll
+                for (int v = 0; v < R + KC - 1; ++v) w[v] = gm[p16((base + v) & mask)];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const bool live = k0 + q < K;
+                    const float hr = live ? sload(hj + 2 * (k0 + q)) : 0.0f;
+                    const float hi = live ? sload(hj + 2 * (k0 + q) + 1) : 0.0f;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const cx<float> gv = w[r + q];  // conj(h) * G_y
+                        gx[r].x = fmaf(hr, gv.x, fmaf(hi, gv.y, gx[r].x));
+                        gx[r].y = fmaf(hr, gv.y, fmaf(-hi, gv.x, gx[r].y));
+                    }
+                }
+            }
+        }
+        // every read of G_y (and of the waves' sums) is done; one atomic per
+        // parameter of the step, and G_x replaces G_y
         __syncthreads();
         for (int i = threadIdx.x; i < 2 * K + KP; i += LDBP_NT) {
             double st = 0;
@@ -321,22 +368,10 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                                     : a.grad + (long long)a.J * K * 2 + (long long)j * KP + (i - 2 * K);
             atomicAdd(dst, st);
         }
-        // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c) of every
-        // mode, staged in zs
-        if (threadIdx.x < K) h[threadIdx.x] = conj(a.fir[j * K + threadIdx.x]);
-        __syncthreads();
+        if (has_unit) {
 #pragma unroll
-        for (int md = 0; md < M; ++md) {
-            const cx<float>* gm = g + md * N;
-            for (int s = threadIdx.x; s < N; s += LDBP_NT) {
-                cx<float> acc = mk<float>(0, 0);
-#pragma unroll 16
-                for (int k = 0; k < K; ++k) acc = acc + h[k] * gm[(s + k - c) & mask];
-                zs[md * N + s] = acc;
-            }
+            for (int r = 0; r < R; ++r) g[um * S + p16(u0 + r)] = gx[r];
         }
-        __syncthreads();
-        for (int s = threadIdx.x; s < M * N; s += LDBP_NT) g[s] = zs[s];
         __syncthreads();
     }
 }
