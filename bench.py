@@ -4,7 +4,7 @@
 20 x 80 km SSMF with the ADAPTIVE grid (max nonlinear phase 1e-3 rad,
 max step 1 km), EDFA gain 16 dB and seeded ASE at NF 5 dB, on one MI355X.
 
-One bench step = the whole 20-span link for the batch (default 32 independent
+One bench step = the whole 20-span link for the batch (default 64 independent
 realisations).
 Metric: GSample-steps/s = sum over waveforms of samples x split steps / s.
 Prints one JSON line.  FP64 (complex128) is the headline; the FP32 path
@@ -176,7 +176,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--precision", choices=["f64", "f32", "both"], default="both")
     ap.add_argument("--cpu-km", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")

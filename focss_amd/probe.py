@@ -14,7 +14,7 @@ launches each --
     samples, both directions, FP64;
   WdmPPE training, c = 30, d = 2, 2^14 samples (features, Gram, sums).
 Usage:
-    python -m focss_amd.probe [--batch 32] [--pairs 12] [--precision f64|f32] [--variant V]
+    python -m focss_amd.probe [--batch 64] [--pairs 12] [--precision f64|f32] [--variant V]
     python -m focss_amd.probe --workload side [--reps 5]
 """
 import argparse
@@ -28,7 +28,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--pairs", type=int, default=12)
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
