@@ -193,3 +193,23 @@ def test_plain_ssfm_c1_config(ctx):
     assert steps == ref_steps == 800
     assert err <= TOL64
     assert err32 <= TOL32
+
+
+@pytest.mark.parametrize("modes,grid,length", [(1, (0, 3, 0.0, 0.0), 80e3), (2, (2, 0, 1e-2, 1e3), 1.5e3)],
+                         ids=["uniform_1mode", "adaptive_manakov"])
+def test_ssfm_parity_max_size(ctx, modes, grid, length):
+    """The largest field a link program takes, 2^20 samples per mode (the
+    two-pass kernels at N1 = N2 = 1024), against the oracle at 1e-10: three
+    UNIFORM steps over one span, or the ADAPTIVE grid over 1.5 km at a high
+    launch power (Manakov, kappa 8/9)."""
+    import focss_amd as F
+    n = 1 << 20
+    fib = ssmf(length=length, kappa=8.0 / 9.0 if modes == 2 else 1.0)
+    x, _, dt = qam_waveform(77 + modes, n, sps=8, power_dbm=8.0, modes=modes)
+    ref, ref_steps = _oracle_link(fib, grid, 1.0, 1, x, dt, modes, 1.0, 1.0, 0.0, 0)
+    link = _link(F, fib, grid, 1.0, 1, n, dt, modes=modes)
+    y, steps = ctx.link_run_host(link, x, precision=F.F64)
+    err = rel_l2(y, ref)
+    print(f"2^20 x {modes} modes: {steps} steps (oracle {ref_steps}), rel L2 {err:.3e}")
+    assert steps == ref_steps
+    assert err <= TOL64
