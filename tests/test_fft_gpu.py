@@ -62,3 +62,25 @@ def test_add_noise_matches_oracle(ctx):
     ref = O.amplify(x, gain, nf, fc, dt, 77, 3, modes=modes)  # zero field: noise only
     assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
     buf.free()
+
+
+@pytest.mark.parametrize("logn", [4, 7, 10])
+@pytest.mark.parametrize("modes", [1, 2])
+def test_fft_against_pocketfft_fixture(ctx, logn, modes):
+    """The device DFT against the committed numpy.fft (pocketfft FP64)
+    fixtures with Field::fft_inplace's normalisation: FP64 to 1e-14, FP32 to
+    1e-6 relative L2."""
+    import os
+    import focss_amd as F
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_pocketfft.npz"))
+    x = z[f"x_{logn}_{modes}"].ravel()
+    n = 1 << logn
+    for prec, dt, tol in ((F.F64, np.complex128, 1e-14), (F.F32, np.complex64, 1e-6)):
+        buf = ctx.alloc(x.size * np.dtype(dt).itemsize)
+        for sign, ref in ((-1, z[f"fwd_{logn}_{modes}"]), (+1, z[f"inv_{logn}_{modes}"])):
+            buf.upload(x.astype(dt))
+            ctx.fft(buf, modes, n, sign, precision=prec)
+            y = buf.download(np.empty(x.size, dtype=dt)).astype(np.complex128)
+            err = np.linalg.norm(y - ref.ravel()) / np.linalg.norm(ref)
+            assert err < tol, (prec, sign, err)
+        buf.free()
