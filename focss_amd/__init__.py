@@ -88,6 +88,12 @@ _sig = {
                                        _P, C.c_size_t, _P, _P]),
     "focss_ldbp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P,
                                         C.c_int, C.c_float, _P, _P, _P, _P, C.c_size_t, _P]),
+    "focss_tmlp_param_count": (C.c_int, []),
+    "focss_tmlp_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "focss_tmlp_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "focss_tmlp_gradients": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, C.c_size_t, _P, _P]),
+    "focss_tmlp_train_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int, C.c_float, _P, _P,
+                                        _P, C.c_size_t, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -407,6 +413,88 @@ class LearnedDBP:
                                           self.batch, self.params.ptr, self.m.ptr, self.v.ptr, self.t, lr,
                                           self.lr_scale.ptr if self.lr_scale else None, inp.ptr, target.ptr,
                                           self.ws.ptr, self.ws_bytes, C.byref(loss)))
+        return loss.value
+
+
+class TripletMLP:
+    """Device-resident triplet MLP post-equaliser over symbols: 1089
+    perturbation triplets x_md(s+m) sum_p x_p(s+n) conj(x_p(s+m+n)),
+    |m|, |n| <= 16, as 2178 real features -> 128 ReLU units -> a complex
+    correction added to the symbol.  BF16 matrix-core products with FP32
+    accumulation for the 2240-wide layer and its weight gradient.  Symbols
+    are complex64 [batch][modes][symbols]; symbols a multiple of 128."""
+
+    HIDDEN, FEATURES, TRIPLETS, RADIUS = 128, 2240, 1089, 16
+
+    def __init__(self, ctx, modes, symbols, batch, params=None, seed=0, w2_scale=1e-2):
+        self.ctx, self.modes, self.symbols, self.batch = ctx, modes, symbols, batch
+        self.n_params = _lib.focss_tmlp_param_count()
+        host = self.init_params(seed, w2_scale) if params is None else np.ascontiguousarray(params, np.float32)
+        if host.size != self.n_params:
+            raise FocssError("triplet MLP: %d parameters expected" % self.n_params)
+        self.params = ctx.alloc(host.nbytes)
+        self.params.upload(host)
+        self.m = ctx.alloc(host.nbytes)
+        self.v = ctx.alloc(host.nbytes)
+        zeros = np.zeros_like(host)
+        self.m.upload(zeros)
+        self.v.upload(zeros)
+        self.t = 0
+        self.ws_bytes = _lib.focss_tmlp_workspace_bytes(modes, symbols, batch)
+        self.ws = None
+        self.grad = None
+
+    @classmethod
+    def init_params(cls, seed=0, w2_scale=1e-2):
+        """He-scaled W1 on the 2178 used features (zero padding), b1 = 0,
+        small W2, b2 = 0: the network starts close to the identity."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        h, kf, used = cls.HIDDEN, cls.FEATURES, 2 * cls.TRIPLETS
+        w1 = np.zeros((h, kf), dtype=np.float32)
+        w1[:, :used] = rng.normal(0.0, np.sqrt(2.0 / used), (h, used))
+        w2 = rng.normal(0.0, w2_scale, (2, h))
+        return np.concatenate([w1.ravel(), np.zeros(h), w2.ravel(), np.zeros(2)]).astype(np.float32)
+
+    def _training_buffers(self):
+        if self.ws is None:
+            self.ws = self.ctx.alloc(self.ws_bytes)
+            self.grad = self.ctx.alloc(8 * (self.n_params + 1))
+
+    def free(self):
+        for b in (self.params, self.m, self.v, self.ws, self.grad):
+            if b is not None:
+                b.free()
+
+    def get(self):
+        return self.params.download(np.empty(self.n_params, dtype=np.float32))
+
+    def forward(self, inp, out, target=None, batch=None):
+        """out = the equalised symbols; returns the loss against target (the
+        call then synchronises) or None."""
+        loss = C.c_double(0)
+        _check(_lib.focss_tmlp_forward(self.ctx.handle, self.modes, self.symbols, batch or self.batch,
+                                       self.params.ptr, inp.ptr, out.ptr, target.ptr if target else None,
+                                       C.byref(loss) if target else None))
+        return loss.value if target else None
+
+    def gradients(self, inp, target):
+        """(loss, dL/dparams as float64 [param_count])."""
+        self._training_buffers()
+        self.grad.upload(np.zeros(self.n_params + 1, dtype=np.float64))
+        gp = self.grad.ptr.value
+        _check(_lib.focss_tmlp_gradients(self.ctx.handle, self.modes, self.symbols, self.batch, self.params.ptr,
+                                         inp.ptr, target.ptr, self.ws.ptr, self.ws_bytes, C.c_void_p(gp + 8),
+                                         C.c_void_p(gp)))
+        g = self.grad.download(np.empty(self.n_params + 1, dtype=np.float64))
+        return g[0], g[1:]
+
+    def train_step(self, inp, target, lr):
+        self._training_buffers()
+        self.t += 1
+        loss = C.c_double(0)
+        _check(_lib.focss_tmlp_train_step(self.ctx.handle, self.modes, self.symbols, self.batch, self.params.ptr,
+                                          self.m.ptr, self.v.ptr, self.t, lr, inp.ptr, target.ptr, self.ws.ptr,
+                                          self.ws_bytes, C.byref(loss)))
         return loss.value
 
 

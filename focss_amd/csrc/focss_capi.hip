@@ -17,6 +17,7 @@
 #include "fft_kernels.h"
 #include "triplet_kernels.h"
 #include "ldbp_kernels.h"
+#include "tmlp_kernels.h"
 #include "dsp_kernels.h"
 
 using namespace focss_hip;
@@ -267,6 +268,8 @@ struct focss_ctx {
     size_t ldbp_frag_bytes = 0;
     double* ldbp_acc = nullptr;  // learned-DBP training: loss and gradient accumulators
     size_t ldbp_acc_bytes = 0;
+    unsigned short* tmlp_w1b = nullptr;  // triplet MLP: bf16 copy of W1 [128][2240]
+    double* tmlp_acc = nullptr;          // triplet MLP: loss, then the gradient [TM_NP]
     void* dsp_tmp = nullptr;    // transmitter/receiver DSP work space
     size_t dsp_tmp_bytes = 0;
 };
@@ -1211,6 +1214,8 @@ int focss_ctx_destroy(focss_ctx* c) {
     }
     if (c->ldbp_frag) (void)hipFree(c->ldbp_frag);
     if (c->ldbp_acc) (void)hipFree(c->ldbp_acc);
+    if (c->tmlp_w1b) (void)hipFree(c->tmlp_w1b);
+    if (c->tmlp_acc) (void)hipFree(c->tmlp_acc);
     if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
     if (c->d_statics) {
         (void)hipFree(c->d_statics);
@@ -1663,6 +1668,145 @@ int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int 
     if (rc == 0 && loss_host) {
         HIP_TRY(hipMemcpyAsync(loss_host, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+// ---- triplet MLP post-equaliser ----
+
+namespace {
+
+int tmlp_check(int modes, int symbols, int batch) {
+    if (modes != 1 && modes != 2) return fail(FOCSS_E_ARG, "triplet MLP: modes must be 1 or 2");
+    if (symbols < TM_ROWS || symbols % TM_ROWS != 0 || symbols > (1 << 20))
+        return fail(FOCSS_E_ARG, "triplet MLP: symbols must be a multiple of 128 in [128, 2^20]");
+    if (batch < 1 || (long long)batch * modes * symbols > (1LL << 27))
+        return fail(FOCSS_E_ARG, "triplet MLP: batch must be >= 1 with batch * modes * symbols <= 2^27");
+    return 0;
+}
+
+int tmlp_splits(int blocks) { return std::min(TM_SPLITS, blocks); }
+
+// workspace: hidden-layer gradient [128][rows] bf16, per-block sums
+// [TM_SMALL][blocks] float, weight-gradient partials [splits][128][2240]
+// float; each part starts on a 256-byte boundary
+size_t tmlp_align(size_t b) { return (b + 255) & ~(size_t)255; }
+size_t tmlp_small_offset(size_t rows) { return tmlp_align(rows * TM_H * sizeof(unsigned short)); }
+size_t tmlp_part_offset(size_t rows) {
+    return tmlp_small_offset(rows) + tmlp_align((size_t)TM_SMALL * (rows / TM_ROWS) * sizeof(float));
+}
+
+// the bf16 copy of W1 in the context (fixed size, allocated on first use)
+int tmlp_pack(focss_ctx* c, const float* params) {
+    if (!c->tmlp_w1b) HIP_TRY(hipMalloc(&c->tmlp_w1b, (size_t)TM_H * TM_KF * sizeof(unsigned short)));
+    const int n = TM_H * TM_KF;
+    hipLaunchKernelGGL(k_tmlp_pack, dim3((n + 255) / 256), dim3(256), 0, c->stream, params, c->tmlp_w1b, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+TmlpArgs tmlp_args(focss_ctx* c, int modes, int symbols, int batch, const float* params, const void* in) {
+    TmlpArgs a = {};
+    a.x = (const cx<float>*)in;
+    a.w1b = c->tmlp_w1b;
+    a.b1 = params + TM_H * TM_KF;
+    a.w2 = a.b1 + TM_H;
+    a.b2 = a.w2 + 2 * TM_H;
+    a.S = symbols;
+    a.rows = batch * modes * symbols;
+    a.blocks = a.rows / TM_ROWS;
+    a.inv_count = 1.0f / (float)a.rows;
+    return a;
+}
+
+}  // namespace
+
+int focss_tmlp_param_count(void) { return TM_NP; }
+
+size_t focss_tmlp_workspace_bytes(int modes, int symbols, int batch) {
+    if (modes < 1 || symbols < TM_ROWS || batch < 1) return 0;
+    const size_t rows = (size_t)batch * modes * symbols, blocks = rows / TM_ROWS;
+    return tmlp_part_offset(rows) + (size_t)tmlp_splits((int)blocks) * TM_H * TM_KF * sizeof(float);
+}
+
+int focss_tmlp_forward(focss_ctx* c, int modes, int symbols, int batch, const float* params, const void* in,
+                       void* out, const void* target, double* loss) {
+    if (!c || !params || !in || !out || (target && !loss)) return fail(FOCSS_E_ARG, "triplet MLP: null argument");
+    if (in == out) return fail(FOCSS_E_ARG, "triplet MLP: in and out must not alias");
+    if (int rc = tmlp_check(modes, symbols, batch)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = tmlp_pack(c, params)) return rc;
+    TmlpArgs a = tmlp_args(c, modes, symbols, batch, params, in);
+    a.out = (cx<float>*)out;
+    if (modes == 1)
+        hipLaunchKernelGGL((k_tmlp_fwd<1, false>), dim3(a.blocks), dim3(TM_NT), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_tmlp_fwd<2, false>), dim3(a.blocks), dim3(TM_NT), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    if (!target) return 0;
+    if (!c->tmlp_acc) HIP_TRY(hipMalloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(c->tmlp_acc, 0, sizeof(double), c->stream));
+    const long long n = (long long)a.rows;
+    const int grid = (int)std::min<long long>((n + TM_NT - 1) / TM_NT, 4096);
+    hipLaunchKernelGGL(k_tmlp_loss, dim3(grid), dim3(TM_NT), 0, c->stream, (const cx<float>*)out,
+                       (const cx<float>*)target, n, 1.0 / (double)n, c->tmlp_acc);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(loss, c->tmlp_acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int focss_tmlp_gradients(focss_ctx* c, int modes, int symbols, int batch, const float* params, const void* in,
+                         const void* target, void* workspace, size_t workspace_bytes, double* grad, double* loss) {
+    if (!c || !params || !in || !target || !workspace || !grad)
+        return fail(FOCSS_E_ARG, "triplet MLP: null argument");
+    if (int rc = tmlp_check(modes, symbols, batch)) return rc;
+    if (workspace_bytes < focss_tmlp_workspace_bytes(modes, symbols, batch))
+        return fail(FOCSS_E_ARG, "triplet MLP: workspace too small");
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = tmlp_pack(c, params)) return rc;
+    TmlpArgs a = tmlp_args(c, modes, symbols, batch, params, in);
+    a.target = (const cx<float>*)target;
+    a.dpre = (unsigned short*)workspace;
+    a.small = (float*)((char*)workspace + tmlp_small_offset((size_t)a.rows));
+    a.part = (float*)((char*)workspace + tmlp_part_offset((size_t)a.rows));
+    const int splits = tmlp_splits(a.blocks);
+    const dim3 g_w1(TM_TP / 32 / 4 + 1, splits);  // 9 x 4 waves cover the 35 triplet blocks
+    if (modes == 1) {
+        hipLaunchKernelGGL((k_tmlp_fwd<1, true>), dim3(a.blocks), dim3(TM_NT), 0, c->stream, a);
+        hipLaunchKernelGGL(k_tmlp_dw1<1>, g_w1, dim3(TM_NT), 0, c->stream, a, splits);
+    } else {
+        hipLaunchKernelGGL((k_tmlp_fwd<2, true>), dim3(a.blocks), dim3(TM_NT), 0, c->stream, a);
+        hipLaunchKernelGGL(k_tmlp_dw1<2>, g_w1, dim3(TM_NT), 0, c->stream, a, splits);
+    }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_tmlp_reduce_w1, dim3((TM_H * TM_KF + 255) / 256), dim3(256), 0, c->stream,
+                       (const float*)a.part, splits, grad);
+    hipLaunchKernelGGL(k_tmlp_reduce_small, dim3(TM_SMALL), dim3(TM_NT), 0, c->stream, (const float*)a.small,
+                       a.blocks, grad, loss);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int focss_tmlp_train_step(focss_ctx* c, int modes, int symbols, int batch, float* params, float* adam_m,
+                          float* adam_v, int t, float lr, const void* in, const void* target, void* workspace,
+                          size_t workspace_bytes, double* loss_host) {
+    if (!c || !params || !adam_m || !adam_v || t < 1) return fail(FOCSS_E_ARG, "triplet MLP: bad argument");
+    if (int rc = tmlp_check(modes, symbols, batch)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->tmlp_acc) HIP_TRY(hipMalloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));
+    double* acc = c->tmlp_acc;
+    HIP_TRY(hipMemsetAsync(acc, 0, (size_t)(TM_NP + 1) * sizeof(double), c->stream));
+    int rc = focss_tmlp_gradients(c, modes, symbols, batch, params, in, target, workspace, workspace_bytes, acc + 1,
+                                  acc);
+    if (rc == 0) {
+        hipLaunchKernelGGL(k_adam, dim3((TM_NP + 255) / 256), dim3(256), 0, c->stream, params,
+                           (const double*)(acc + 1), adam_m, adam_v, TM_NP, lr, 0.9f, 0.999f, 1e-8f, t,
+                           (const float*)nullptr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(FOCSS_E_HIP, std::string("adam: ") + hipGetErrorString(e));
+    }
+    if (rc == 0 && loss_host) HIP_TRY(hipMemcpyAsync(loss_host, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return rc;
 }

@@ -272,6 +272,43 @@ int focss_ldbp_train_step(focss_ctx* ctx, int steps, int taps, int kerr_taps, in
                           const void* in, const void* target, void* workspace, size_t workspace_bytes,
                           double* loss_host);
 
+/* ---- triplet MLP post-equaliser (no counterpart in the reference) ----
+ * Widens the perturbation-triplet equaliser (src/focss/equalizer/sino.cpp,
+ * term at sino.cpp:154-168) into a two-layer network over symbols.  For
+ * symbol s of mode md (symbols periodic, indices mod `symbols`):
+ *   T_t(s) = x_md(s+m) sum_p x_p(s+n) conj(x_p(s+m+n)),  t = (m+16) 33 + n+16,
+ *   |m|, |n| <= 16 (1089 triplets); f = (Re T_0, Im T_0, Re T_1, ...) padded
+ *   with zeros to FOCSS_TMLP_FEATURES;
+ *   y = x + (o_0 + i o_1),  o = W2 max(W1 f + b1, 0) + b2.
+ * params (device, float, focss_tmlp_param_count() values):
+ *   W1 [128][FOCSS_TMLP_FEATURES] (the padding columns stay zero under
+ *   training), b1 [128], W2 [2][128], b2 [2].
+ * Symbols are complex64 device buffers [batch][modes][symbols], modes 1 or
+ * 2, symbols a multiple of 128 in [128, 2^20], batch modes symbols <= 2^27.
+ * The two long products (W1 f and its weight gradient) run on the matrix
+ * cores with BF16 operands (features, W1, hidden-layer gradient) and FP32
+ * accumulation.  Loss = (1/(batch modes symbols)) sum |y - target|^2.
+ * Callers: post-DBP equalisation and its training. */
+#define FOCSS_TMLP_HIDDEN 128
+#define FOCSS_TMLP_FEATURES 2240
+int focss_tmlp_param_count(void);
+size_t focss_tmlp_workspace_bytes(int modes, int symbols, int batch);
+/* Inference into out (must not alias in; stream-ordered), and the loss
+ * against target when target is given (*loss, host double; the call then
+ * synchronises). */
+int focss_tmlp_forward(focss_ctx* ctx, int modes, int symbols, int batch, const float* params, const void* in,
+                       void* out, const void* target, double* loss);
+/* Accumulates into grad (device double [param_count], zeroed by the caller)
+ * dL/dparams and into *loss (device double, may be null) the loss.
+ * Stream-ordered, no host sync. */
+int focss_tmlp_gradients(focss_ctx* ctx, int modes, int symbols, int batch, const float* params, const void* in,
+                         const void* target, void* workspace, size_t workspace_bytes, double* grad, double* loss);
+/* One training step: gradients then Adam (t = step number from 1);
+ * *loss_host (may be null) receives the batch loss before the update. */
+int focss_tmlp_train_step(focss_ctx* ctx, int modes, int symbols, int batch, float* params, float* adam_m,
+                          float* adam_v, int t, float lr, const void* in, const void* target, void* workspace,
+                          size_t workspace_bytes, double* loss_host);
+
 /* Tuning: choose the tile shape of the two-pass kernels for one length,
  * for both passes (focss_set_tile) or for one (pass 0 = time-domain pass,
  * 1 = spectral pass).  Exposed for the autotuner; no counterpart in the

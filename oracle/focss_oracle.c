@@ -1226,3 +1226,125 @@ double orc_ldbp_backward(int steps, int taps, int ktaps, int modes, const orc_c1
     free(xs);
     return loss / mn;
 }
+
+/* ---- triplet MLP post-equaliser (no reference counterpart; the triplet is
+   the term of sino.cpp:154-168 with periodic indices) ---- */
+
+enum { TM_R = 16, TM_W = 33, TM_T = 1089, TM_H = 128, TM_KF = 2240 };
+
+int orc_tmlp_n_params(void) { return TM_H * TM_KF + TM_H + 2 * TM_H + 2; }
+
+/* double -> float -> bfloat16 (round to nearest even), as a double */
+static double tm_bf16(double v) {
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+/* features of symbol s of mode md of one waveform x [modes][S] */
+static void tm_features(int modes, int S, const orc_c128* x, int md, int s, int bf16, double* f) {
+    for (int t = 0; t < TM_T; ++t) {
+        int m = t / TM_W - TM_R, n = t % TM_W - TM_R;
+        orc_c128 acc = {0, 0};
+        for (int p = 0; p < modes; ++p)
+            acc = c_add(acc, c_mul(x[(long)p * S + wrap(s + n, S)], c_conj(x[(long)p * S + wrap(s + m + n, S)])));
+        orc_c128 T = c_mul(x[(long)md * S + wrap(s + m, S)], acc);
+        f[2 * t] = bf16 ? tm_bf16(T.re) : T.re;
+        f[2 * t + 1] = bf16 ? tm_bf16(T.im) : T.im;
+    }
+}
+
+/* one row: pre-activations, hidden, output correction */
+static void tm_row(const double* w1, const double* b1, const double* w2, const double* b2, const double* f,
+                   double* pre, double* h, double* o) {
+    o[0] = b2[0];
+    o[1] = b2[1];
+    for (int j = 0; j < TM_H; ++j) {
+        double a = 0;
+        const double* wj = w1 + (long)j * TM_KF;
+        for (int k = 0; k < 2 * TM_T; ++k) a += wj[k] * f[k];
+        pre[j] = a + b1[j];
+        h[j] = pre[j] > 0 ? pre[j] : 0;
+        o[0] += w2[j] * h[j];
+        o[1] += w2[TM_H + j] * h[j];
+    }
+}
+
+static double* tm_w1(const double* params, int bf16) {
+    long n = (long)TM_H * TM_KF;
+    double* w = (double*)malloc(sizeof(double) * n);
+    for (long i = 0; i < n; ++i) w[i] = bf16 ? tm_bf16(params[i]) : params[i];
+    return w;
+}
+
+double orc_tmlp_forward(int modes, int symbols, int batch, const double* params, int bf16,
+                        const orc_c128* in, orc_c128* out, const orc_c128* target) {
+    const double* b1 = params + (long)TM_H * TM_KF;
+    const double* w2 = b1 + TM_H;
+    const double* b2 = w2 + 2 * TM_H;
+    double* w1 = tm_w1(params, bf16);
+    double f[2 * TM_T], pre[TM_H], h[TM_H], o[2];
+    double loss = 0;
+    long rows = (long)batch * modes * symbols;
+    for (int b = 0; b < batch; ++b) {
+        const orc_c128* x = in + (long)b * modes * symbols;
+        for (int md = 0; md < modes; ++md)
+            for (int s = 0; s < symbols; ++s) {
+                long r = ((long)b * modes + md) * symbols + s;
+                tm_features(modes, symbols, x, md, s, bf16, f);
+                tm_row(w1, b1, w2, b2, f, pre, h, o);
+                orc_c128 y = {in[r].re + o[0], in[r].im + o[1]};
+                if (out) out[r] = y;
+                if (target) {
+                    double e0 = y.re - target[r].re, e1 = y.im - target[r].im;
+                    loss += e0 * e0 + e1 * e1;
+                }
+            }
+    }
+    free(w1);
+    return loss / rows;
+}
+
+double orc_tmlp_backward(int modes, int symbols, int batch, const double* params, int bf16,
+                         const orc_c128* in, const orc_c128* target, double* grad) {
+    const double* b1 = params + (long)TM_H * TM_KF;
+    const double* w2 = b1 + TM_H;
+    const double* b2 = w2 + 2 * TM_H;
+    double* gw1 = grad;
+    double* gb1 = gw1 + (long)TM_H * TM_KF;
+    double* gw2 = gb1 + TM_H;
+    double* gb2 = gw2 + 2 * TM_H;
+    memset(grad, 0, sizeof(double) * orc_tmlp_n_params());
+    double* w1 = tm_w1(params, bf16);
+    double f[2 * TM_T], pre[TM_H], h[TM_H], o[2];
+    double loss = 0;
+    long rows = (long)batch * modes * symbols;
+    for (int b = 0; b < batch; ++b) {
+        const orc_c128* x = in + (long)b * modes * symbols;
+        for (int md = 0; md < modes; ++md)
+            for (int s = 0; s < symbols; ++s) {
+                long r = ((long)b * modes + md) * symbols + s;
+                tm_features(modes, symbols, x, md, s, bf16, f);
+                tm_row(w1, b1, w2, b2, f, pre, h, o);
+                double e0 = in[r].re + o[0] - target[r].re, e1 = in[r].im + o[1] - target[r].im;
+                loss += e0 * e0 + e1 * e1;
+                double d0 = 2 * e0 / rows, d1 = 2 * e1 / rows;
+                gb2[0] += d0;
+                gb2[1] += d1;
+                for (int j = 0; j < TM_H; ++j) {
+                    gw2[j] += d0 * h[j];
+                    gw2[TM_H + j] += d1 * h[j];
+                    double dp = pre[j] > 0 ? w2[j] * d0 + w2[TM_H + j] * d1 : 0;
+                    gb1[j] += dp;
+                    if (bf16) dp = tm_bf16(dp);
+                    double* gj = gw1 + (long)j * TM_KF;
+                    for (int k = 0; k < 2 * TM_T; ++k) gj[k] += dp * f[k];
+                }
+            }
+    }
+    free(w1);
+    return loss / rows;
+}

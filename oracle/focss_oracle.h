@@ -142,6 +142,21 @@ double orc_ldbp_backward(int steps, int taps, int ktaps, int modes, const orc_c1
                          int samples, const orc_c128* in, const orc_c128* target,
                          orc_c128* grad_fir, double* grad_kerr);
 
+/* ---- triplet MLP post-equaliser (no reference counterpart; its triplet is
+ * the term of src/focss/equalizer/sino.cpp:154-168, with periodic indices,
+ * over the full square |m|, |n| <= 16) ----
+ * params: W1 [128][2240] (features 2t, 2t+1 = Re, Im of triplet
+ * t = (m+16) 33 + n+16; columns from 2178 on unused), b1 [128], W2 [2][128],
+ * b2 [2].  Symbols [batch][modes][symbols].  bf16 != 0 rounds the features
+ * and W1 (and, in the backward, the hidden-layer gradient before the W1
+ * product) to bfloat16, round to nearest even, as the matrix-core kernels
+ * do; everything else is FP64. */
+int orc_tmlp_n_params(void);
+double orc_tmlp_forward(int modes, int symbols, int batch, const double* params, int bf16,
+                        const orc_c128* in, orc_c128* out, const orc_c128* target);
+double orc_tmlp_backward(int modes, int symbols, int batch, const double* params, int bf16,
+                         const orc_c128* in, const orc_c128* target, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,9 @@ _sig = {
     "orc_wdmppe_train": (C.c_int, [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_ldbp_forward": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
     "orc_ldbp_backward": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P, _P]),
+    "orc_tmlp_n_params": (C.c_int, []),
+    "orc_tmlp_forward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, _P, _P]),
+    "orc_tmlp_backward": (C.c_double, [C.c_int, C.c_int, C.c_int, _P, C.c_int, _P, _P, _P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -473,3 +476,33 @@ def ldbp_backward(fir, kerr, x, target, modes=1):
     loss = _lib.orc_ldbp_backward(steps, taps, k.shape[1], modes, _ptr(fir), _ptr(k), x.size // modes, _ptr(x),
                                   _ptr(target), _ptr(gf), _ptr(gk))
     return loss, gf, gk.reshape(shape)
+
+
+TMLP_HIDDEN, TMLP_FEATURES, TMLP_TRIPLETS, TMLP_RADIUS = 128, 2240, 1089, 16
+
+
+def tmlp_n_params():
+    return _lib.orc_tmlp_n_params()
+
+
+def tmlp_forward(params, x, modes, symbols, target=None, bf16=False):
+    """Triplet MLP on symbols x [batch][modes][symbols] (flattened);
+    returns (out, loss) (loss 0 without target)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = _c(x)
+    batch = x.size // (modes * symbols)
+    out = np.zeros_like(x)
+    tgt = _c(target) if target is not None else None
+    loss = _lib.orc_tmlp_forward(modes, symbols, batch, _ptr(p), int(bf16), _ptr(x), _ptr(out),
+                                 _ptr(tgt) if tgt is not None else None)
+    return out, loss
+
+
+def tmlp_backward(params, x, target, modes, symbols, bf16=False):
+    """(loss, dL/dparams [n_params])."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x, target = _c(x), _c(target)
+    batch = x.size // (modes * symbols)
+    g = np.zeros(tmlp_n_params())
+    loss = _lib.orc_tmlp_backward(modes, symbols, batch, _ptr(p), int(bf16), _ptr(x), _ptr(target), _ptr(g))
+    return loss, g
