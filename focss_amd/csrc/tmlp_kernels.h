@@ -29,6 +29,7 @@ constexpr int TM_NP = TM_H * TM_KF + TM_H + 2 * TM_H + 2;  // parameters
 constexpr int TM_SMALL = TM_H + 2 * TM_H + 2 + 1;          // b1, W2, b2 gradients and the loss
 constexpr int TM_ROWS = 128;                // symbols per workgroup block
 constexpr int TM_WIN = TM_ROWS + 4 * TM_R;  // window: 2R symbols either side
+constexpr int TM_QW = TM_ROWS + 2 * TM_R;   // positions of the pair table (window positions R .. R + QW - 1)
 constexpr int TM_NT = 256;                  // 4 waves, 32 symbols each
 constexpr int TM_KC = 64;                   // features per staged W1 chunk
 constexpr int TM_LDW = TM_KC + 8;           // padded LDS row of the chunk (bf16)
@@ -45,20 +46,36 @@ __device__ __forceinline__ unsigned tm_bf16(float f) {
 
 __device__ __forceinline__ tm_bf8 tm_frag(uint4 v) { return __builtin_bit_cast(tm_bf8, v); }
 
-// triplet t at window position c (the window holds every mode of one
-// waveform, [M][TM_WIN], position TM_R * 2 + i is symbol s0 + i)
+// two floats -> packed bf16 pair (v_cvt_pk_bf16_f32, round to nearest even)
+__device__ __forceinline__ unsigned tm_pack2(float lo, float hi) {
+    typedef __bf16 tm_bf2 __attribute__((ext_vector_type(2)));
+    const tm_bf2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(unsigned, v);
+}
+
+// Pair table of one window (the window holds every mode of one waveform,
+// [M][TM_WIN], position 2R + i is symbol s0 + i):
+//     Q_m(c) = sum_p x_p(c) conj(x_p(c + m)),  qt[(m + R) TM_QW + c - R],
+// so that the triplet at window position c is T_{m,n}(c) = x_md(c + m) Q_m(c + n).
 template <int M>
-__device__ __forceinline__ cx<float> tm_triplet(const cx<float>* xw, int md, int c, int t) {
-    const int m = t / TM_W - TM_R, n = t % TM_W - TM_R;
-    cx<float> acc = mk<float>(0.0f, 0.0f);
+__device__ __forceinline__ void tm_build_pairs(cx<float>* qt, const cx<float>* xw) {
+    for (int i = threadIdx.x; i < TM_W * TM_QW; i += TM_NT) {
+        const int m = i / TM_QW - TM_R, c = i % TM_QW + TM_R;
+        cx<float> acc = mk<float>(0.0f, 0.0f);
 #pragma unroll
-    for (int p = 0; p < M; ++p) {
-        const cx<float> a = xw[p * TM_WIN + c + n], b = xw[p * TM_WIN + c + m + n];
-        acc.x = fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x));
-        acc.y = fmaf(a.y, b.x, fmaf(-a.x, b.y, acc.y));
+        for (int p = 0; p < M; ++p) {
+            const cx<float> a = xw[p * TM_WIN + c], b = xw[p * TM_WIN + c + m];
+            acc.x = fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x));
+            acc.y = fmaf(a.y, b.x, fmaf(-a.x, b.y, acc.y));
+        }
+        qt[i] = acc;
     }
-    const cx<float> u = xw[md * TM_WIN + c + m];
-    return mk<float>(u.x * acc.x - u.y * acc.y, u.x * acc.y + u.y * acc.x);
+}
+
+__device__ __forceinline__ cx<float> tm_triplet(const cx<float>* qt, const cx<float>* xm, int c, int m, int n) {
+    const cx<float> q = qt[(m + TM_R) * TM_QW + c + n - TM_R];
+    const cx<float> u = xm[c + m];
+    return mk<float>(u.x * q.x - u.y * q.y, u.x * q.y + u.y * q.x);
 }
 
 // every mode of waveform seq / M, symbols s0 - 2R .. s0 + TM_ROWS + 2R - 1 (mod S)
@@ -102,6 +119,7 @@ struct TmlpArgs {
 template <int M, bool TRAIN>
 __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
     __shared__ cx<float> xw[M * TM_WIN];
+    __shared__ cx<float> qt[TM_W * TM_QW];
     __shared__ __align__(16) unsigned short w1s[TM_H * TM_LDW];
     __shared__ float red[TRAIN ? 4 * TM_SMALL : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, cl = lane & 31, hl = lane >> 5;
@@ -127,6 +145,9 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
     };
     fetch(0);
     stage();
+    __syncthreads();  // window loaded
+    tm_build_pairs<M>(qt, xw);
+    const cx<float>* xm = xw + md * TM_WIN;
     tm_f16 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = tm_f16{};
@@ -144,8 +165,8 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
                 const int t = (TM_KC / 2) * c + 8 * ks + 4 * hl + q;
                 unsigned v = 0;
                 if (t < TM_T) {
-                    const cx<float> T = tm_triplet<M>(xw, md, c_sym, t);
-                    v = tm_bf16(T.x) | (tm_bf16(T.y) << 16);
+                    const cx<float> T = tm_triplet(qt, xm, c_sym, t / TM_W - TM_R, t % TM_W - TM_R);
+                    v = tm_pack2(T.x, T.y);
                 }
                 ap[q] = v;
             }
@@ -230,7 +251,7 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
             const int j = 32 * t + cl;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                unsigned pk[2];
+                float dpv[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int v = 4 * g + e;
@@ -238,14 +259,10 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
                     const float h = fmaxf(pre, 0.0f);
                     gw0[t] = fmaf(d0[v], h, gw0[t]);
                     gw1[t] = fmaf(d1[v], h, gw1[t]);
-                    const float dp = pre > 0.0f ? fmaf(w20[t], d0[v], w21[t] * d1[v]) : 0.0f;
-                    gb1[t] += dp;
-                    const unsigned bits = tm_bf16(dp);
-                    if (e & 1)
-                        pk[e >> 1] |= bits << 16;
-                    else
-                        pk[e >> 1] = bits;
+                    dpv[e] = pre > 0.0f ? fmaf(w20[t], d0[v], w21[t] * d1[v]) : 0.0f;
+                    gb1[t] += dpv[e];
                 }
+                const unsigned pk[2] = {tm_pack2(dpv[0], dpv[1]), tm_pack2(dpv[2], dpv[3])};
                 const long long r = rbase + 8 * g + 4 * hl;
                 *reinterpret_cast<uint2*>(a.dpre + (size_t)j * a.rows + r) = make_uint2(pk[0], pk[1]);
             }
@@ -286,10 +303,12 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
 template <int M>
 __global__ __launch_bounds__(TM_NT) void k_tmlp_dw1(TmlpArgs a, int splits) {
     __shared__ cx<float> xw[M * TM_WIN];
+    __shared__ cx<float> qt[TM_W * TM_QW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, cl = lane & 31, hl = lane >> 5;
     const int tb = 4 * blockIdx.x + w;  // triplet block of this wave
     const int t = 32 * tb + cl;
     const bool live = t < TM_T;
+    const int tm = t / TM_W - TM_R, tn = t % TM_W - TM_R;
     const int split = blockIdx.y;
     const int b_lo = (int)((long long)a.blocks * split / splits), b_hi = (int)((long long)a.blocks * (split + 1) / splits);
     tm_f16 accr[4], acci[4];
@@ -298,9 +317,12 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_dw1(TmlpArgs a, int splits) {
     for (int blk = b_lo; blk < b_hi; ++blk) {
         const int row0 = blk * TM_ROWS;
         const int seq = row0 / a.S, s0 = row0 % a.S, md = seq % M;
-        __syncthreads();  // the previous block's window is no longer read
+        __syncthreads();  // the previous block's window and table are no longer read
         tm_load_window<M>(xw, a.x, seq, s0, a.S);
         __syncthreads();
+        tm_build_pairs<M>(qt, xw);
+        __syncthreads();
+        const cx<float>* xm = xw + md * TM_WIN;
         if (tb < TM_TP / 32) {
 #pragma unroll 2
             for (int ks = 0; ks < TM_ROWS / 16; ++ks) {
@@ -312,10 +334,10 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_dw1(TmlpArgs a, int splits) {
                 for (int u = 0; u < 4; ++u) {
                     unsigned lr = 0, li = 0;
                     if (live) {
-                        const cx<float> T0 = tm_triplet<M>(xw, md, 2 * TM_R + r + 2 * u, t);
-                        const cx<float> T1 = tm_triplet<M>(xw, md, 2 * TM_R + r + 2 * u + 1, t);
-                        lr = tm_bf16(T0.x) | (tm_bf16(T1.x) << 16);
-                        li = tm_bf16(T0.y) | (tm_bf16(T1.y) << 16);
+                        const cx<float> T0 = tm_triplet(qt, xm, 2 * TM_R + r + 2 * u, tm, tn);
+                        const cx<float> T1 = tm_triplet(qt, xm, 2 * TM_R + r + 2 * u + 1, tm, tn);
+                        lr = tm_pack2(T0.x, T1.x);
+                        li = tm_pack2(T0.y, T1.y);
                     }
                     pr[u] = lr;
                     pi[u] = li;
