@@ -72,6 +72,24 @@ __device__ __forceinline__ void tm_build_pairs(cx<float>* qt, const cx<float>* x
     }
 }
 
+// The rows m_lo and m_lo + 1 of the pair table (the m values of one chunk of
+// 32 consecutive triplets): qt2[(m - m_lo) TM_QW + c - R].
+template <int M>
+__device__ __forceinline__ void tm_build_pairs2(cx<float>* qt2, const cx<float>* xw, int m_lo) {
+    for (int i = threadIdx.x; i < 2 * TM_QW; i += TM_NT) {
+        const int m = m_lo + i / TM_QW, c = i % TM_QW + TM_R;
+        if (m > TM_R) continue;
+        cx<float> acc = mk<float>(0.0f, 0.0f);
+#pragma unroll
+        for (int p = 0; p < M; ++p) {
+            const cx<float> a = xw[p * TM_WIN + c], b = xw[p * TM_WIN + c + m];
+            acc.x = fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x));
+            acc.y = fmaf(a.y, b.x, fmaf(-a.x, b.y, acc.y));
+        }
+        qt2[i] = acc;
+    }
+}
+
 __device__ __forceinline__ cx<float> tm_triplet(const cx<float>* qt, const cx<float>* xm, int c, int m, int n) {
     const cx<float> q = qt[(m + TM_R) * TM_QW + c + n - TM_R];
     const cx<float> u = xm[c + m];
@@ -119,7 +137,7 @@ struct TmlpArgs {
 template <int M, bool TRAIN>
 __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
     __shared__ cx<float> xw[M * TM_WIN];
-    __shared__ cx<float> qt[TM_W * TM_QW];
+    __shared__ cx<float> qt2[2 * TM_QW];  // the pair-table rows of the current chunk
     __shared__ __align__(16) unsigned short w1s[TM_H * TM_LDW];
     __shared__ float red[TRAIN ? 4 * TM_SMALL : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, cl = lane & 31, hl = lane >> 5;
@@ -127,26 +145,35 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
     const int row0 = blk * TM_ROWS;
     const int seq = row0 / a.S, s0 = row0 % a.S, md = seq % M;
     tm_load_window<M>(xw, a.x, seq, s0, a.S);
-    // W1 chunk c: rows j, features 64 c .. 64 c + 63; thread loads 4 x 16 B
-    uint4 pre_w[4];
+    // W1 chunk c: rows j, features 64 c .. 64 c + 63; thread moves 4 x 16 B
+    // (piece q: row (tid + 256 q) / 8, 8 features at 8 ((tid + 256 q) % 8))
+    uint4 pw0, pw1, pw2, pw3;
+    auto w1_src = [&](int c, int q) {
+        const int idx = threadIdx.x + TM_NT * q;
+        return reinterpret_cast<const uint4*>(a.w1b + (size_t)(idx >> 3) * TM_KF + TM_KC * c + 8 * (idx & 7));
+    };
+    auto w1_dst = [&](int q) {
+        const int idx = threadIdx.x + TM_NT * q;
+        return reinterpret_cast<uint4*>(w1s + (idx >> 3) * TM_LDW + 8 * (idx & 7));
+    };
     auto fetch = [&](int c) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int idx = threadIdx.x + TM_NT * q, j = idx >> 3, piece = idx & 7;
-            pre_w[q] = *reinterpret_cast<const uint4*>(a.w1b + (size_t)j * TM_KF + TM_KC * c + 8 * piece);
-        }
+        pw0 = *w1_src(c, 0);
+        pw1 = *w1_src(c, 1);
+        pw2 = *w1_src(c, 2);
+        pw3 = *w1_src(c, 3);
     };
     auto stage = [&]() {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int idx = threadIdx.x + TM_NT * q, j = idx >> 3, piece = idx & 7;
-            *reinterpret_cast<uint4*>(w1s + j * TM_LDW + 8 * piece) = pre_w[q];
-        }
+        *w1_dst(0) = pw0;
+        *w1_dst(1) = pw1;
+        *w1_dst(2) = pw2;
+        *w1_dst(3) = pw3;
     };
+    // first m of chunk c (32 triplets, 33 per value of m)
+    auto chunk_m = [](int c) { return (TM_KC / 2) * c / TM_W - TM_R; };
     fetch(0);
     stage();
     __syncthreads();  // window loaded
-    tm_build_pairs<M>(qt, xw);
+    tm_build_pairs2<M>(qt2, xw, chunk_m(0));
     const cx<float>* xm = xw + md * TM_WIN;
     tm_f16 acc[4];
 #pragma unroll
@@ -156,19 +183,21 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
     __syncthreads();
     for (int c = 0; c < NCH; ++c) {
         if (c + 1 < NCH) fetch(c + 1);
+        const int m_lo = chunk_m(c);
 #pragma unroll
         for (int ks = 0; ks < TM_KC / 16; ++ks) {
             uint4 av;
             unsigned* ap = reinterpret_cast<unsigned*>(&av);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
+                // branch-free: the padding slots t >= 1089 (last chunk only,
+                // m = 17, table row 1) read in-bounds LDS and are zeroed
                 const int t = (TM_KC / 2) * c + 8 * ks + 4 * hl + q;
-                unsigned v = 0;
-                if (t < TM_T) {
-                    const cx<float> T = tm_triplet(qt, xm, c_sym, t / TM_W - TM_R, t % TM_W - TM_R);
-                    v = tm_pack2(T.x, T.y);
-                }
-                ap[q] = v;
+                const int m = t / TM_W - TM_R, n = t % TM_W - TM_R;
+                const cx<float> qv = qt2[(m - m_lo) * TM_QW + c_sym + n - TM_R];
+                const cx<float> u = xm[c_sym + m];
+                const unsigned v = tm_pack2(u.x * qv.x - u.y * qv.y, u.x * qv.y + u.y * qv.x);
+                ap[q] = t < TM_T ? v : 0u;
             }
             const tm_bf8 A = tm_frag(av);
 #pragma unroll
@@ -180,6 +209,7 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
         if (c + 1 < NCH) {
             __syncthreads();  // every wave is done with chunk c
             stage();
+            tm_build_pairs2<M>(qt2, xw, chunk_m(c + 1));
             __syncthreads();
         }
     }

@@ -118,6 +118,26 @@ def side(args):
         for _ in range(args.reps):
             ctx.wdm_ppe_train(30, 2, ta, tr, n)
         print(f"probe side: WdmPPE training c=30 d=2 on 2^14 samples, {args.reps} runs", flush=True)
+        # triplet MLP: inference on 512 x 8192 symbols, training steps on 16 x 8192
+        S = 8192
+        for batch, train in ((512, False), (16, True)):
+            x = (0.3 * crandn(batch * S)).astype(np.complex64)
+            mlp = F.TripletMLP(ctx, 1, S, batch, seed=1)
+            bi, bo = ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
+            bi.upload(x)
+            bo.upload(x)
+            step = (lambda: mlp.train_step(bi, bo, 1e-4)) if train else (lambda: mlp.forward(bi, bo))
+            step()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.reps):
+                step()
+            ctx.synchronize()
+            t = (time.perf_counter() - t0) / args.reps
+            what = "training step" if train else "inference"
+            print(f"probe side: triplet MLP {what}, {batch} x {S} symbols: {t * 1e3:.3f} ms", flush=True)
+            for b in (bi, bo, mlp):
+                b.free()
     return 0
 
 
