@@ -267,6 +267,21 @@ def main():
             "waveforms_per_s": {k: round(v, 1) for k, v in rep["waveforms_per_s"].items()},
             "speedup_learned_vs_dbp16": round(rep["speedup_learned_vs_dbp16"], 2),
             "train_seconds": rep["train_seconds"]}
+        # the learned configuration's optional network: the triplet MLP after
+        # 1-step-per-span DBP, trained on 10^6 simulator symbols
+        from focss_amd.triplet_mlp import parser as tmlp_parser, run as tmlp_run
+        tr = tmlp_run(tmlp_parser().parse_args([]), log=lambda m: print(m, file=sys.stderr, flush=True))
+        line["triplet_mlp"] = {
+            "workload": "16 held-out waveforms x 8192 symbols (16QAM 32 GBd), 20x80 km, launch 2..6 dBm, after plain "
+                        "FP64 DBP at 1 step/span; 1089 triplets (|m|,|n|<=16) -> 128 ReLU -> complex correction, "
+                        "BF16 matrix cores with FP32 accumulation; trained on 128 waveforms (10^6 symbols); "
+                        f"inference rate on {tr['inference']['waveforms']} waveforms",
+            "q_db": tr["q_db"], "mlp_gain_db": tr["mlp_gain_db"],
+            "train_seconds": tr["train"]["seconds"],
+            "inference": {"ms": tr["inference"]["ms"], "symbols_per_s": round(tr["inference"]["symbols_per_s"]),
+                          "roofline": {"bound": "mfma", "achieved": round(tr["inference"]["tflops"], 1),
+                                       "peak": 2500.0, "unit": "TFLOP/s",
+                                       "frac": round(tr["inference"]["frac_bf16_peak"], 4)}}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(field, dt, args.cpu_km)
     line["finite"] = all(r["finite"] for r in res.values())
