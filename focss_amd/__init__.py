@@ -63,6 +63,9 @@ _sig = {
     "focss_bandpass_filter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P]),
     "focss_wdm_select": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _P, _P]),
     "focss_decimate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "focss_pse_estimate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
+    "focss_scalar_estimate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+    "focss_homothety_estimate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
     "focss_filter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P]),
     "focss_rrc_shaping": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_double, C.c_double, _P]),
     "focss_rrc_sampling": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_double, C.c_double, _P]),
@@ -226,6 +229,27 @@ class Context:
 
     def decimate(self, inp, out, modes, samples, factor, precision=F64):
         _check(_lib.focss_decimate(self.handle, precision, modes, samples, factor, inp.ptr, out.ptr))
+
+    def pse_estimate(self, desired, actual, modes, samples, angle_steps=3600, precision=F64):
+        """PhaseShiftEqualizer::estimate: the grid angle whose rotation of
+        actual best matches desired (q2_factor)."""
+        a = C.c_double(0)
+        _check(_lib.focss_pse_estimate(self.handle, precision, modes, samples, desired.ptr, actual.ptr, angle_steps,
+                                       C.byref(a)))
+        return a.value
+
+    def scalar_estimate(self, desired, actual, modes, samples, precision=F64):
+        """ScalarEqualizer::estimate: sum conj(a) d / sum |a|^2 (complex)."""
+        v = (C.c_double * 2)()
+        _check(_lib.focss_scalar_estimate(self.handle, precision, modes, samples, desired.ptr, actual.ptr, v))
+        return complex(v[0], v[1])
+
+    def homothety_estimate(self, desired, actual, modes, samples, precision=F64):
+        """HomothetyEqualizer::estimate: |sum conj(a) d / sum |a|^2|."""
+        v = C.c_double(0)
+        _check(_lib.focss_homothety_estimate(self.handle, precision, modes, samples, desired.ptr, actual.ptr,
+                                             C.byref(v)))
+        return v.value
 
     def filter(self, buf, modes, samples, impulse_response, precision=F64):
         h = np.ascontiguousarray(impulse_response, dtype=np.complex128)

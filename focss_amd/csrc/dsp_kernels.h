@@ -134,4 +134,35 @@ __global__ __launch_bounds__(IO_NT) void k_envelope_transpose(const void* __rest
     }
 }
 
+// Sums behind the constant equalisers of src/focss/equalizer (phase search
+// phase_shift_equalizer.cpp:30-46, complex scalar scalar_equalizer.cpp:22-33,
+// real scalar homothety_equalizer.cpp:22-33), over n complex elements in FP64:
+//     out[0] += sum |d|^2,  out[1] += sum |a|^2,  out[2] + i out[3] += sum conj(a) d.
+constexpr int LS_NT = 256;
+template <typename R>
+__global__ __launch_bounds__(LS_NT) void k_linear_stats(const cx<R>* __restrict__ d, const cx<R>* __restrict__ a,
+                                                        long long n, double* __restrict__ out) {
+    __shared__ double red[4][LS_NT / 64];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long long i = blockIdx.x * (long long)LS_NT + threadIdx.x; i < n; i += (long long)gridDim.x * LS_NT) {
+        const double dr = d[i].x, di = d[i].y, ar = a[i].x, ai = a[i].y;
+        s[0] += dr * dr + di * di;
+        s[1] += ar * ar + ai * ai;
+        s[2] += ar * dr + ai * di;
+        s[3] += ar * di - ai * dr;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = s[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < LS_NT / 64; ++w) t += red[threadIdx.x][w];
+        atomicAdd(out + threadIdx.x, t);
+    }
+}
+
 }  // namespace focss_hip

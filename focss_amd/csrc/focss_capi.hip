@@ -270,6 +270,7 @@ struct focss_ctx {
     size_t ldbp_acc_bytes = 0;
     unsigned short* tmlp_w1b = nullptr;  // triplet MLP: bf16 copy of W1 [128][2240]
     double* tmlp_acc = nullptr;          // triplet MLP: loss, then the gradient [TM_NP]
+    double* lin_stats = nullptr;         // constant equalisers: the four sums of k_linear_stats
     void* dsp_tmp = nullptr;    // transmitter/receiver DSP work space
     size_t dsp_tmp_bytes = 0;
 };
@@ -1216,6 +1217,7 @@ int focss_ctx_destroy(focss_ctx* c) {
     if (c->ldbp_acc) (void)hipFree(c->ldbp_acc);
     if (c->tmlp_w1b) (void)hipFree(c->tmlp_w1b);
     if (c->tmlp_acc) (void)hipFree(c->tmlp_acc);
+    if (c->lin_stats) (void)hipFree(c->lin_stats);
     if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
     if (c->d_statics) {
         (void)hipFree(c->d_statics);
@@ -1863,6 +1865,88 @@ int focss_filter(focss_ctx* c, int precision, int modes, int samples, int taps, 
     HIP_TRY(hipSetDevice(c->device));
     FOCSS_DISPATCH((filter_entry<double>(c, modes, samples, taps, impulse_response, data)),
                    (filter_entry<float>(c, modes, samples, taps, impulse_response, data)));
+}
+
+// ---- constant equalisers (src/focss/equalizer/linear.h) ----
+
+namespace {
+
+// st = {sum |d|^2, sum |a|^2, Re sum conj(a) d, Im sum conj(a) d} (host), synchronous
+int linear_stats(focss_ctx* c, int precision, int modes, int samples, const void* desired, const void* actual,
+                 double st[4]) {
+    if (!c || !desired || !actual) return fail(FOCSS_E_ARG, "linear equaliser: null argument");
+    if (modes < 1 || samples < 1 || (long long)modes * samples > (1LL << 31))
+        return fail(FOCSS_E_ARG, "linear equaliser: modes and samples must be >= 1");
+    if (precision != FOCSS_F64 && precision != FOCSS_F32)
+        return fail(FOCSS_E_ARG, "precision must be FOCSS_F64 or FOCSS_F32");
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->lin_stats) HIP_TRY(hipMalloc(&c->lin_stats, 4 * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(c->lin_stats, 0, 4 * sizeof(double), c->stream));
+    const long long n = (long long)modes * samples;
+    const int grid = (int)std::min<long long>((n + LS_NT - 1) / LS_NT, 1024);
+    if (precision == FOCSS_F64)
+        hipLaunchKernelGGL(k_linear_stats<double>, dim3(grid), dim3(LS_NT), 0, c->stream,
+                           (const cx<double>*)desired, (const cx<double>*)actual, n, c->lin_stats);
+    else
+        hipLaunchKernelGGL(k_linear_stats<float>, dim3(grid), dim3(LS_NT), 0, c->stream,
+                           (const cx<float>*)desired, (const cx<float>*)actual, n, c->lin_stats);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(st, c->lin_stats, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+// phase_shift_equalizer.cpp:30-46: q2 of actual e^{i angle} against desired
+// for angle = 2 pi i / angle_steps, the first maximum (starting from the
+// unrotated field).  sum |d - a e^{i t}|^2 = sum |d|^2 + sum |a|^2
+// - 2 Re(e^{-i t} sum d conj(a)), so one pass over the fields gives every
+// candidate's q2.
+int focss_pse_estimate(focss_ctx* c, int precision, int modes, int samples, const void* desired, const void* actual,
+                       int angle_steps, double* angle) {
+    if (!angle) return fail(FOCSS_E_ARG, "pse_estimate: null argument");
+    if (angle_steps < 0) return fail(FOCSS_E_ARG, "pse_estimate: angle_steps must be >= 0");
+    double st[4];
+    if (int rc = linear_stats(c, precision, modes, samples, desired, actual, st)) return rc;
+    const double pd = st[0], pa = st[1], cr = st[2], ci = st[3];  // C = sum d conj(a) = sum conj(a) d
+    auto q2 = [&](double t) {
+        const double num = pd + pa - 2.0 * (cr * std::cos(t) + ci * std::sin(t));
+        return num > 0.0 ? -10.0 * std::log10(num / pd) : HUGE_VAL;
+    };
+    const double two_pi = 6.283185307179586476925286766559;
+    double best = q2(0.0), arg = 0.0;
+    for (int i = 0; i < angle_steps; ++i) {
+        const double t = two_pi * (double)i / angle_steps;
+        const double q = q2(t);
+        if (best < q) {
+            best = q;
+            arg = t;
+        }
+    }
+    *angle = arg;
+    return 0;
+}
+
+// scalar_equalizer.cpp:22-33: sum conj(a) d / sum |a|^2 (complex, [re, im])
+int focss_scalar_estimate(focss_ctx* c, int precision, int modes, int samples, const void* desired,
+                          const void* actual, double* scalar) {
+    if (!scalar) return fail(FOCSS_E_ARG, "scalar_estimate: null argument");
+    double st[4];
+    if (int rc = linear_stats(c, precision, modes, samples, desired, actual, st)) return rc;
+    scalar[0] = st[2] / st[1];
+    scalar[1] = st[3] / st[1];
+    return 0;
+}
+
+// homothety_equalizer.cpp:22-33: |sum conj(a) d / sum |a|^2|
+int focss_homothety_estimate(focss_ctx* c, int precision, int modes, int samples, const void* desired,
+                             const void* actual, double* scalar) {
+    if (!scalar) return fail(FOCSS_E_ARG, "homothety_estimate: null argument");
+    double st[4];
+    if (int rc = linear_stats(c, precision, modes, samples, desired, actual, st)) return rc;
+    *scalar = std::hypot(st[2] / st[1], st[3] / st[1]);
+    return 0;
 }
 
 int focss_rrc_shaping(focss_ctx* c, int precision, int modes, int n_symbols, const void* symbols, int osf,

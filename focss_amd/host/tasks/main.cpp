@@ -9,6 +9,9 @@
 //                       (tx, rx: 6 envelope rows; out: C (2c+1)^2 if c >= 0,
 //                        D (2d+1)^2 if d >= 0, intercept, linear weight, then
 //                        the equalized COI (rx_coi - cubic(tx)) / w, 2 rows)
+//   focss_tasks linear-eq <desired> <actual> <out> <samples> <modes> <angle_steps>
+//                       (out: PSE angle, homothety scalar, complex scalar as 4
+//                        doubles, then PhaseShiftEqualizer::train_equalize)
 //   focss_tasks forward <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
 //                       <max_phase_shift> <max_step> <nf_db> <seed> <center_frequency>
 //   focss_tasks save-field <in> <set> <tag> <index> <channels> <modes> <samples>
@@ -27,6 +30,7 @@
 #include <string>
 #include <vector>
 
+#include "focss/equalizer/linear.h"
 #include "focss/equalizer/sino.h"
 #include "focss/equalizer/wdm_sino.h"
 #include "focss/field.h"
@@ -117,6 +121,24 @@ int main(int argc, char** argv) {
         out.write(reinterpret_cast<const char*>(&ic), sizeof(ic));
         out.write(reinterpret_cast<const char*>(&lw), sizeof(lw));
         focss::Field eq = sino.equalize(rx, tx);
+        out.write(reinterpret_cast<const char*>(eq.raw()), (std::streamsize)(eq.size() * sizeof(focss::complex_t)));
+        return 0;
+    }
+    if (cmd == "linear-eq" && argc == 8) {
+        int samples = std::atoi(argv[5]), modes = std::atoi(argv[6]);
+        focss::Field d = read_field(argv[2], samples, modes, 1e-11, 0.0);
+        focss::Field a = read_field(argv[3], samples, modes, 1e-11, 0.0);
+        focss::PhaseShiftEqualizer pse(std::atoi(argv[7]));
+        pse.train(d, a);
+        focss::HomothetyEqualizer hom;
+        hom.train(d, a);
+        focss::ScalarEqualizer sca;
+        sca.train(d, a);
+        const double head[4] = {pse.get_angle(), hom.get_scalar(), sca.get_scalar().real(),
+                                sca.get_scalar().imag()};
+        std::ofstream out(argv[4], std::ios::binary);
+        out.write(reinterpret_cast<const char*>(head), sizeof(head));
+        focss::Field eq = pse.train_equalize(d, a);
         out.write(reinterpret_cast<const char*>(eq.raw()), (std::streamsize)(eq.size() * sizeof(focss::complex_t)));
         return 0;
     }

@@ -230,6 +230,23 @@ int focss_field_from_records(focss_ctx* ctx, int precision, int envelopes, int s
 int focss_field_to_records(focss_ctx* ctx, int precision, int envelopes, int samples, const void* field,
                            void* records);
 
+/* ---- constant equalisers (src/focss/equalizer/linear.h) ----
+ * desired and actual are device fields [modes][samples] in the given
+ * precision; one device pass forms sum |d|^2, sum |a|^2 and sum conj(a) d in
+ * FP64; each call synchronises and returns its estimate on the host. */
+/* PhaseShiftEqualizer::estimate (phase_shift_equalizer.cpp:30-46): the angle
+ * 2 pi i / angle_steps whose rotation of actual maximises q2_factor against
+ * desired, the first maximum, 0 when no rotation beats the unrotated field. */
+int focss_pse_estimate(focss_ctx* ctx, int precision, int modes, int samples, const void* desired, const void* actual,
+                       int angle_steps, double* angle);
+/* ScalarEqualizer::estimate (scalar_equalizer.cpp:22-33): sum conj(a) d / sum |a|^2,
+ * written as scalar[0] + i scalar[1]. */
+int focss_scalar_estimate(focss_ctx* ctx, int precision, int modes, int samples, const void* desired,
+                          const void* actual, double* scalar);
+/* HomothetyEqualizer::estimate (homothety_equalizer.cpp:22-33): |sum conj(a) d / sum |a|^2|. */
+int focss_homothety_estimate(focss_ctx* ctx, int precision, int modes, int samples, const void* desired,
+                             const void* actual, double* scalar);
+
 /* ---- learned few-step DBP (no counterpart in the reference) ----
  * Step j: y = complex FIR (taps, circular, centre taps/2) of x, then
  * z = y exp(i phi), phi(t) = sum_m g_{j,m} |y(t - m + kerr_taps/2)|^2 (a real

@@ -695,6 +695,22 @@ double orc_pse_estimate(int size, const orc_c128* desired, const orc_c128* actua
     return argmax;
 }
 
+/* ScalarEqualizer::estimate: sum conj(a) d / sum |a|^2   scalar_equalizer.cpp:22-33 */
+orc_c128 orc_scalar_estimate(int size, const orc_c128* desired, const orc_c128* actual) {
+    double xx = 0;
+    orc_c128 xy = {0, 0};
+    for (int i = 0; i < size; ++i) xx += c_norm(actual[i]);
+    for (int i = 0; i < size; ++i) xy = c_add(xy, c_mul(c_conj(actual[i]), desired[i]));
+    orc_c128 r = {xy.re / xx, xy.im / xx};
+    return r;
+}
+
+/* HomothetyEqualizer::estimate: |sum conj(a) d / sum |a|^2|   homothety_equalizer.cpp:22-33 */
+double orc_homothety_estimate(int size, const orc_c128* desired, const orc_c128* actual) {
+    orc_c128 r = orc_scalar_estimate(size, desired, actual);
+    return hypot(r.re, r.im);
+}
+
 /* ------------------------------------------------------------------ */
 /* Triplet equalizer                           sino.cpp                 */
 /* ------------------------------------------------------------------ */

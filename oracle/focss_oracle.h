@@ -105,6 +105,8 @@ void orc_hd_sequence_16qam(int modes, int samples, const orc_c128* f, int* codes
 double orc_ber_16qam(int modes, int samples, const orc_c128* tx, const orc_c128* rx, int cut);
 double orc_q2_factor(int modes, int samples, const orc_c128* tx, const orc_c128* rx, int cut);
 double orc_pse_estimate(int size, const orc_c128* desired, const orc_c128* actual, int angle_steps);
+orc_c128 orc_scalar_estimate(int size, const orc_c128* desired, const orc_c128* actual);
+double orc_homothety_estimate(int size, const orc_c128* desired, const orc_c128* actual);
 
 /* ---- perturbation-triplet equalizer (src/focss/equalizer/sino.cpp) ---- */
 void orc_sino_term(int modes, int samples, const orc_c128* f, int mode, int k, int m, int n,

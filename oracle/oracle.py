@@ -103,6 +103,8 @@ _sig = {
     "orc_ber_16qam": (C.c_double, [C.c_int, C.c_int, _P, _P, C.c_int]),
     "orc_q2_factor": (C.c_double, [C.c_int, C.c_int, _P, _P, C.c_int]),
     "orc_pse_estimate": (C.c_double, [C.c_int, _P, _P, C.c_int]),
+    "orc_scalar_estimate": (C128, [C.c_int, _P, _P]),
+    "orc_homothety_estimate": (C.c_double, [C.c_int, _P, _P]),
     "orc_sino_term": (None, [C.c_int, C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "orc_sino_cubic": (None, [C.c_int, _P, C.c_int, C.c_int, _P, _P]),
     "orc_sino_n_features": (C.c_int, [C.c_int, C.c_int]),
@@ -333,6 +335,17 @@ def ber_16qam(tx, rx, modes=1, cut=0):
 def q2_factor(tx, rx, modes=1, cut=0):
     tx, rx = _c(tx), _c(rx)
     return _lib.orc_q2_factor(modes, tx.size // modes, _ptr(tx), _ptr(rx), cut)
+
+
+def scalar_estimate(desired, actual):
+    d, a = _c(desired), _c(actual)
+    r = _lib.orc_scalar_estimate(d.size, _ptr(d), _ptr(a))
+    return complex(r.re, r.im)
+
+
+def homothety_estimate(desired, actual):
+    d, a = _c(desired), _c(actual)
+    return _lib.orc_homothety_estimate(d.size, _ptr(d), _ptr(a))
 
 
 def pse_estimate(desired, actual, angle_steps=3600):
