@@ -558,16 +558,17 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
     }
 }
 
-// Adam (Kingma & Ba, 2015) on float parameters with double gradients;
-// p holds the FIR taps (re, im interleaved) followed by the intensity taps.
+// Adam (Kingma & Ba, 2015) on float parameters with double gradients (the
+// learned DBP's FIR and intensity taps, the triplet MLP's weights).
 __global__ void k_adam(float* __restrict__ p, const double* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, int n, float lr, float beta1, float beta2, float eps, int t,
                        const float* __restrict__ lr_scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    // the taps carry the Wirtinger gradient dL/d(conj h) = (dL/dRe + i dL/dIm) / 2;
-    // Adam normalises every coordinate, so the factor 2 against the real
-    // intensity-tap gradients does not change the update
+    // learned DBP: the taps carry the Wirtinger gradient dL/d(conj h) =
+    // (dL/dRe + i dL/dIm) / 2; Adam normalises every coordinate, so the
+    // factor 2 against the real intensity-tap gradients does not change the
+    // update
     float gi = (float)g[i];
     float mi = beta1 * m[i] + (1 - beta1) * gi;
     float vi = beta2 * v[i] + (1 - beta2) * gi * gi;

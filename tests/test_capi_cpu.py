@@ -86,3 +86,22 @@ def test_host_api_built():
     """The C++ host library and its driver are built next to the HIP library."""
     for name in ("libfocss.so", "focss_tasks"):
         assert os.path.exists(os.path.join(ROOT, "focss_amd", name)), name
+
+
+def test_tmlp_and_equaliser_guards():
+    """The triplet-MLP and constant-equaliser entries reject null arguments
+    before any device call; the MLP's sizes are host arithmetic."""
+    import ctypes
+    import focss_amd as F
+    lib = F._lib
+    assert lib.focss_tmlp_param_count() == 128 * 2240 + 128 + 2 * 128 + 2
+    assert lib.focss_tmlp_workspace_bytes(1, 100, 1) == 0
+    rows = 2 * 256
+    ws = lib.focss_tmlp_workspace_bytes(1, 256, 2)
+    assert ws >= rows * 128 * 2 + 387 * (rows // 128) * 4 + (rows // 128) * 128 * 2240 * 4
+    assert lib.focss_tmlp_forward(None, 1, 256, 1, None, None, None, None, None) == -1
+    assert b"triplet MLP" in lib.focss_last_error()
+    a = ctypes.c_double(0)
+    assert lib.focss_pse_estimate(None, F.F64, 1, 16, None, None, 3600, ctypes.byref(a)) == -1
+    assert b"linear equaliser" in lib.focss_last_error()
+    assert lib.focss_scalar_estimate(None, F.F64, 1, 16, None, None, None) == -1
