@@ -472,16 +472,26 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             acc[it] = ldbp_f16{};
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
+                // the lane's column and half, opaque to the compiler within the
+                // step: otherwise it hoists every gather address of the step
+                // (2 tiles x NKS x 4 per lane) out of the step loop and spills
+                // them to scratch
+                int colj = col, hlj = hl;
+                asm volatile("" : "+v"(colj), "+v"(hlj));
                 const cx_h* xm = xs + (tile / mode_tiles) * N;
-                const int blk = (tile % mode_tiles) * 32 + col;
-                const int off = c - (K - 1) - DELTA + 4 * hl;  // window start relative to 16 blk
+                const int blk = (tile % mode_tiles) * 32 + colj;
+                constexpr int C0 = c - (K - 1) - DELTA;  // window start relative to 16 blk, half 0
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) {
                     ldbp_h8 bf;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int o = off + 8 * s + v;  // sample 16 blk + o = row o & 15 of block blk + (o >> 4)
-                        const cx_h xv = xm[((o & 15) << lnb) + ((blk + (o >> 4)) & (nb - 1))];
+                        // sample 16 blk + o = row o & 15 of block blk + (o >> 4),
+                        // o = C0 + 4 hl + 8 s + v
+                        const int oa = C0 + 8 * s + v, ob = oa + 4;
+                        const int row = hlj ? (ob & 15) : (oa & 15);
+                        const int bo = hlj ? (ob >> 4) : (oa >> 4);
+                        const cx_h xv = xm[(row << lnb) + ((blk + bo) & (nb - 1))];
                         bf[2 * v] = xv.x;
                         bf[2 * v + 1] = xv.y;
                     }
@@ -540,8 +550,15 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
                 float phi = 0.0f;
 #pragma unroll
                 for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
+                // the hardware sine and cosine (v_sin_f32 / v_cos_f32, ~1e-6
+                // absolute): the field itself is held in FP16 here
                 float sn, cs;
-                sincos_fast(phi, &sn, &cs);
+                if (fabsf(phi) <= 1000.0f) {
+                    sn = __sinf(phi);
+                    cs = __cosf(phi);
+                } else {
+                    sincos_fast(phi, &sn, &cs);
+                }
 #pragma unroll
                 for (int md = 0; md < M; ++md) {
                     const cx_h yh = xs[md * N + (r << lnb) + blk];
