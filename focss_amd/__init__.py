@@ -458,15 +458,11 @@ class TripletMLP:
             raise FocssError("triplet MLP: %d parameters expected" % self.n_params)
         self.params = ctx.alloc(host.nbytes)
         self.params.upload(host)
-        self.m = ctx.alloc(host.nbytes)
-        self.v = ctx.alloc(host.nbytes)
-        zeros = np.zeros_like(host)
-        self.m.upload(zeros)
-        self.v.upload(zeros)
         self.t = 0
+        # Adam state, workspace and gradient on the first gradients /
+        # train_step call; inference needs none
         self.ws_bytes = _lib.focss_tmlp_workspace_bytes(modes, symbols, batch)
-        self.ws = None
-        self.grad = None
+        self.m = self.v = self.ws = self.grad = None
 
     @classmethod
     def init_params(cls, seed=0, w2_scale=1e-2):
@@ -483,6 +479,11 @@ class TripletMLP:
         if self.ws is None:
             self.ws = self.ctx.alloc(self.ws_bytes)
             self.grad = self.ctx.alloc(8 * (self.n_params + 1))
+            zeros = np.zeros(self.n_params, dtype=np.float32)
+            self.m = self.ctx.alloc(zeros.nbytes)
+            self.v = self.ctx.alloc(zeros.nbytes)
+            self.m.upload(zeros)
+            self.v.upload(zeros)
 
     def free(self):
         for b in (self.params, self.m, self.v, self.ws, self.grad):
@@ -493,8 +494,9 @@ class TripletMLP:
         return self.params.download(np.empty(self.n_params, dtype=np.float32))
 
     def forward(self, inp, out, target=None, batch=None):
-        """out = the equalised symbols; returns the loss against target (the
-        call then synchronises) or None."""
+        """out = the equalised symbols of `batch` waveforms (default: the
+        batch given at construction; inference takes any batch); returns the
+        loss against target (the call then synchronises) or None."""
         loss = C.c_double(0)
         _check(_lib.focss_tmlp_forward(self.ctx.handle, self.modes, self.symbols, batch or self.batch,
                                        self.params.ptr, inp.ptr, out.ptr, target.ptr if target else None,

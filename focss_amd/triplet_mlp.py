@@ -82,13 +82,11 @@ def run(args, log=print):
             f"{np.mean(losses[-20:]):.4e}, {t_train:.1f} s")
 
         def apply(x):
-            m = len(x)
-            inf = F.TripletMLP(ctx, 1, S, m, params=mlp.get())
             ib, ob = ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
             ib.upload(np.ascontiguousarray(x).ravel())
-            inf.forward(ib, ob)
+            mlp.forward(ib, ob, batch=len(x))
             y = ob.download(np.empty(x.size, dtype=np.complex64)).reshape(x.shape)
-            for b in (inf, ib, ob):
+            for b in (ib, ob):
                 b.free()
             return y
 
@@ -99,17 +97,16 @@ def run(args, log=print):
         # inference rate: rate_batch waveforms (the test set repeated), device-resident
         R = args.rate_batch
         big = np.concatenate([x_te] * (R // args.test + 1))[:R]
-        inf = F.TripletMLP(ctx, 1, S, R, params=mlp.get())
         ib, ob = ctx.alloc(big.nbytes), ctx.alloc(big.nbytes)
         ib.upload(big.ravel())
-        inf.forward(ib, ob)
+        mlp.forward(ib, ob, batch=R)
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.reps):
-            inf.forward(ib, ob)
+            mlp.forward(ib, ob, batch=R)
         ctx.synchronize()
         t_inf = (time.perf_counter() - t0) / args.reps
-        for b in (inf, ib, ob, mlp, xb, tb):
+        for b in (ib, ob, mlp, xb, tb):
             b.free()
         flops = 2.0 * 128 * 2 * 1089 * R * S  # algorithmic: the 2178-wide layer
         res = {
