@@ -184,6 +184,12 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
     for (int c = 0; c < NCH; ++c) {
         if (c + 1 < NCH) fetch(c + 1);
         const int m_lo = chunk_m(c);
+        // the lane's triplets of this chunk are t0 + d, d = 8 ks + q < 32 < 33:
+        // (m, n) follow from t0's by at most one carry of n into m
+        const int t0 = (TM_KC / 2) * c + 4 * hl;
+        const int m0 = t0 / TM_W, n0 = t0 - TM_W * m0;  // m = m0 - R (+1 on carry), n likewise
+        const int qbase = (m0 - TM_R - m_lo) * TM_QW + c_sym + n0 - 2 * TM_R;
+        const int xbase = c_sym + m0 - TM_R;
 #pragma unroll
         for (int ks = 0; ks < TM_KC / 16; ++ks) {
             uint4 av;
@@ -192,12 +198,12 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
             for (int q = 0; q < 4; ++q) {
                 // branch-free: the padding slots t >= 1089 (last chunk only,
                 // m = 17, table row 1) read in-bounds LDS and are zeroed
-                const int t = (TM_KC / 2) * c + 8 * ks + 4 * hl + q;
-                const int m = t / TM_W - TM_R, n = t % TM_W - TM_R;
-                const cx<float> qv = qt2[(m - m_lo) * TM_QW + c_sym + n - TM_R];
-                const cx<float> u = xm[c_sym + m];
+                const int d = 8 * ks + q;
+                const bool carry = n0 + d >= TM_W;
+                const cx<float> qv = qt2[qbase + d + (carry ? TM_QW - TM_W : 0)];
+                const cx<float> u = xm[xbase + (carry ? 1 : 0)];
                 const unsigned v = tm_pack2(u.x * qv.x - u.y * qv.y, u.x * qv.y + u.y * qv.x);
-                ap[q] = t < TM_T ? v : 0u;
+                ap[q] = t0 + d < TM_T ? v : 0u;
             }
             const tm_bf8 A = tm_frag(av);
 #pragma unroll
