@@ -135,7 +135,7 @@ struct TmlpArgs {
 // symbol, 8 consecutive features = 4 triplets per lane), W1 chunks staged in
 // LDS the B operand.  TRAIN adds the loss and the backward epilogue.
 template <int M, bool TRAIN>
-__global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
+__global__ __launch_bounds__(TM_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_tmlp_fwd(TmlpArgs a) {
     __shared__ cx<float> xw[M * TM_WIN];
     __shared__ cx<float> qt2[2 * TM_QW];  // the pair-table rows of the current chunk
     __shared__ __align__(16) unsigned short w1s[TM_H * TM_LDW];
@@ -337,7 +337,8 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_fwd(TmlpArgs a) {
 // unit, 8 consecutive symbols per lane); B operand: the lane's triplet over
 // the same 8 symbols, real and imaginary parts as two column tiles.
 template <int M>
-__global__ __launch_bounds__(TM_NT) void k_tmlp_dw1(TmlpArgs a, int splits) {
+__global__ __launch_bounds__(TM_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_tmlp_dw1(TmlpArgs a,
+                                                                                            int splits) {
     __shared__ cx<float> xw[M * TM_WIN];
     __shared__ cx<float> qt[TM_W * TM_QW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, cl = lane & 31, hl = lane >> 5;
@@ -345,6 +346,7 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_dw1(TmlpArgs a, int splits) {
     const int t = 32 * tb + cl;
     const bool live = t < TM_T;
     const int tm = t / TM_W - TM_R, tn = t % TM_W - TM_R;
+    const int tmc = live ? tm : 0, tnc = live ? tn : 0;
     const int split = blockIdx.y;
     const int b_lo = (int)((long long)a.blocks * split / splits), b_hi = (int)((long long)a.blocks * (split + 1) / splits);
     tm_f16 accr[4], acci[4];
@@ -360,30 +362,30 @@ __global__ __launch_bounds__(TM_NT) void k_tmlp_dw1(TmlpArgs a, int splits) {
         __syncthreads();
         const cx<float>* xm = xw + md * TM_WIN;
         if (tb < TM_TP / 32) {
-#pragma unroll 2
+#pragma unroll 1
             for (int ks = 0; ks < TM_ROWS / 16; ++ks) {
                 const int r = 16 * ks + 8 * hl;  // first of the lane's 8 symbols
+                // the A operands first: their loads fly while the triplets form
+                uint4 av[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    av[q] = *reinterpret_cast<const uint4*>(a.dpre + (size_t)(32 * q + cl) * a.rows + row0 + r);
                 uint4 br, bi;
                 unsigned* pr = reinterpret_cast<unsigned*>(&br);
                 unsigned* pi = reinterpret_cast<unsigned*>(&bi);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    unsigned lr = 0, li = 0;
-                    if (live) {
-                        const cx<float> T0 = tm_triplet(qt, xm, 2 * TM_R + r + 2 * u, tm, tn);
-                        const cx<float> T1 = tm_triplet(qt, xm, 2 * TM_R + r + 2 * u + 1, tm, tn);
-                        lr = tm_pack2(T0.x, T1.x);
-                        li = tm_pack2(T0.y, T1.y);
-                    }
-                    pr[u] = lr;
-                    pi[u] = li;
+                    // branch-free: a padding lane (t >= 1089) reads the table
+                    // at m = n = 0 and is zeroed
+                    const cx<float> T0 = tm_triplet(qt, xm, 2 * TM_R + r + 2 * u, tmc, tnc);
+                    const cx<float> T1 = tm_triplet(qt, xm, 2 * TM_R + r + 2 * u + 1, tmc, tnc);
+                    pr[u] = live ? tm_pack2(T0.x, T1.x) : 0u;
+                    pi[u] = live ? tm_pack2(T0.y, T1.y) : 0u;
                 }
                 const tm_bf8 Br = tm_frag(br), Bi = tm_frag(bi);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const uint4 av =
-                        *reinterpret_cast<const uint4*>(a.dpre + (size_t)(32 * q + cl) * a.rows + row0 + r);
-                    const tm_bf8 A = tm_frag(av);
+                    const tm_bf8 A = tm_frag(av[q]);
                     accr[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Br, accr[q], 0, 0, 0);
                     acci[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bi, acci[q], 0, 0, 0);
                 }
