@@ -172,3 +172,57 @@ def test_tmlp_gain_over_dbp():
     res = TM.run(TM.parser().parse_args(["--rate-batch", "32", "--reps", "1"]), log=print)
     assert res["mlp_gain_db"] > 0.15
     assert res["train"]["loss_last"] < 0.7 * res["train"]["loss_first"]
+
+
+def test_tmlp_full_size_batch_invariance(ctx):
+    """Inference at the full size, 512 waveforms x 8192 symbols: each
+    workgroup's rows are independent, so waveforms run alone reproduce their
+    rows of the batch bit for bit; one of them also against the oracle."""
+    import focss_amd as F
+    S, B = 8192, 512
+    rng = np.random.default_rng(31)
+    lv = np.array([-3, -1, 1, 3]) / np.sqrt(10)
+    x = ((rng.choice(lv, (B, S)) + 1j * rng.choice(lv, (B, S))) * (1 + 0.05 * rng.normal(size=(B, S))))
+    x = x.astype(np.complex64)
+    p = F.TripletMLP.init_params(2, w2_scale=0.3)
+    p[H * KF:H * KF + H] = rng.normal(0, 0.1, H).astype(np.float32)
+    mlp = F.TripletMLP(ctx, 1, S, B, params=p)
+    xb, ob = _up(ctx, x.ravel()), ctx.alloc(x.nbytes)
+    mlp.forward(xb, ob)
+    out = ob.download(np.empty(B * S, dtype=np.complex64)).reshape(B, S)
+    for i in (0, 257, 511):
+        xi, oi = _up(ctx, np.ascontiguousarray(x[i])), ctx.alloc(S * 8)
+        mlp.forward(xi, oi, batch=1)
+        assert np.array_equal(oi.download(np.empty(S, dtype=np.complex64)), out[i]), i
+        xi.free()
+        oi.free()
+    x64 = x[257].astype(np.complex128)
+    want, _ = O.tmlp_forward(p.astype(np.float64), x64, 1, S, bf16=True)
+    assert _rel(out[257].astype(np.complex128) - x64, want - x64) <= 2e-4
+    for b in (mlp, xb, ob):
+        b.free()
+
+
+def test_tmlp_gradient_is_the_mean_over_waveforms(ctx):
+    """The loss and gradient of a training batch (16 x 8192 symbols) equal
+    the mean of those of its waveforms taken one at a time (the loss is a
+    mean over rows): linearity over the batch, at the training size."""
+    import focss_amd as F
+    S, B = 8192, 16
+    x, tgt, p = _case(41, 1, S, B)
+    mlp = F.TripletMLP(ctx, 1, S, B, params=p)
+    xb, tb = _up(ctx, x), _up(ctx, tgt)
+    loss, g = mlp.gradients(xb, tb)
+    one = F.TripletMLP(ctx, 1, S, 1, params=p)
+    ls, gs = 0.0, np.zeros_like(g)
+    for i in range(B):
+        xi, ti = _up(ctx, np.ascontiguousarray(x[i * S:(i + 1) * S])), _up(ctx, np.ascontiguousarray(tgt[i * S:(i + 1) * S]))
+        li, gi = one.gradients(xi, ti)
+        ls += li / B
+        gs += gi / B
+        xi.free()
+        ti.free()
+    assert abs(loss - ls) <= 1e-6 * ls
+    assert _rel(g, gs) <= 1e-5
+    for b in (mlp, one, xb, tb):
+        b.free()
