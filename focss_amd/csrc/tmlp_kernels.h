@@ -190,6 +190,8 @@ __global__ __launch_bounds__(TM_NT) __attribute__((amdgpu_waves_per_eu(2))) void
         const int m0 = t0 / TM_W, n0 = t0 - TM_W * m0;  // m = m0 - R (+1 on carry), n likewise
         const int qbase = (m0 - TM_R - m_lo) * TM_QW + c_sym + n0 - 2 * TM_R;
         const int xbase = c_sym + m0 - TM_R;
+        // x_md(s + m) takes two values in a chunk: m and m + 1 after a carry
+        const cx<float> u_lo = xm[xbase], u_hi = xm[xbase + 1];
 #pragma unroll
         for (int ks = 0; ks < TM_KC / 16; ++ks) {
             uint4 av;
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(TM_NT) __attribute__((amdgpu_waves_per_eu(2))) void
                 const int d = 8 * ks + q;
                 const bool carry = n0 + d >= TM_W;
                 const cx<float> qv = qt2[qbase + d + (carry ? TM_QW - TM_W : 0)];
-                const cx<float> u = xm[xbase + (carry ? 1 : 0)];
+                const cx<float> u = carry ? u_hi : u_lo;
                 const unsigned v = tm_pack2(u.x * qv.x - u.y * qv.y, u.x * qv.y + u.y * qv.x);
                 ap[q] = t0 + d < TM_T ? v : 0u;
             }
