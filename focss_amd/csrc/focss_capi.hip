@@ -1740,10 +1740,18 @@ int focss_tmlp_forward(focss_ctx* c, int modes, int symbols, int batch, const fl
     if (int rc = tmlp_pack(c, params)) return rc;
     TmlpArgs a = tmlp_args(c, modes, symbols, batch, params, in);
     a.out = (cx<float>*)out;
-    if (modes == 1)
+    if (symbols % TI_ROWS == 0) {
+        // 256-symbol blocks, two row tiles per wave
+        const dim3 grid(a.rows / TI_ROWS);
+        if (modes == 1)
+            hipLaunchKernelGGL(k_tmlp_infer<1>, grid, dim3(TM_NT), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_tmlp_infer<2>, grid, dim3(TM_NT), 0, c->stream, a);
+    } else if (modes == 1) {
         hipLaunchKernelGGL((k_tmlp_fwd<1, false>), dim3(a.blocks), dim3(TM_NT), 0, c->stream, a);
-    else
+    } else {
         hipLaunchKernelGGL((k_tmlp_fwd<2, false>), dim3(a.blocks), dim3(TM_NT), 0, c->stream, a);
+    }
     HIP_TRY(hipGetLastError());
     if (!target) return 0;
     if (!c->tmlp_acc) HIP_TRY(hipMalloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));

@@ -74,15 +74,15 @@ __device__ __forceinline__ void tm_build_pairs(cx<float>* qt, const cx<float>* x
 
 // The rows m_lo and m_lo + 1 of the pair table (the m values of one chunk of
 // 32 consecutive triplets): qt2[(m - m_lo) TM_QW + c - R].
-template <int M>
+template <int M, int WIN = TM_WIN, int QW = TM_QW>
 __device__ __forceinline__ void tm_build_pairs2(cx<float>* qt2, const cx<float>* xw, int m_lo) {
-    for (int i = threadIdx.x; i < 2 * TM_QW; i += TM_NT) {
-        const int m = m_lo + i / TM_QW, c = i % TM_QW + TM_R;
+    for (int i = threadIdx.x; i < 2 * QW; i += TM_NT) {
+        const int m = m_lo + i / QW, c = i % QW + TM_R;
         if (m > TM_R) continue;
         cx<float> acc = mk<float>(0.0f, 0.0f);
 #pragma unroll
         for (int p = 0; p < M; ++p) {
-            const cx<float> a = xw[p * TM_WIN + c], b = xw[p * TM_WIN + c + m];
+            const cx<float> a = xw[p * WIN + c], b = xw[p * WIN + c + m];
             acc.x = fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x));
             acc.y = fmaf(a.y, b.x, fmaf(-a.x, b.y, acc.y));
         }
@@ -96,13 +96,13 @@ __device__ __forceinline__ cx<float> tm_triplet(const cx<float>* qt, const cx<fl
     return mk<float>(u.x * q.x - u.y * q.y, u.x * q.y + u.y * q.x);
 }
 
-// every mode of waveform seq / M, symbols s0 - 2R .. s0 + TM_ROWS + 2R - 1 (mod S)
-template <int M>
+// every mode of waveform seq / M, symbols s0 - 2R .. s0 + WIN - 2R - 1 (mod S)
+template <int M, int WIN = TM_WIN>
 __device__ __forceinline__ void tm_load_window(cx<float>* xw, const cx<float>* __restrict__ x, int seq, int s0,
                                                int S) {
     const long long base = (long long)(seq / M) * M * S;
-    for (int i = threadIdx.x; i < M * TM_WIN; i += TM_NT) {
-        const int p = i / TM_WIN, k = i % TM_WIN;
+    for (int i = threadIdx.x; i < M * WIN; i += TM_NT) {
+        const int p = i / WIN, k = i % WIN;
         const int s = (s0 - 2 * TM_R + k + S) % S;
         xw[i] = x[base + (long long)p * S + s];
     }
@@ -330,6 +330,153 @@ __global__ __launch_bounds__(TM_NT) __attribute__((amdgpu_waves_per_eu(2))) void
         for (int q = threadIdx.x; q < TM_SMALL; q += TM_NT)
             a.small[(size_t)q * a.blocks + blk] = red[q] + red[TM_SMALL + q] + red[2 * TM_SMALL + q] +
                                                   red[3 * TM_SMALL + q];
+    }
+}
+
+// Inference on blocks of 256 symbols: as k_tmlp_fwd<M, false>, but each wave
+// owns two 32-symbol row tiles (symbols 64 w .. 64 w + 63), so every W1
+// operand read from LDS feeds two matrix-core products.
+constexpr int TI_ROWS = 2 * TM_ROWS;        // symbols per workgroup
+constexpr int TI_WIN = TI_ROWS + 4 * TM_R;  // window
+constexpr int TI_QW = TI_ROWS + 2 * TM_R;   // pair-table positions
+
+template <int M>
+__global__ __launch_bounds__(TM_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_tmlp_infer(TmlpArgs a) {
+    __shared__ cx<float> xw[M * TI_WIN];
+    __shared__ cx<float> qt2[2 * TI_QW];
+    __shared__ __align__(16) unsigned short w1s[TM_H * TM_LDW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, cl = lane & 31, hl = lane >> 5;
+    const int row0 = blockIdx.x * TI_ROWS;
+    const int seq = row0 / a.S, s0 = row0 % a.S, md = seq % M;
+    tm_load_window<M, TI_WIN>(xw, a.x, seq, s0, a.S);
+    uint4 pw0, pw1, pw2, pw3;
+    auto w1_src = [&](int c, int q) {
+        const int idx = threadIdx.x + TM_NT * q;
+        return reinterpret_cast<const uint4*>(a.w1b + (size_t)(idx >> 3) * TM_KF + TM_KC * c + 8 * (idx & 7));
+    };
+    auto w1_dst = [&](int q) {
+        const int idx = threadIdx.x + TM_NT * q;
+        return reinterpret_cast<uint4*>(w1s + (idx >> 3) * TM_LDW + 8 * (idx & 7));
+    };
+    auto fetch = [&](int c) {
+        pw0 = *w1_src(c, 0);
+        pw1 = *w1_src(c, 1);
+        pw2 = *w1_src(c, 2);
+        pw3 = *w1_src(c, 3);
+    };
+    auto stage = [&]() {
+        *w1_dst(0) = pw0;
+        *w1_dst(1) = pw1;
+        *w1_dst(2) = pw2;
+        *w1_dst(3) = pw3;
+    };
+    auto chunk_m = [](int c) { return (TM_KC / 2) * c / TM_W - TM_R; };
+    fetch(0);
+    stage();
+    __syncthreads();  // window loaded
+    tm_build_pairs2<M, TI_WIN, TI_QW>(qt2, xw, chunk_m(0));
+    const cx<float>* xm = xw + md * TI_WIN;
+    tm_f16 acc[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[rt][t] = tm_f16{};
+    const int c_sym0 = 2 * TM_R + 64 * w + cl;  // window position of the lane's symbol in row tile 0
+    constexpr int NCH = TM_KF / TM_KC;
+    __syncthreads();
+    for (int c = 0; c < NCH; ++c) {
+        if (c + 1 < NCH) fetch(c + 1);
+        const int m_lo = chunk_m(c);
+        const int t0 = (TM_KC / 2) * c + 4 * hl;
+        const int m0 = t0 / TM_W, n0 = t0 - TM_W * m0;
+        // row tile rt sits 32 rt window positions further on
+        const int qbase = (m0 - TM_R - m_lo) * TI_QW + c_sym0 + n0 - 2 * TM_R;
+        const int xbase = c_sym0 + m0 - TM_R;
+        cx<float> u_lo[2], u_hi[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+            u_lo[rt] = xm[xbase + 32 * rt];
+            u_hi[rt] = xm[xbase + 32 * rt + 1];
+        }
+#pragma unroll
+        for (int ks = 0; ks < TM_KC / 16; ++ks) {
+            tm_bf8 A[2];
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                uint4 av;
+                unsigned* ap = reinterpret_cast<unsigned*>(&av);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    // branch-free: the padding slots t >= 1089 read in-bounds LDS and are zeroed
+                    const int d = 8 * ks + q;
+                    const bool carry = n0 + d >= TM_W;
+                    const cx<float> qv = qt2[qbase + 32 * rt + d + (carry ? TI_QW - TM_W : 0)];
+                    const cx<float> u = carry ? u_hi[rt] : u_lo[rt];
+                    const unsigned v = tm_pack2(u.x * qv.x - u.y * qv.y, u.x * qv.y + u.y * qv.x);
+                    ap[q] = t0 + d < TM_T ? v : 0u;
+                }
+                A[rt] = tm_frag(av);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const tm_bf8 B =
+                    tm_frag(*reinterpret_cast<const uint4*>(w1s + (32 * t + cl) * TM_LDW + 16 * ks + 8 * hl));
+                acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0], B, acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1], B, acc[1][t], 0, 0, 0);
+            }
+        }
+        if (c + 1 < NCH) {
+            __syncthreads();  // every wave is done with chunk c
+            stage();
+            tm_build_pairs2<M, TI_WIN, TI_QW>(qt2, xw, chunk_m(c + 1));
+            __syncthreads();
+        }
+    }
+    float b1v[4], w20[4], w21[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        b1v[t] = a.b1[32 * t + cl];
+        w20[t] = a.w2[32 * t + cl];
+        w21[t] = a.w2[TM_H + 32 * t + cl];
+    }
+    const float b20 = a.b2[0], b21 = a.b2[1];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+        // lane holds pre-activations of hidden j = 32 t + cl for the symbols
+        // r(v) = 8 (v / 4) + 4 hl + v % 4 of row tile rt
+        float o0[16], o1[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            float s0v = 0.0f, s1v = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float h = fmaxf(acc[rt][t][v] + b1v[t], 0.0f);
+                s0v = fmaf(w20[t], h, s0v);
+                s1v = fmaf(w21[t], h, s1v);
+            }
+            o0[v] = s0v;
+            o1[v] = s1v;
+        }
+#pragma unroll
+        for (int off = 1; off < 32; off <<= 1)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                o0[v] += __shfl_xor(o0[v], off, 64);
+                o1[v] += __shfl_xor(o1[v], off, 64);
+            }
+        if (cl < 16) {
+            const int v = cl, r = 8 * (v >> 2) + 4 * hl + (v & 3);
+            float y0 = 0.0f, y1 = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (u == v) {
+                    y0 = o0[u];
+                    y1 = o1[u];
+                }
+            const int pos = 64 * w + 32 * rt + r;  // symbol of the block
+            const cx<float> xv = xm[2 * TM_R + pos];
+            a.out[(long long)row0 + pos] = mk<float>(xv.x + y0 + b20, xv.y + y1 + b21);
+        }
     }
 }
 

@@ -80,8 +80,10 @@ def _rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("modes,S,B", [(1, 256, 2), (2, 128, 2), (1, 384, 1)])
+@pytest.mark.parametrize("modes,S,B", [(1, 256, 2), (2, 256, 1), (2, 128, 2), (1, 384, 1)])
 def test_tmlp_forward(ctx, modes, S, B):
+    """Symbol counts that are multiples of 256 run the two-row-tile inference
+    kernel, the others (128, 384) the 128-symbol one."""
     import focss_amd as F
     x, tgt, p = _case(1 + modes + S, modes, S, B)
     mlp = F.TripletMLP(ctx, modes, S, B, params=p)
