@@ -404,6 +404,10 @@ typedef float ldbp_f16 __attribute__((ext_vector_type(16)));
 struct cx_h {
     _Float16 x, y;
 };
+// two consecutive samples (rows 2 rp and 2 rp + 1 of one block)
+struct __align__(8) cx_h2 {
+    cx_h lo, hi;
+};
 
 constexpr int LI_NT = 1024;  // 16 waves; each wave owns up to 2 tiles of 512 outputs
 
@@ -437,17 +441,22 @@ __global__ void k_ldbp_afrag(const cx<float>* __restrict__ fir, int J, ldbp_h8* 
     afrag[i] = A;
 }
 
-// M modes: LDS holds the field [M][16][nb] (FP16) and each mode's
-// intensities [M][16][nb]; the M N / 512 tiles (at most 32: M N <= 16384)
-// are dealt over the 16 waves, and the nonlinear phase sums the modes.
+// M modes: LDS holds the field (FP16) and each mode's intensities, both as
+// [M][8][nb] pairs of consecutive samples: pair (rp, blk) holds samples
+// 16 blk + 2 rp and 16 blk + 2 rp + 1, so one 8-byte access moves two samples
+// and neighbouring lanes (blocks) stay in neighbouring banks.  The M N / 512
+// tiles (at most 32: M N <= 16384) are dealt over the 16 waves, and the
+// nonlinear phase sums the modes.
 template <int K, int KP, int M>
 __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8* __restrict__ afrag) {
     using IS = InferShape<K>;
     constexpr int c = IS::c, DELTA = IS::DELTA, NKS = IS::NKS, P = KP / 2;
+    constexpr int C0 = c - (K - 1) - DELTA;  // operand window start relative to 16 blk, half 0
+    static_assert(C0 % 4 == 0, "four operand samples start on an even row of one block");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int N = a.N, nb = N >> 4, lnb = __builtin_ctz(N >> 4);
-    cx_h* xs = reinterpret_cast<cx_h*>(smem_raw);                                     // [M][16][nb]
-    float* pb = reinterpret_cast<float*>(smem_raw + (size_t)M * N * sizeof(cx_h));   // [M][16][nb]
+    const int N = a.N, nb = N >> 4, lnb = __builtin_ctz(N >> 4), NP = N >> 1;
+    cx_h2* xs = reinterpret_cast<cx_h2*>(smem_raw);                                   // [M][8][nb]
+    float2* pb = reinterpret_cast<float2*>(smem_raw + (size_t)M * N * sizeof(cx_h));  // [M][8][nb]
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int hl = lane >> 5, col = lane & 31;
     const int mode_tiles = N / 512, n_tiles = M * mode_tiles;
@@ -455,10 +464,10 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
     const cx<float>* in = a.in + (long long)b * M * N;
 #pragma unroll
     for (int md = 0; md < M; ++md)
-        for (int p = threadIdx.x; p < N; p += LI_NT) {
-            const int t = ((p & (nb - 1)) << 4) + (p >> lnb);  // sample stored at position p
-            const cx<float> v = in[(long long)md * N + t];
-            xs[md * N + p] = cx_h{(_Float16)v.x, (_Float16)v.y};
+        for (int q = threadIdx.x; q < NP; q += LI_NT) {
+            const int t = ((q & (nb - 1)) << 4) + ((q >> lnb) << 1);  // first sample of pair q
+            const cx<float> v0 = in[(long long)md * N + t], v1 = in[(long long)md * N + t + 1];
+            xs[md * NP + q] = cx_h2{cx_h{(_Float16)v0.x, (_Float16)v0.y}, cx_h{(_Float16)v1.x, (_Float16)v1.y}};
         }
     __syncthreads();
     for (int j = 0; j < a.J; ++j) {
@@ -474,27 +483,29 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             if (tile < n_tiles) {
                 // the lane's column and half, opaque to the compiler within the
                 // step: otherwise it hoists every gather address of the step
-                // (2 tiles x NKS x 4 per lane) out of the step loop and spills
-                // them to scratch
+                // out of the step loop and spills them to scratch
                 int colj = col, hlj = hl;
                 asm volatile("" : "+v"(colj), "+v"(hlj));
-                const cx_h* xm = xs + (tile / mode_tiles) * N;
+                const cx_h2* xm = xs + (tile / mode_tiles) * NP;
                 const int blk = (tile % mode_tiles) * 32 + colj;
-                constexpr int C0 = c - (K - 1) - DELTA;  // window start relative to 16 blk, half 0
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) {
+                    // samples 16 blk + o .. + 3, o = C0 + 4 hl + 8 s: rows o & 15
+                    // .. + 3 (pairs (o & 15) / 2 and + 1) of block blk + (o >> 4)
+                    const int oa = C0 + 8 * s, ob = oa + 4;
+                    const int rp = hlj ? ((ob & 15) >> 1) : ((oa & 15) >> 1);
+                    const int bo = hlj ? (ob >> 4) : (oa >> 4);
+                    const int q = (rp << lnb) + ((blk + bo) & (nb - 1));
+                    const cx_h2 p0 = xm[q], p1 = xm[q + nb];
                     ldbp_h8 bf;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        // sample 16 blk + o = row o & 15 of block blk + (o >> 4),
-                        // o = C0 + 4 hl + 8 s + v
-                        const int oa = C0 + 8 * s + v, ob = oa + 4;
-                        const int row = hlj ? (ob & 15) : (oa & 15);
-                        const int bo = hlj ? (ob >> 4) : (oa >> 4);
-                        const cx_h xv = xm[(row << lnb) + ((blk + bo) & (nb - 1))];
-                        bf[2 * v] = xv.x;
-                        bf[2 * v + 1] = xv.y;
-                    }
+                    bf[0] = p0.lo.x;
+                    bf[1] = p0.lo.y;
+                    bf[2] = p0.hi.x;
+                    bf[3] = p0.hi.y;
+                    bf[4] = p1.lo.x;
+                    bf[5] = p1.lo.y;
+                    bf[6] = p1.hi.x;
+                    bf[7] = p1.hi.y;
                     acc[it] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[s], bf, acc[it], 0, 0, 0);
                 }
             }
@@ -505,17 +516,17 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
         for (int it = 0; it < 2; ++it) {
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
-                const int mo = (tile / mode_tiles) * N;
+                const int mo = (tile / mode_tiles) * NP;
                 const int blk = (tile % mode_tiles) * 32 + col;
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int jj = 0; jj < 2; ++jj) {
-                        const int row = 4 * i + 2 * hl + jj;
-                        const float re = acc[it][4 * i + 2 * jj], im = acc[it][4 * i + 2 * jj + 1];
-                        xs[mo + (row << lnb) + blk] = cx_h{(_Float16)re, (_Float16)im};
-                        pb[mo + (row << lnb) + blk] = re * re + im * im;
-                    }
+                for (int i = 0; i < 4; ++i) {
+                    // rows 4 i + 2 hl and 4 i + 2 hl + 1: pair 2 i + hl
+                    const float r0 = acc[it][4 * i], i0 = acc[it][4 * i + 1];
+                    const float r1 = acc[it][4 * i + 2], i1 = acc[it][4 * i + 3];
+                    const int q = mo + ((2 * i + hl) << lnb) + blk;
+                    xs[q] = cx_h2{cx_h{(_Float16)r0, (_Float16)i0}, cx_h{(_Float16)r1, (_Float16)i1}};
+                    pb[q] = make_float2(r0 * r0 + i0 * i0, r1 * r1 + i1 * i1);
+                }
             }
         }
         __syncthreads();
@@ -525,48 +536,58 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
 #pragma unroll
         for (int mm = 0; mm < KP; ++mm) g[mm] = sload(a.kerr + j * KP + mm);
         for (int blk = threadIdx.x; blk < nb; blk += LI_NT) {
+            // intensity of row r of block bl, summed over the modes
+            auto inten = [&](int r, int bl) {
+                float v = 0.0f;
+#pragma unroll
+                for (int md = 0; md < M; ++md) {
+                    const float2 pp = pb[md * NP + ((r >> 1) << lnb) + bl];
+                    v += (r & 1) ? pp.y : pp.x;
+                }
+                return v;
+            };
             float pe[16 + 2 * P];  // p(16 blk - P + u), summed over the modes
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                pe[P + r] = pb[(r << lnb) + blk];
-#pragma unroll
-                for (int md = 1; md < M; ++md) pe[P + r] += pb[md * N + (r << lnb) + blk];
-            }
+            for (int r = 0; r < 16; ++r) pe[P + r] = inten(r, blk);
             if constexpr (P > 0) {
                 const int bp = (blk + nb - 1) & (nb - 1), bn = (blk + 1) & (nb - 1);
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
-                    pe[i] = pb[((16 - P + i) << lnb) + bp];
-                    pe[P + 16 + i] = pb[(i << lnb) + bn];
-#pragma unroll
-                    for (int md = 1; md < M; ++md) {
-                        pe[i] += pb[md * N + ((16 - P + i) << lnb) + bp];
-                        pe[P + 16 + i] += pb[md * N + (i << lnb) + bn];
-                    }
+                    pe[i] = inten(16 - P + i, bp);
+                    pe[P + 16 + i] = inten(i, bn);
                 }
             }
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float phi = 0.0f;
+            for (int rp = 0; rp < 8; ++rp) {
+                float sn[2], cs[2];
 #pragma unroll
-                for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
-                // the hardware sine and cosine (v_sin_f32 / v_cos_f32, ~1e-6
-                // absolute): the field itself is held in FP16 here
-                float sn, cs;
-                if (fabsf(phi) <= 1000.0f) {
-                    sn = __sinf(phi);
-                    cs = __cosf(phi);
-                } else {
-                    sincos_fast(phi, &sn, &cs);
+                for (int e = 0; e < 2; ++e) {
+                    const int r = 2 * rp + e;
+                    float phi = 0.0f;
+#pragma unroll
+                    for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
+                    // the hardware sine and cosine (v_sin_f32 / v_cos_f32, ~1e-6
+                    // absolute): the field itself is held in FP16 here
+                    if (fabsf(phi) <= 1000.0f) {
+                        sn[e] = __sinf(phi);
+                        cs[e] = __cosf(phi);
+                    } else {
+                        sincos_fast(phi, &sn[e], &cs[e]);
+                    }
                 }
 #pragma unroll
                 for (int md = 0; md < M; ++md) {
-                    const cx_h yh = xs[md * N + (r << lnb) + blk];
-                    const cx<float> z = mk<float>((float)yh.x, (float)yh.y) * mk<float>(cs, sn);
-                    if (last)
-                        a.out[((long long)b * M + md) * N + 16 * blk + r] = z;
-                    else
-                        xs[md * N + (r << lnb) + blk] = cx_h{(_Float16)z.x, (_Float16)z.y};
+                    const int q = md * NP + (rp << lnb) + blk;
+                    const cx_h2 yh = xs[q];
+                    const cx<float> z0 = mk<float>((float)yh.lo.x, (float)yh.lo.y) * mk<float>(cs[0], sn[0]);
+                    const cx<float> z1 = mk<float>((float)yh.hi.x, (float)yh.hi.y) * mk<float>(cs[1], sn[1]);
+                    if (last) {
+                        cx<float>* o = a.out + ((long long)b * M + md) * N + 16 * blk + 2 * rp;
+                        o[0] = z0;
+                        o[1] = z1;
+                    } else {
+                        xs[q] = cx_h2{cx_h{(_Float16)z0.x, (_Float16)z0.y}, cx_h{(_Float16)z1.x, (_Float16)z1.y}};
+                    }
                 }
             }
         }
