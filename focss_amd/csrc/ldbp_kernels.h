@@ -404,9 +404,14 @@ typedef float ldbp_f16 __attribute__((ext_vector_type(16)));
 struct cx_h {
     _Float16 x, y;
 };
-// two consecutive samples (rows 2 rp and 2 rp + 1 of one block)
+// two consecutive samples
 struct __align__(8) cx_h2 {
     cx_h lo, hi;
+};
+// four consecutive samples (rows 4 rq .. 4 rq + 3 of one block); their bytes
+// are the FP16 B operand of one matrix-core product
+struct __align__(16) cx_h4 {
+    cx_h s[4];
 };
 
 constexpr int LI_NT = 1024;  // 16 waves; each wave owns up to 2 tiles of 512 outputs
@@ -442,21 +447,21 @@ __global__ void k_ldbp_afrag(const cx<float>* __restrict__ fir, int J, ldbp_h8* 
 }
 
 // M modes: LDS holds the field (FP16) and each mode's intensities, both as
-// [M][8][nb] pairs of consecutive samples: pair (rp, blk) holds samples
-// 16 blk + 2 rp and 16 blk + 2 rp + 1, so one 8-byte access moves two samples
-// and neighbouring lanes (blocks) stay in neighbouring banks.  The M N / 512
-// tiles (at most 32: M N <= 16384) are dealt over the 16 waves, and the
-// nonlinear phase sums the modes.
+// [M][4][nb] quads of consecutive samples: quad (rq, blk) holds samples
+// 16 blk + 4 rq .. 16 blk + 4 rq + 3, so one 16-byte read is one operand of
+// the FIR product and neighbouring lanes (blocks) stay in neighbouring banks.
+// The M N / 512 tiles (at most 32: M N <= 16384) are dealt over the 16 waves,
+// and the nonlinear phase sums the modes.
 template <int K, int KP, int M>
 __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8* __restrict__ afrag) {
     using IS = InferShape<K>;
     constexpr int c = IS::c, DELTA = IS::DELTA, NKS = IS::NKS, P = KP / 2;
     constexpr int C0 = c - (K - 1) - DELTA;  // operand window start relative to 16 blk, half 0
-    static_assert(C0 % 4 == 0, "four operand samples start on an even row of one block");
+    static_assert(C0 % 4 == 0, "the four samples of an operand are one quad");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int N = a.N, nb = N >> 4, lnb = __builtin_ctz(N >> 4), NP = N >> 1;
-    cx_h2* xs = reinterpret_cast<cx_h2*>(smem_raw);                                   // [M][8][nb]
-    float2* pb = reinterpret_cast<float2*>(smem_raw + (size_t)M * N * sizeof(cx_h));  // [M][8][nb]
+    const int N = a.N, nb = N >> 4, lnb = __builtin_ctz(N >> 4), NQ = N >> 2;
+    cx_h4* xs = reinterpret_cast<cx_h4*>(smem_raw);                                   // [M][4][nb]
+    float4* pb = reinterpret_cast<float4*>(smem_raw + (size_t)M * N * sizeof(cx_h));  // [M][4][nb]
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int hl = lane >> 5, col = lane & 31;
     const int mode_tiles = N / 512, n_tiles = M * mode_tiles;
@@ -464,10 +469,15 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
     const cx<float>* in = a.in + (long long)b * M * N;
 #pragma unroll
     for (int md = 0; md < M; ++md)
-        for (int q = threadIdx.x; q < NP; q += LI_NT) {
-            const int t = ((q & (nb - 1)) << 4) + ((q >> lnb) << 1);  // first sample of pair q
-            const cx<float> v0 = in[(long long)md * N + t], v1 = in[(long long)md * N + t + 1];
-            xs[md * NP + q] = cx_h2{cx_h{(_Float16)v0.x, (_Float16)v0.y}, cx_h{(_Float16)v1.x, (_Float16)v1.y}};
+        for (int q = threadIdx.x; q < NQ; q += LI_NT) {
+            const int t = ((q & (nb - 1)) << 4) + ((q >> lnb) << 2);  // first sample of quad q
+            cx_h4 h;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const cx<float> v = in[(long long)md * N + t + e];
+                h.s[e] = cx_h{(_Float16)v.x, (_Float16)v.y};
+            }
+            xs[md * NQ + q] = h;
         }
     __syncthreads();
     for (int j = 0; j < a.J; ++j) {
@@ -486,26 +496,16 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
                 // out of the step loop and spills them to scratch
                 int colj = col, hlj = hl;
                 asm volatile("" : "+v"(colj), "+v"(hlj));
-                const cx_h2* xm = xs + (tile / mode_tiles) * NP;
+                const cx_h4* xm = xs + (tile / mode_tiles) * NQ;
                 const int blk = (tile % mode_tiles) * 32 + colj;
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) {
-                    // samples 16 blk + o .. + 3, o = C0 + 4 hl + 8 s: rows o & 15
-                    // .. + 3 (pairs (o & 15) / 2 and + 1) of block blk + (o >> 4)
+                    // samples 16 blk + o .. + 3, o = C0 + 4 hl + 8 s: quad
+                    // (o & 15) / 4 of block blk + (o >> 4)
                     const int oa = C0 + 8 * s, ob = oa + 4;
-                    const int rp = hlj ? ((ob & 15) >> 1) : ((oa & 15) >> 1);
+                    const int rq = hlj ? ((ob & 15) >> 2) : ((oa & 15) >> 2);
                     const int bo = hlj ? (ob >> 4) : (oa >> 4);
-                    const int q = (rp << lnb) + ((blk + bo) & (nb - 1));
-                    const cx_h2 p0 = xm[q], p1 = xm[q + nb];
-                    ldbp_h8 bf;
-                    bf[0] = p0.lo.x;
-                    bf[1] = p0.lo.y;
-                    bf[2] = p0.hi.x;
-                    bf[3] = p0.hi.y;
-                    bf[4] = p1.lo.x;
-                    bf[5] = p1.lo.y;
-                    bf[6] = p1.hi.x;
-                    bf[7] = p1.hi.y;
+                    const ldbp_h8 bf = __builtin_bit_cast(ldbp_h8, xm[(rq << lnb) + ((blk + bo) & (nb - 1))]);
                     acc[it] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[s], bf, acc[it], 0, 0, 0);
                 }
             }
@@ -516,16 +516,17 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
         for (int it = 0; it < 2; ++it) {
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
-                const int mo = (tile / mode_tiles) * NP;
+                const int mo = (tile / mode_tiles) * NQ;
                 const int blk = (tile % mode_tiles) * 32 + col;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    // rows 4 i + 2 hl and 4 i + 2 hl + 1: pair 2 i + hl
+                    // rows 4 i + 2 hl and 4 i + 2 hl + 1: half hl of quad i
                     const float r0 = acc[it][4 * i], i0 = acc[it][4 * i + 1];
                     const float r1 = acc[it][4 * i + 2], i1 = acc[it][4 * i + 3];
-                    const int q = mo + ((2 * i + hl) << lnb) + blk;
-                    xs[q] = cx_h2{cx_h{(_Float16)r0, (_Float16)i0}, cx_h{(_Float16)r1, (_Float16)i1}};
-                    pb[q] = make_float2(r0 * r0 + i0 * i0, r1 * r1 + i1 * i1);
+                    const int q = mo + (i << lnb) + blk;
+                    reinterpret_cast<cx_h2*>(xs + q)[hl] =
+                        cx_h2{cx_h{(_Float16)r0, (_Float16)i0}, cx_h{(_Float16)r1, (_Float16)i1}};
+                    reinterpret_cast<float2*>(pb + q)[hl] = make_float2(r0 * r0 + i0 * i0, r1 * r1 + i1 * i1);
                 }
             }
         }
@@ -536,33 +537,61 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
 #pragma unroll
         for (int mm = 0; mm < KP; ++mm) g[mm] = sload(a.kerr + j * KP + mm);
         for (int blk = threadIdx.x; blk < nb; blk += LI_NT) {
-            // intensity of row r of block bl, summed over the modes
-            auto inten = [&](int r, int bl) {
-                float v = 0.0f;
+            // intensities of quad rq of block bl, summed over the modes
+            auto inten4 = [&](int rq, int bl) {
+                float4 v = pb[(rq << lnb) + bl];
 #pragma unroll
-                for (int md = 0; md < M; ++md) {
-                    const float2 pp = pb[md * NP + ((r >> 1) << lnb) + bl];
-                    v += (r & 1) ? pp.y : pp.x;
+                for (int md = 1; md < M; ++md) {
+                    const float4 u = pb[md * NQ + (rq << lnb) + bl];
+                    v.x += u.x;
+                    v.y += u.y;
+                    v.z += u.z;
+                    v.w += u.w;
                 }
                 return v;
             };
             float pe[16 + 2 * P];  // p(16 blk - P + u), summed over the modes
 #pragma unroll
-            for (int r = 0; r < 16; ++r) pe[P + r] = inten(r, blk);
+            for (int rq = 0; rq < 4; ++rq) {
+                const float4 v = inten4(rq, blk);
+                pe[P + 4 * rq] = v.x;
+                pe[P + 4 * rq + 1] = v.y;
+                pe[P + 4 * rq + 2] = v.z;
+                pe[P + 4 * rq + 3] = v.w;
+            }
             if constexpr (P > 0) {
                 const int bp = (blk + nb - 1) & (nb - 1), bn = (blk + 1) & (nb - 1);
+                // rows 16 - P .. 15 of the block before, rows 0 .. P - 1 of the one after
+                float lo[16], hi[16];
+#pragma unroll
+                for (int rq = 0; rq < 4; ++rq) {
+                    if (4 * rq + 3 >= 16 - P) {
+                        const float4 v = inten4(rq, bp);
+                        lo[4 * rq] = v.x;
+                        lo[4 * rq + 1] = v.y;
+                        lo[4 * rq + 2] = v.z;
+                        lo[4 * rq + 3] = v.w;
+                    }
+                    if (4 * rq < P) {
+                        const float4 v = inten4(rq, bn);
+                        hi[4 * rq] = v.x;
+                        hi[4 * rq + 1] = v.y;
+                        hi[4 * rq + 2] = v.z;
+                        hi[4 * rq + 3] = v.w;
+                    }
+                }
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
-                    pe[i] = inten(16 - P + i, bp);
-                    pe[P + 16 + i] = inten(i, bn);
+                    pe[i] = lo[16 - P + i];
+                    pe[P + 16 + i] = hi[i];
                 }
             }
 #pragma unroll
-            for (int rp = 0; rp < 8; ++rp) {
-                float sn[2], cs[2];
+            for (int rq = 0; rq < 4; ++rq) {
+                float sn[4], cs[4];
 #pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const int r = 2 * rp + e;
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 4 * rq + e;
                     float phi = 0.0f;
 #pragma unroll
                     for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
@@ -577,16 +606,20 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
                 }
 #pragma unroll
                 for (int md = 0; md < M; ++md) {
-                    const int q = md * NP + (rp << lnb) + blk;
-                    const cx_h2 yh = xs[q];
-                    const cx<float> z0 = mk<float>((float)yh.lo.x, (float)yh.lo.y) * mk<float>(cs[0], sn[0]);
-                    const cx<float> z1 = mk<float>((float)yh.hi.x, (float)yh.hi.y) * mk<float>(cs[1], sn[1]);
+                    const int q = md * NQ + (rq << lnb) + blk;
+                    cx_h4 yh = xs[q];
+                    cx<float> z[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        z[e] = mk<float>((float)yh.s[e].x, (float)yh.s[e].y) * mk<float>(cs[e], sn[e]);
                     if (last) {
-                        cx<float>* o = a.out + ((long long)b * M + md) * N + 16 * blk + 2 * rp;
-                        o[0] = z0;
-                        o[1] = z1;
+                        cx<float>* o = a.out + ((long long)b * M + md) * N + 16 * blk + 4 * rq;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) o[e] = z[e];
                     } else {
-                        xs[q] = cx_h2{cx_h{(_Float16)z0.x, (_Float16)z0.y}, cx_h{(_Float16)z1.x, (_Float16)z1.y}};
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) yh.s[e] = cx_h{(_Float16)z[e].x, (_Float16)z[e].y};
+                        xs[q] = yh;
                     }
                 }
             }
