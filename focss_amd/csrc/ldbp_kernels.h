@@ -491,11 +491,7 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             acc[it] = ldbp_f16{};
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
-                // the lane's column and half, opaque to the compiler within the
-                // step: otherwise it hoists every gather address of the step
-                // out of the step loop and spills them to scratch
-                int colj = col, hlj = hl;
-                asm volatile("" : "+v"(colj), "+v"(hlj));
+                const int colj = col, hlj = hl;
                 const cx_h4* xm = xs + (tile / mode_tiles) * NQ;
                 const int blk = (tile % mode_tiles) * 32 + colj;
 #pragma unroll
