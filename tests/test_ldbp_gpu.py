@@ -94,12 +94,25 @@ def test_ldbp_adam_decreases_loss(ctx):
     assert losses[-1] < 0.2 * losses[0]
 
 
+def _tap_rounding(fir):
+    """Relative field error the FP16 operands' tap rounding adds: the L2 norm
+    of each step's tap error (the gain of the error filter on a white
+    field), the steps' independent errors combined as a root sum of squares
+    like the field's own FP16 rounding (round to nearest even, as the
+    kernels' (_Float16) conversion)."""
+    f16 = fir.real.astype(np.float16).astype(np.float64) + 1j * fir.imag.astype(np.float16).astype(np.float64)
+    return float(np.sqrt(sum(np.linalg.norm(f16[j] - fir[j]) ** 2 for j in range(fir.shape[0]))))
+
+
 @pytest.mark.parametrize("taps,ktaps,n,batch,steps", [(31, 9, 1 << 14, 2, 6), (63, 15, 8192, 2, 6), (5, 1, 2048, 3, 6),
-                                                      (15, 5, 2048, 2, 6), (47, 1, 4096, 1, 6), (31, 9, 4096, 2, 80)])
+                                                      (15, 5, 2048, 2, 6), (47, 1, 4096, 1, 6), (23, 15, 1024, 2, 6),
+                                                      (31, 9, 4096, 2, 80)])
 def test_ldbp_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
     """Matrix-core inference (FP16 operands, FP32 accumulation, FP16 field
     between steps) against the FP64 oracle: each step rounds the field to
-    FP16 (2^-11, ~2.8e-4 RMS relative), so the bound is 6e-4 sqrt(steps).
+    FP16 (2^-11, ~2.8e-4 RMS relative), so the bound is 6e-4 sqrt(steps),
+    plus the FP16 rounding of the taps themselves (_tap_rounding: the centre
+    taps near 1 can sit up to 4.9e-4 from their FP16 value).
     Well-conditioned chains: near-identity FIRs, small phases."""
     import focss_amd as F
     r = np.random.default_rng(42)
@@ -117,7 +130,7 @@ def test_ldbp_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
     model.forward(bx, bf)
     out = bo.download(np.empty_like(x))
     out32 = bf.download(np.empty_like(x))
-    tol = 6e-4 * np.sqrt(steps)
+    tol = 6e-4 * np.sqrt(steps) + _tap_rounding(fir)
     for b in range(batch):
         sl = slice(b * n, (b + 1) * n)
         if steps <= 6:
@@ -164,10 +177,12 @@ def test_ldbp_two_modes_forward_and_gradients(ctx, taps, ktaps, n, batch):
     assert np.linalg.norm(gt - gt_ref) / np.linalg.norm(gt_ref) < 2e-4
 
 
-@pytest.mark.parametrize("taps,ktaps,n,batch,steps", [(31, 9, 8192, 2, 6), (15, 1, 2048, 3, 6), (31, 9, 4096, 1, 60)])
+@pytest.mark.parametrize("taps,ktaps,n,batch,steps", [(31, 9, 8192, 2, 6), (15, 1, 2048, 3, 6), (63, 5, 1024, 2, 6),
+                                                      (31, 9, 4096, 1, 60)])
 def test_ldbp_two_modes_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
     """Manakov matrix-core inference against the oracle (6 steps) or the FP32
-    kernels (60 steps), bound 6e-4 sqrt(steps) as for one mode."""
+    kernels (60 steps), bound 6e-4 sqrt(steps) plus the taps' FP16 rounding,
+    as for one mode."""
     import focss_amd as F
     print(f"ldbp two modes infer taps={taps} ktaps={ktaps} n={n} batch={batch} steps={steps}", flush=True)
     r = np.random.default_rng(43)
@@ -186,7 +201,7 @@ def test_ldbp_two_modes_infer_fp16_mfma(ctx, taps, ktaps, n, batch, steps):
     model.forward(bx, bf)
     out = bo.download(np.empty_like(x))
     out32 = bf.download(np.empty_like(x))
-    tol = 6e-4 * np.sqrt(steps)
+    tol = 6e-4 * np.sqrt(steps) + _tap_rounding(fir)
     for b in range(batch):
         sl = slice(b * 2 * n, (b + 1) * 2 * n)
         o = O.ldbp_forward(fir, g, x[sl].astype(np.complex128), modes=2)[0] if steps <= 6 else out32[sl]
