@@ -111,6 +111,14 @@ def _check(rc):
         raise FocssError(f"focss error {rc}: {_lib.focss_last_error().decode()}")
 
 
+def _need(buf, nbytes, what):
+    """The C-ABI takes no buffer sizes: refuse a device buffer smaller than
+    the call will touch."""
+    if buf is None or buf.nbytes < nbytes:
+        have = "none" if buf is None else f"{buf.nbytes} bytes"
+        raise FocssError(f"{what}: device buffer of {nbytes} bytes needed, {have} given")
+
+
 def build_info():
     return _lib.focss_build_info().decode()
 
@@ -131,11 +139,15 @@ class DeviceBuffer:
 
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)
-        assert arr.nbytes <= self.nbytes
+        if arr.nbytes > self.nbytes:
+            raise FocssError(f"upload of {arr.nbytes} bytes into a {self.nbytes}-byte device buffer")
         _check(_lib.focss_memcpy_h2d(self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes))
 
     def download(self, arr):
-        assert arr.flags.c_contiguous and arr.nbytes <= self.nbytes
+        if not arr.flags.c_contiguous:
+            raise FocssError("download needs a C-contiguous host array")
+        if arr.nbytes > self.nbytes:
+            raise FocssError(f"download of {arr.nbytes} bytes from a {self.nbytes}-byte device buffer")
         _check(_lib.focss_memcpy_d2h(self.ctx.handle, arr.ctypes.data, self.ptr, arr.nbytes))
         return arr
 
@@ -397,9 +409,18 @@ class LearnedDBP:
         self.lr_scale = self.ctx.alloc(scale.nbytes)
         self.lr_scale.upload(scale)
 
+    def _field_bytes(self, batch):
+        return batch * self.modes * self.samples * 8  # complex64
+
     def forward(self, inp, out=None, target=None, batch=None):
         loss = C.c_double(0)
         b = batch or self.batch
+        nb = self._field_bytes(b)
+        _need(inp, nb, "LearnedDBP.forward input")
+        if out is not None:
+            _need(out, nb, "LearnedDBP.forward output")
+        if target is not None:
+            _need(target, nb, "LearnedDBP.forward target")
         _check(_lib.focss_ldbp_forward(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
                                        self.samples, b,
                                        self.params.ptr, inp.ptr, out.ptr if out else None,
@@ -410,12 +431,16 @@ class LearnedDBP:
         """Inference with the FIR on FP16 matrix-core products (FP32
         accumulation); stream-ordered."""
         b = batch or self.batch
+        _need(inp, self._field_bytes(b), "LearnedDBP.infer input")
+        _need(out, self._field_bytes(b), "LearnedDBP.infer output")
         _check(_lib.focss_ldbp_infer(self.ctx.handle, self.steps, self.taps, self.kerr_taps, self.modes,
                                      self.samples, b,
                                      self.params.ptr, inp.ptr, out.ptr))
 
     def gradients(self, inp, target):
         """(loss, Wirtinger grad of the taps [steps][taps] complex, dL/dg [steps][kerr_taps])."""
+        _need(inp, self._field_bytes(self.batch), "LearnedDBP.gradients input")
+        _need(target, self._field_bytes(self.batch), "LearnedDBP.gradients target")
         self._training_buffers()
         self.grad.upload(np.zeros(self.n_params + 1, dtype=np.float64))
         gp = self.grad.ptr.value
@@ -429,6 +454,8 @@ class LearnedDBP:
                 g[1 + nf:].reshape(self.steps, self.kerr_taps))
 
     def train_step(self, inp, target, lr):
+        _need(inp, self._field_bytes(self.batch), "LearnedDBP.train_step input")
+        _need(target, self._field_bytes(self.batch), "LearnedDBP.train_step target")
         self._training_buffers()
         self.t += 1
         loss = C.c_double(0)
@@ -497,14 +524,25 @@ class TripletMLP:
         """out = the equalised symbols of `batch` waveforms (default: the
         batch given at construction; inference takes any batch); returns the
         loss against target (the call then synchronises) or None."""
+        b = batch or self.batch
+        nb = b * self.modes * self.symbols * 8  # complex64
+        _need(inp, nb, "TripletMLP.forward input")
+        _need(out, nb, "TripletMLP.forward output")
+        if target is not None:
+            _need(target, nb, "TripletMLP.forward target")
         loss = C.c_double(0)
         _check(_lib.focss_tmlp_forward(self.ctx.handle, self.modes, self.symbols, batch or self.batch,
                                        self.params.ptr, inp.ptr, out.ptr, target.ptr if target else None,
                                        C.byref(loss) if target else None))
         return loss.value if target else None
 
+    def _batch_bytes(self):
+        return self.batch * self.modes * self.symbols * 8
+
     def gradients(self, inp, target):
         """(loss, dL/dparams as float64 [param_count])."""
+        _need(inp, self._batch_bytes(), "TripletMLP.gradients input")
+        _need(target, self._batch_bytes(), "TripletMLP.gradients target")
         self._training_buffers()
         self.grad.upload(np.zeros(self.n_params + 1, dtype=np.float64))
         gp = self.grad.ptr.value
@@ -515,6 +553,8 @@ class TripletMLP:
         return g[0], g[1:]
 
     def train_step(self, inp, target, lr):
+        _need(inp, self._batch_bytes(), "TripletMLP.train_step input")
+        _need(target, self._batch_bytes(), "TripletMLP.train_step target")
         self._training_buffers()
         self.t += 1
         loss = C.c_double(0)

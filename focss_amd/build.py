@@ -2,6 +2,8 @@
 the C++ host library focss_amd/libfocss.so (focss::Field / SSFM / link
 drivers over the C-ABI).  Run as `python focss_amd/build.py` (by path: importing the package needs the
 library already built)."""
+import concurrent.futures as cf
+import glob
 import os
 import subprocess
 import sys
@@ -11,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 instructions in any HIP code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -22,10 +24,33 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_hip():
+def _stale(obj, dep):
+    """True when obj is missing or older than a file its dependency list
+    (written by the compiler with -MD) names."""
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return True
+    t = os.path.getmtime(obj)
+    with open(dep) as f:
+        names = f.read().replace("\\\n", " ").split(":", 1)[-1].split()
+    return any(not os.path.exists(n) or os.path.getmtime(n) > t for n in names)
+
+
+def build_hip(incremental=False):
+    """Every csrc/*.hip compiled to an object in parallel (the kernel
+    registries are split over several files for this), then linked.  With
+    incremental=True only objects older than their sources are rebuilt."""
     out = os.path.join(HERE, "libfocss_hip.so")
-    srcs = [os.path.join(HERE, "csrc", "focss_capi.hip")]
-    _run([HIPCC] + HIP_FLAGS + ["-o", out] + srcs)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    jobs = int(os.environ.get("MAX_JOBS", str(min(8, os.cpu_count() or 1))))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_run, [HIPCC] + HIP_FLAGS + ["-MD", "-MF", o[:-2] + ".d", "-c", "-o", o, src])
+                for src, o in zip(srcs, objs) if not incremental or _stale(o, o[:-2] + ".d")]
+        for f in futs:
+            f.result()
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs)
     return out
 
 
@@ -50,7 +75,7 @@ def build_host():
 def main():
     only_host = "--host" in sys.argv
     if not only_host:
-        build_hip()
+        build_hip(incremental="--incremental" in sys.argv)
     build_host()
 
 

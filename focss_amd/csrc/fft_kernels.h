@@ -3,8 +3,10 @@
 // inverse unscaled, `modes` rows of N samples each.
 //   N <= 8192: one workgroup per row, whole row in LDS.
 //   larger N : column pass (length N1 DFT of stride-N2 columns, x W_N^{+-n2 k1},
-//              in place) then row pass (length N2 DFT of each contiguous row,
-//              written to natural order k1 + N1 k2 in a second buffer).
+//              from the field into a work buffer at the same positions) then
+//              row pass (length N2 DFT of each contiguous row of the work
+//              buffer, written back to the field in natural order k1 + N1 k2):
+//              one read and one write of the field per pass.
 #pragma once
 #include "device_util.h"
 #include "fft_lds.h"
@@ -29,14 +31,16 @@ __global__ __launch_bounds__(NT) void k_fft_whole(cx<R>* data, double scale, Tab
 }
 
 template <typename R, int N1, int C, int SIGN>
-__global__ __launch_bounds__(NT) void k_fft_cols(cx<R>* data, int log_n, int tiles_per_row, Tables<R> tb) {
+__global__ __launch_bounds__(NT) void k_fft_cols(const cx<R>* data, cx<R>* work, int log_n, int tiles_per_row,
+                                                 Tables<R> tb) {
     __shared__ cx<R> buf[padded(N1 * C)];
     __shared__ cx<R> tw_t[N1];
     const int N = 1 << log_n;
     const int N2 = N / N1;
     const int row = blockIdx.x / tiles_per_row;
     const int n2_0 = (blockIdx.x % tiles_per_row) * C;
-    cx<R>* g = data + (long long)row * N;
+    const cx<R>* g = data + (long long)row * N;
+    cx<R>* o = work + (long long)row * N;
     load_tw_full<R, N1, NT>(tw_t, tb.bank);
     const TwFull<R> tw{tw_t};
     for (int i = threadIdx.x; i < N1 * C; i += NT) {
@@ -48,7 +52,7 @@ __global__ __launch_bounds__(NT) void k_fft_cols(cx<R>* data, int log_n, int til
     for (int i = threadIdx.x; i < N1 * C; i += NT) {
         int k1 = i / C, c = i % C;
         cx<R> w = tw_big<R>(tb, (unsigned)((n2_0 + c) * k1) & (unsigned)(N - 1), log_n);
-        g[(long long)k1 * N2 + n2_0 + c] = buf[pad(i)] * (SIGN < 0 ? w : conj(w));
+        o[(long long)k1 * N2 + n2_0 + c] = buf[pad(i)] * (SIGN < 0 ? w : conj(w));
     }
 }
 

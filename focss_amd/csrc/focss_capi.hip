@@ -13,8 +13,7 @@
 #include <vector>
 
 #include "../../include/focss.h"
-#include "ssfm_kernels.h"
-#include "fft_kernels.h"
+#include "registry.h"
 #include "triplet_kernels.h"
 #include "ldbp_kernels.h"
 #include "tmlp_kernels.h"
@@ -43,183 +42,6 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                      \
             return fail(FOCSS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
-
-// ------------------------------------------------------------------
-// kernel registry
-// ------------------------------------------------------------------
-template <typename R>
-using KernelFn = void (*)(PassArgs, Tables<R>);
-
-constexpr int kTileVariants = 3;
-constexpr int kTileElems[kTileVariants] = {4096, 2048, 1024};
-
-template <typename R>
-struct Registry {
-    // resident[log_n][modes-1]
-    KernelFn<R> resident[21][2] = {};
-    // time[log_n][modes-1][variant], spec[log_n][variant]
-    KernelFn<R> time_k[21][2][kTileVariants] = {};
-    KernelFn<R> spec_k[21][kTileVariants] = {};
-    int time_cols[21][2][kTileVariants] = {};
-    int spec_rows[21][kTileVariants] = {};
-    int time_threads[21][2][kTileVariants] = {};
-    int spec_threads[21][kTileVariants] = {};
-};
-
-template <typename R, int N, int M>
-void reg_resident(Registry<R>& r) {
-    r.resident[ilog2<N>::value][M - 1] = &k_resident<R, N, M>;
-}
-
-template <typename R, int LOGN, int M, int V>
-void reg_time(Registry<R>& r) {
-    constexpr int N1 = 1 << (LOGN / 2);
-    constexpr int N2 = 1 << (LOGN - LOGN / 2);
-    constexpr int C0 = kTileElems[V] / (N1 * M);
-    constexpr int C = C0 < 1 ? 0 : (C0 > N2 ? N2 : C0);
-    if constexpr (C >= 1) {
-        r.time_k[LOGN][M - 1][V] = &k_time_pass<R, N1, C, M>;
-        r.time_cols[LOGN][M - 1][V] = C;
-        r.time_threads[LOGN][M - 1][V] = pass_threads(C * N1 / 8);
-    }
-}
-
-template <typename R, int LOGN, int V>
-void reg_spec(Registry<R>& r) {
-    constexpr int N2 = 1 << (LOGN - LOGN / 2);
-    constexpr int S0 = kTileElems[V] / N2;
-    constexpr int S = S0 < 1 ? 0 : S0;
-    if constexpr (S >= 1) {
-        r.spec_k[LOGN][V] = &k_spec_pass<R, N2, S>;
-        r.spec_rows[LOGN][V] = S;
-        r.spec_threads[LOGN][V] = pass_threads(S * N2 / 8);
-    }
-}
-
-template <typename R, int LOGN>
-void reg_two_pass(Registry<R>& r) {
-    reg_time<R, LOGN, 1, 0>(r);
-    reg_time<R, LOGN, 1, 1>(r);
-    reg_time<R, LOGN, 1, 2>(r);
-    reg_time<R, LOGN, 2, 0>(r);
-    reg_time<R, LOGN, 2, 1>(r);
-    reg_time<R, LOGN, 2, 2>(r);
-    reg_spec<R, LOGN, 0>(r);
-    reg_spec<R, LOGN, 1>(r);
-    reg_spec<R, LOGN, 2>(r);
-}
-
-template <typename R>
-Registry<R> make_registry() {
-    Registry<R> r;
-    reg_resident<R, 64, 1>(r);
-    reg_resident<R, 128, 1>(r);
-    reg_resident<R, 256, 1>(r);
-    reg_resident<R, 512, 1>(r);
-    reg_resident<R, 1024, 1>(r);
-    reg_resident<R, 2048, 1>(r);
-    reg_resident<R, 4096, 1>(r);
-    reg_resident<R, 64, 2>(r);
-    reg_resident<R, 128, 2>(r);
-    reg_resident<R, 256, 2>(r);
-    reg_resident<R, 512, 2>(r);
-    reg_resident<R, 1024, 2>(r);
-    reg_resident<R, 2048, 2>(r);
-    reg_resident<R, 8192, 1>(r);
-    reg_resident<R, 4096, 2>(r);
-    reg_two_pass<R, 12>(r);
-    reg_two_pass<R, 13>(r);
-    reg_two_pass<R, 14>(r);
-    reg_two_pass<R, 15>(r);
-    reg_two_pass<R, 16>(r);
-    reg_two_pass<R, 17>(r);
-    reg_two_pass<R, 18>(r);
-    reg_two_pass<R, 19>(r);
-    reg_two_pass<R, 20>(r);
-    return r;
-}
-
-const Registry<double>& reg64() {
-    static Registry<double> r = make_registry<double>();
-    return r;
-}
-const Registry<float>& reg32() {
-    static Registry<float> r = make_registry<float>();
-    return r;
-}
-
-template <typename R>
-const Registry<R>& reg();
-template <>
-const Registry<double>& reg<double>() {
-    return reg64();
-}
-template <>
-const Registry<float>& reg<float>() {
-    return reg32();
-}
-
-// standalone DFT kernels: [log_n][sign 0 = forward, 1 = inverse]
-template <typename R>
-struct FftRegistry {
-    void (*whole[21][2])(cx<R>*, double, Tables<R>) = {};
-    void (*cols[21][2])(cx<R>*, int, int, Tables<R>) = {};
-    void (*rows[21][2])(const cx<R>*, cx<R>*, int, int, double, Tables<R>) = {};
-    int ncols[21] = {}, nrows[21] = {};
-};
-
-template <typename R, int LOGN>
-void reg_fft_whole(FftRegistry<R>& r) {
-    r.whole[LOGN][0] = &k_fft_whole<R, (1 << LOGN), -1>;
-    r.whole[LOGN][1] = &k_fft_whole<R, (1 << LOGN), +1>;
-}
-
-template <typename R, int LOGN>
-void reg_fft_two(FftRegistry<R>& r) {
-    constexpr int N1 = 1 << (LOGN / 2);
-    constexpr int N2 = 1 << (LOGN - LOGN / 2);
-    constexpr int TILE = sizeof(R) == 8 ? 2048 : 4096;
-    constexpr int C = TILE / N1 > N2 ? N2 : TILE / N1;
-    constexpr int S = TILE / N2;
-    r.cols[LOGN][0] = &k_fft_cols<R, N1, C, -1>;
-    r.cols[LOGN][1] = &k_fft_cols<R, N1, C, +1>;
-    r.rows[LOGN][0] = &k_fft_rows<R, N2, S, -1>;
-    r.rows[LOGN][1] = &k_fft_rows<R, N2, S, +1>;
-    r.ncols[LOGN] = C;
-    r.nrows[LOGN] = S;
-}
-
-template <typename R>
-FftRegistry<R> make_fft_registry() {
-    FftRegistry<R> r;
-    reg_fft_whole<R, 1>(r);
-    reg_fft_whole<R, 2>(r);
-    reg_fft_whole<R, 3>(r);
-    reg_fft_whole<R, 4>(r);
-    reg_fft_whole<R, 5>(r);
-    reg_fft_whole<R, 6>(r);
-    reg_fft_whole<R, 7>(r);
-    reg_fft_whole<R, 8>(r);
-    reg_fft_whole<R, 9>(r);
-    reg_fft_whole<R, 10>(r);
-    reg_fft_whole<R, 11>(r);
-    reg_fft_whole<R, 12>(r);
-    reg_fft_whole<R, 13>(r);
-    reg_fft_two<R, 14>(r);
-    reg_fft_two<R, 15>(r);
-    reg_fft_two<R, 16>(r);
-    reg_fft_two<R, 17>(r);
-    reg_fft_two<R, 18>(r);
-    reg_fft_two<R, 19>(r);
-    reg_fft_two<R, 20>(r);
-    return r;
-}
-
-template <typename R>
-const FftRegistry<R>& fft_reg() {
-    static FftRegistry<R> r = make_fft_registry<R>();
-    return r;
-}
 
 int ilog2_int(int n) {
     int p = 0;
@@ -321,20 +143,20 @@ void drop_graph(focss_ctx* c) {
 }
 
 bool resident_ok(int precision, int log_n, int modes) {
-    if (precision == FOCSS_F64) return reg64().resident[log_n][modes - 1] != nullptr;
-    return reg32().resident[log_n][modes - 1] != nullptr;
+    if (precision == FOCSS_F64) return reg<double>().resident[log_n][modes - 1] != nullptr;
+    return reg<float>().resident[log_n][modes - 1] != nullptr;
 }
 
 bool time_ok(int precision, int log_n, int modes, int variant) {
     if (log_n < 0 || log_n > 20 || variant < 0 || variant >= kTileVariants) return false;
-    if (precision == FOCSS_F64) return reg64().time_k[log_n][modes - 1][variant] != nullptr;
-    return reg32().time_k[log_n][modes - 1][variant] != nullptr;
+    if (precision == FOCSS_F64) return reg<double>().time_k[log_n][modes - 1][variant] != nullptr;
+    return reg<float>().time_k[log_n][modes - 1][variant] != nullptr;
 }
 
 bool spec_ok(int precision, int log_n, int variant) {
     if (log_n < 0 || log_n > 20 || variant < 0 || variant >= kTileVariants) return false;
-    if (precision == FOCSS_F64) return reg64().spec_k[log_n][variant] != nullptr;
-    return reg32().spec_k[log_n][variant] != nullptr;
+    if (precision == FOCSS_F64) return reg<double>().spec_k[log_n][variant] != nullptr;
+    return reg<float>().spec_k[log_n][variant] != nullptr;
 }
 
 bool two_pass_ok(int precision, int log_n, int modes, int variant) {
@@ -675,10 +497,10 @@ int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data) {
         int n2 = samples / n1;
         int tc = n2 / r.ncols[log_n];
         int tr = n1 / r.nrows[log_n];
-        hipLaunchKernelGGL(r.cols[log_n][si], dim3(modes * tc), dim3(NT), 0, c->stream, d, log_n, tc, tb);
-        hipLaunchKernelGGL(r.rows[log_n][si], dim3(modes * tr), dim3(NT), 0, c->stream, (const cx<R>*)d,
-                           (cx<R>*)c->scratch, log_n, tr, scale, tb);
-        HIP_TRY(hipMemcpyAsync(d, c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(r.cols[log_n][si], dim3(modes * tc), dim3(NT), 0, c->stream, (const cx<R>*)d,
+                           (cx<R>*)c->scratch, log_n, tc, tb);
+        hipLaunchKernelGGL(r.rows[log_n][si], dim3(modes * tr), dim3(NT), 0, c->stream,
+                           (const cx<R>*)c->scratch, d, log_n, tr, scale, tb);
     }
     HIP_TRY(hipGetLastError());
     return 0;

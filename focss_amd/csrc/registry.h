@@ -1,0 +1,135 @@
+// Kernel registries of the SSFM passes and the standalone DFT: tables of
+// kernel pointers indexed by (log2 N, modes, tile variant), filled from
+// several translation units (registry_*.hip) so the ~300 instantiations
+// compile in parallel.
+#pragma once
+#include "ssfm_kernels.h"
+#include "fft_kernels.h"
+
+namespace focss_hip {
+
+template <typename R>
+using KernelFn = void (*)(PassArgs, Tables<R>);
+
+constexpr int kTileVariants = 3;
+constexpr int kTileElems[kTileVariants] = {4096, 2048, 1024};
+
+template <typename R>
+struct Registry {
+    // resident[log_n][modes-1]
+    KernelFn<R> resident[21][2] = {};
+    // time[log_n][modes-1][variant], spec[log_n][variant]
+    KernelFn<R> time_k[21][2][kTileVariants] = {};
+    KernelFn<R> spec_k[21][kTileVariants] = {};
+    int time_cols[21][2][kTileVariants] = {};
+    int spec_rows[21][kTileVariants] = {};
+    int time_threads[21][2][kTileVariants] = {};
+    int spec_threads[21][kTileVariants] = {};
+};
+
+template <typename R, int N, int M>
+void reg_resident(Registry<R>& r) {
+    r.resident[ilog2<N>::value][M - 1] = &k_resident<R, N, M>;
+}
+
+template <typename R, int LOGN, int M, int V>
+void reg_time(Registry<R>& r) {
+    constexpr int N1 = 1 << (LOGN / 2);
+    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int C0 = kTileElems[V] / (N1 * M);
+    constexpr int C = C0 < 1 ? 0 : (C0 > N2 ? N2 : C0);
+    if constexpr (C >= 1) {
+        r.time_k[LOGN][M - 1][V] = &k_time_pass<R, N1, C, M>;
+        r.time_cols[LOGN][M - 1][V] = C;
+        r.time_threads[LOGN][M - 1][V] = pass_threads(C * N1 / 8);
+    }
+}
+
+template <typename R, int LOGN, int V>
+void reg_spec(Registry<R>& r) {
+    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int S0 = kTileElems[V] / N2;
+    constexpr int S = S0 < 1 ? 0 : S0;
+    if constexpr (S >= 1) {
+        r.spec_k[LOGN][V] = &k_spec_pass<R, N2, S>;
+        r.spec_rows[LOGN][V] = S;
+        r.spec_threads[LOGN][V] = pass_threads(S * N2 / 8);
+    }
+}
+
+template <typename R, int LOGN>
+void reg_two_pass(Registry<R>& r) {
+    reg_time<R, LOGN, 1, 0>(r);
+    reg_time<R, LOGN, 1, 1>(r);
+    reg_time<R, LOGN, 1, 2>(r);
+    reg_time<R, LOGN, 2, 0>(r);
+    reg_time<R, LOGN, 2, 1>(r);
+    reg_time<R, LOGN, 2, 2>(r);
+    reg_spec<R, LOGN, 0>(r);
+    reg_spec<R, LOGN, 1>(r);
+    reg_spec<R, LOGN, 2>(r);
+}
+
+// standalone DFT kernels: [log_n][sign 0 = forward, 1 = inverse]
+template <typename R>
+struct FftRegistry {
+    void (*whole[21][2])(cx<R>*, double, Tables<R>) = {};
+    void (*cols[21][2])(const cx<R>*, cx<R>*, int, int, Tables<R>) = {};
+    void (*rows[21][2])(const cx<R>*, cx<R>*, int, int, double, Tables<R>) = {};
+    int ncols[21] = {}, nrows[21] = {};
+};
+
+template <typename R, int LOGN>
+void reg_fft_whole(FftRegistry<R>& r) {
+    r.whole[LOGN][0] = &k_fft_whole<R, (1 << LOGN), -1>;
+    r.whole[LOGN][1] = &k_fft_whole<R, (1 << LOGN), +1>;
+}
+
+template <typename R, int LOGN>
+void reg_fft_two(FftRegistry<R>& r) {
+    constexpr int N1 = 1 << (LOGN / 2);
+    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int TILE = sizeof(R) == 8 ? 2048 : 4096;
+    constexpr int C = TILE / N1 > N2 ? N2 : TILE / N1;
+    constexpr int S = TILE / N2;
+    r.cols[LOGN][0] = &k_fft_cols<R, N1, C, -1>;
+    r.cols[LOGN][1] = &k_fft_cols<R, N1, C, +1>;
+    r.rows[LOGN][0] = &k_fft_rows<R, N2, S, -1>;
+    r.rows[LOGN][1] = &k_fft_rows<R, N2, S, +1>;
+    r.ncols[LOGN] = C;
+    r.nrows[LOGN] = S;
+}
+
+// the parts, one per translation unit
+void fill_resident(Registry<double>& r);
+void fill_resident(Registry<float>& r);
+void fill_two_pass_small(Registry<double>& r);  // 2^12 .. 2^15
+void fill_two_pass_small(Registry<float>& r);
+void fill_two_pass_large(Registry<double>& r);  // 2^16 .. 2^20
+void fill_two_pass_large(Registry<float>& r);
+void fill_fft(FftRegistry<double>& r);
+void fill_fft(FftRegistry<float>& r);
+
+template <typename R>
+const Registry<R>& reg() {
+    static const Registry<R> r = [] {
+        Registry<R> x;
+        fill_resident(x);
+        fill_two_pass_small(x);
+        fill_two_pass_large(x);
+        return x;
+    }();
+    return r;
+}
+
+template <typename R>
+const FftRegistry<R>& fft_reg() {
+    static const FftRegistry<R> r = [] {
+        FftRegistry<R> x;
+        fill_fft(x);
+        return x;
+    }();
+    return r;
+}
+
+}  // namespace focss_hip

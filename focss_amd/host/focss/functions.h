@@ -6,6 +6,7 @@
 #define FOCSS_FUNCTIONS_H_
 
 #include <cstdint>
+#include <fstream>
 #include <iostream>
 #include <string>
 

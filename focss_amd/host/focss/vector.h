@@ -53,7 +53,9 @@ class Vector {
     Scalar* raw() { return view_ ? view_ : store_.data(); }
     const Scalar* raw() const { return view_ ? view_ : store_.data(); }
     Scalar& operator[](const int& i) { return raw()[i]; }
-    const Scalar& operator[](const int& i) const { return raw()[i]; }
+    // by value, as the reference's const subscript (vector.h:169): a caller
+    // may then call non-const members of the copy
+    Scalar operator[](const int& i) const { return raw()[i]; }
 
     template <typename T>
     Vector& operator*=(const T& s) {

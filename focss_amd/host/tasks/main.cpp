@@ -1,5 +1,5 @@
 // focss_tasks: command-line driver over the focss C++ API.
-//   focss_tasks verification <samples> <steps>
+//   focss_tasks soliton dark|bright <samples> <steps>   (prints the relative L2 error)
 //   focss_tasks dbp     <in> <out> <samples> <modes> <dt> <spans> <steps> <alpha> <beta2> <gamma> <length> <kappa>
 //   focss_tasks cdc     <in> <out> <samples> <modes> <dt> <spans> <alpha> <beta2> <gamma> <length> <kappa>
 //   focss_tasks sino    <tx> <rx> <out> <samples> <modes> <radius> <type> <lw_re> <lw_im>
@@ -37,9 +37,8 @@
 #include "focss/functions.h"
 #include "focss/runtime.h"
 #include "tasks/field_io.h"
+#include "tasks/soliton_check.h"
 #include "tasks/systems.h"
-
-double verification(const int& n_samples, const int& n_steps);
 
 namespace {
 focss::Field read_field(const char* path, int samples, int modes, double dt, double fc) {
@@ -71,12 +70,12 @@ focss::LinkParameters params(char** a) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: focss_tasks verification|dbp|cdc|forward ...\n");
+        std::fprintf(stderr, "usage: focss_tasks soliton|dbp|cdc|forward ...\n");
         return 2;
     }
     std::string cmd = argv[1];
-    if (cmd == "verification" && argc == 4) {
-        double e = verification(std::atoi(argv[2]), std::atoi(argv[3]));
+    if (cmd == "soliton" && argc == 5) {
+        double e = focss::soliton_check(std::strcmp(argv[2], "dark") == 0, std::atoi(argv[3]), std::atoi(argv[4]));
         std::cout.precision(17);
         std::cout << e << std::endl;
         return e < 1 ? 0 : 1;

@@ -100,37 +100,76 @@ void orc_gauss_pair(uint64_t seed, uint32_t stream, uint64_t index, double* z0, 
 }
 
 /* ------------------------------------------------------------------ */
-/* FFT: Stockham auto-sort, radix 4 with one radix-2 stage when log2 N  */
-/* is odd (Govindaraju et al., "High performance discrete Fourier       */
-/* transforms on graphics processors", SC'08, Fig. 2 formulation).      */
+/* FFT of any length N (FFTW plans any N: field.cpp:530-541).           */
+/* Stockham auto-sort (Govindaraju et al., "High performance discrete   */
+/* Fourier transforms on graphics processors", SC'08, Fig. 2): one      */
+/* stage per factor of N, radix 4 while 4 divides what is left, then 2, */
+/* then the odd primes, each a direct length-r DFT.  A prime factor     */
+/* above 64 goes through Bluestein's chirp-z identity (Bluestein, 1970; */
+/* jk = (j^2 + k^2 - (k-j)^2)/2) as a circular convolution of a         */
+/* power-of-two length M >= 2N-1.                                        */
+/* The twiddle table W_N^m = (cos 2 pi m/N, sign sin 2 pi m/N) is kept   */
+/* per thread for the last few (N, sign), so repeated transforms of one */
+/* length do not recompute it; its values do not depend on the cache.    */
 /* ------------------------------------------------------------------ */
-static void stockham(int n, int sign, orc_c128* x) {
-    if (n <= 1) return;
-    orc_c128* w = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
-    orc_c128* work = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
+#define ORC_TW_SLOTS 4
+typedef struct {
+    int n, sign;
+    orc_c128* w;
+} tw_slot;
+static __thread tw_slot tw_cache[ORC_TW_SLOTS];
+static __thread int tw_next;
+
+static const orc_c128* twiddles(int n, int sign) {
+    for (int i = 0; i < ORC_TW_SLOTS; ++i)
+        if (tw_cache[i].w && tw_cache[i].n == n && tw_cache[i].sign == sign) return tw_cache[i].w;
+    tw_slot* s = &tw_cache[tw_next];
+    tw_next = (tw_next + 1) % ORC_TW_SLOTS;
+    free(s->w);
+    s->w = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
+    s->n = n;
+    s->sign = sign;
     for (int m = 0; m < n; ++m) {
         double a = 2.0 * kPi * (double)m / (double)n;
-        w[m].re = cos(a);
-        w[m].im = sign * sin(a);
+        s->w[m].re = cos(a);
+        s->w[m].im = sign * sin(a);
     }
+    return s->w;
+}
+
+/* smallest factor of n to take next: 4, 2, then odd primes; 0 if the
+   remaining n is a prime above 64 */
+static int next_radix(int n) {
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p <= 64 && p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n == 1 ? 1 : 0;
+}
+
+static void fft_any(int n, int sign, orc_c128* x);
+
+static void stockham(int n, int sign, orc_c128* x) {
+    if (n <= 1) return;
+    const orc_c128* w = twiddles(n, sign);
+    orc_c128* work = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
     orc_c128* src = x;
     orc_c128* dst = work;
+    orc_c128 v[64], y[64];
     for (int ns = 1; ns < n;) {
-        int radix = ((n / ns) % 4 == 0) ? 4 : 2;
+        int radix = next_radix(n / ns);
         int stride = n / radix;
         int tw_step = n / (ns * radix);
         for (int j = 0; j < stride; ++j) {
             int k = j % ns;
-            orc_c128 v[4];
             for (int r = 0; r < radix; ++r) {
                 orc_c128 a = src[j + r * stride];
-                v[r] = (r == 0) ? a : c_mul(a, w[r * k * tw_step]);
+                v[r] = (r == 0) ? a : c_mul(a, w[(long)r * k * tw_step]);
             }
-            orc_c128 y[4];
             if (radix == 2) {
                 y[0] = c_add(v[0], v[1]);
                 y[1] = c_sub(v[0], v[1]);
-            } else {
+            } else if (radix == 4) {
                 /* W4 = exp(sign * 2 pi i / 4) = sign * i */
                 orc_c128 a0 = c_add(v[0], v[2]);
                 orc_c128 a1 = c_sub(v[0], v[2]);
@@ -141,6 +180,13 @@ static void stockham(int n, int sign, orc_c128* x) {
                 y[2] = c_sub(a0, b0);
                 y[1] = c_add(a1, jb1);
                 y[3] = c_sub(a1, jb1);
+            } else {
+                /* direct length-r DFT, W_r^{pq} = W_N^{(pq mod r) N/r} */
+                for (int p = 0; p < radix; ++p) {
+                    orc_c128 s = v[0];
+                    for (int q = 1; q < radix; ++q) s = c_add(s, c_mul(v[q], w[(long)((p * q) % radix) * stride]));
+                    y[p] = s;
+                }
             }
             int base = (j / ns) * ns * radix + k;
             for (int r = 0; r < radix; ++r) dst[base + r * ns] = y[r];
@@ -152,19 +198,60 @@ static void stockham(int n, int sign, orc_c128* x) {
     }
     if (src != x) memcpy(x, src, sizeof(orc_c128) * (size_t)n);
     free(work);
-    free(w);
+}
+
+/* Bluestein: X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(sign pi i j^2/N) */
+static void bluestein(int n, int sign, orc_c128* x) {
+    int m = 1;
+    while (m < 2 * n - 1) m <<= 1;
+    orc_c128* chirp = (orc_c128*)malloc(sizeof(orc_c128) * (size_t)n);
+    orc_c128* a = (orc_c128*)calloc((size_t)m, sizeof(orc_c128));
+    orc_c128* b = (orc_c128*)calloc((size_t)m, sizeof(orc_c128));
+    for (int j = 0; j < n; ++j) {
+        long long q = ((long long)j * j) % (2LL * n); /* j^2 mod 2N keeps the angle small */
+        double ang = kPi * (double)q / (double)n;
+        chirp[j].re = cos(ang);
+        chirp[j].im = sign * sin(ang);
+    }
+    for (int j = 0; j < n; ++j) a[j] = c_mul(x[j], chirp[j]);
+    b[0] = c_conj(chirp[0]);
+    for (int j = 1; j < n; ++j) b[j] = b[m - j] = c_conj(chirp[j]);
+    stockham(m, -1, a);
+    stockham(m, -1, b);
+    for (int i = 0; i < m; ++i) a[i] = c_mul(a[i], b[i]);
+    stockham(m, +1, a);
+    double inv = 1.0 / (double)m;
+    for (int k = 0; k < n; ++k) x[k] = c_scale(c_mul(a[k], chirp[k]), inv);
+    free(b);
+    free(a);
+    free(chirp);
+}
+
+/* unscaled DFT of length n, exponent sign `sign` */
+static void fft_any(int n, int sign, orc_c128* x) {
+    if (n <= 1) return;
+    int r = n;
+    while (r > 1) {
+        int p = next_radix(r);
+        if (p == 0) break;
+        r /= p;
+    }
+    if (r == 1)
+        stockham(n, sign, x);
+    else
+        bluestein(n, sign, x);
 }
 
 /* Field::fft_inplace: FFTW_FORWARD then x1/N        field.cpp:343-347 */
 void orc_fft_inplace(int modes, int samples, orc_c128* data) {
-    for (int m = 0; m < modes; ++m) stockham(samples, -1, data + (size_t)m * samples);
+    for (int m = 0; m < modes; ++m) fft_any(samples, -1, data + (size_t)m * samples);
     double s = 1.0 / (double)samples;
     for (long i = 0; i < (long)modes * samples; ++i) data[i] = c_scale(data[i], s);
 }
 
 /* Field::ifft_inplace: FFTW_BACKWARD, unscaled      field.cpp:349-352 */
 void orc_ifft_inplace(int modes, int samples, orc_c128* data) {
-    for (int m = 0; m < modes; ++m) stockham(samples, +1, data + (size_t)m * samples);
+    for (int m = 0; m < modes; ++m) fft_any(samples, +1, data + (size_t)m * samples);
 }
 
 void orc_dft_naive(int sign, int n, const orc_c128* in, orc_c128* out) {

@@ -2,7 +2,7 @@
 drivers) over the C-ABI, driven through the focss_tasks executable and
 compared with the CPU oracle.  Covers the reference-side entry points
 backward_propagation, cd_compensation, forward_propagation and the
-verification() dark-soliton task."""
+soliton known-answer task."""
 import os
 import subprocess
 
@@ -27,18 +27,27 @@ def fib_args(f):
     return [repr(f["alpha"]), repr(f["beta2"]), repr(f["gamma"]), repr(f["length"]), repr(f["kappa"])]
 
 
-def test_verification_task(tmp_path):
-    out = run(["verification", 1 << 12, 200], tmp_path)
+@pytest.mark.parametrize("kind", ["dark", "bright"])
+def test_soliton_task(tmp_path, kind):
+    """The soliton known-answer task through SSFM::run: its error equals the
+    oracle's on the same recipe, and is small (the symmetric split step is
+    second order: 400 steps over z = 1)."""
+    n, steps, window = 1 << 11, 400, 40.0
+    out = run(["soliton", kind, n, steps], tmp_path)
     err = float(out.strip().splitlines()[-1])
-    # same recipe on the oracle
-    n, window = 1 << 12, 10.0
     dt = window / n
     t = (np.arange(n) - n // 2) * dt
-    y, _ = O.ssfm_run(O.fiber(0.0, 1.0, 1.0, 1.0, 0.0, 1.0), O.grid(O.UNIFORM, 200), O.FORWARD,
-                      np.tanh(t).astype(np.complex128), dt)
-    ref = np.exp(1j) * np.tanh(t)
-    e_ref = np.sqrt(np.mean(np.abs(y - ref) ** 2) / np.mean(np.abs(ref) ** 2))
-    assert err == pytest.approx(e_ref, rel=1e-9)
+    a = window / 4
+    if kind == "dark":
+        x, beta2, ref = np.tanh(t + a) * np.tanh(a - t), 1.0, np.exp(1j) * np.tanh(t + a) * np.tanh(a - t)
+    else:
+        x, beta2, ref = 1 / np.cosh(t), -1.0, np.exp(0.5j) / np.cosh(t)
+    y, _ = O.ssfm_run(O.fiber(0.0, beta2, 1.0, 1.0, 0.0, 1.0), O.grid(O.UNIFORM, steps), O.FORWARD,
+                      x.astype(np.complex128), dt)
+    e_ref = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    print(f"{kind} soliton: task {err:.3e}, oracle {e_ref:.3e}")
+    assert err == pytest.approx(e_ref, rel=1e-6)
+    assert err < 1e-5
 
 
 @pytest.mark.parametrize("logn,modes,steps", [(12, 1, 3), (14, 1, 0), (11, 2, 2)])

@@ -37,6 +37,32 @@ def test_fft_vs_numpy(n):
     assert np.linalg.norm(z - refi) <= 1e-14 * np.linalg.norm(refi)
 
 
+# lengths that are not powers of two: mixed radix 3/5/7 and odd primes, and
+# primes above 64, which take the Bluestein path (FFTW plans any N,
+# field.cpp:530-541)
+ODD_LENGTHS = [3, 5, 6, 7, 9, 12, 15, 21, 25, 35, 45, 49, 60, 63, 67, 97, 127, 131, 134, 210, 257, 1000]
+
+
+@pytest.mark.parametrize("n", ODD_LENGTHS)
+def test_fft_any_length_vs_bruteforce_long_double(n):
+    x = crandn(rng(n + 7), n)
+    ref_f = O.dft_naive(x, -1)
+    ref_i = O.dft_naive(x, +1)
+    scale = np.max(np.abs(ref_f))
+    assert np.max(np.abs(O.fft(x) - ref_f / n)) < 1e-14 * np.max(np.abs(ref_f / n)) * max(1.0, np.log2(n))
+    assert np.max(np.abs(O.ifft(x) - ref_i)) < 1e-14 * scale * max(1.0, np.log2(n))
+
+
+@pytest.mark.parametrize("n", [3 << 10, 5 << 9, 7 << 9, 3 * 5 * 7 * 64, 4099, 2 * 4099, 2 * 16411, 10 ** 4])
+def test_fft_any_length_vs_numpy(n):
+    x = crandn(rng(n), 2 * n)
+    y = O.fft(x, modes=2)
+    ref = np.concatenate([np.fft.fft(x[:n]), np.fft.fft(x[n:])]) / n
+    assert np.linalg.norm(y - ref) <= 1e-14 * np.linalg.norm(ref) * np.log2(n)
+    z = O.ifft(y, modes=2)
+    assert np.linalg.norm(z - x) <= 1e-14 * np.linalg.norm(x) * np.log2(n)
+
+
 def test_fft_roundtrip_and_parseval():
     x = crandn(rng(2), 4096)
     X = O.fft(x)
