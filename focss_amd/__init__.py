@@ -58,6 +58,7 @@ _sig = {
     "focss_link_run": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, _P]),
     "focss_link_run_host": (C.c_int, [_P, C.c_int, C.POINTER(Link), _P, _P]),
     "focss_fft": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "focss_fft_noalloc": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "focss_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "focss_frequency_shift": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P]),
     "focss_bandpass_filter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P]),
@@ -227,6 +228,13 @@ class Context:
 
     def fft(self, buf, modes, samples, sign, precision=F64):
         _check(_lib.focss_fft(self.handle, precision, sign, modes, samples, buf.ptr))
+
+    def fft_noalloc(self, buf, modes, samples, sign, precision=F64):
+        """focss_fft without device allocation: the two-pass lengths use the
+        library's fixed 32 MiB work buffer of this device (larger fields are
+        refused); calls that overlap on one device race on it."""
+        _need(buf, modes * samples * (16 if precision == F64 else 8), "fft_noalloc")
+        _check(_lib.focss_fft_noalloc(self.handle, precision, sign, modes, samples, buf.ptr))
 
     # ---- transmitter / receiver DSP (device buffers, mode-major) ----
     def frequency_shift(self, buf, modes, samples, dt, frequency, precision=F64):

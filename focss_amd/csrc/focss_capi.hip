@@ -21,6 +21,13 @@
 
 using namespace focss_hip;
 
+// Work buffer of focss_fft_noalloc (the two-pass DFT's intermediate), one
+// per device for the life of the process.  Host code never uses this name as
+// an address: focss_ctx_create looks the device address up once per device
+// (hipGetSymbolAddress) and the entry point passes that address on.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 // plain 16-byte copy: the calibration kernel for HBM byte counters
 __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -82,6 +89,7 @@ struct focss_ctx {
     GraphCache graph;
     void* scratch = nullptr;  // out-of-place buffer of the two-pass DFT
     size_t scratch_bytes = 0;
+    void* noalloc_scratch = nullptr;  // device address of `scratch` on this context's device
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
     long long max_split_steps = 20000000;
     bool use_graph = true;  // FOCSS_NO_GRAPH=1: plain launches (profilers)
@@ -98,6 +106,23 @@ struct focss_ctx {
 };
 
 namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;  // device -> address of `scratch`
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+int scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it == g_scratch_addr.end()) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        it = g_scratch_addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return 0;
+}
 
 template <typename R>
 void host_tables(std::vector<cx<R>>& bank, std::vector<cx<R>>& fine) {
@@ -465,9 +490,12 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
 }  // namespace
 
 namespace {
-// stream-ordered DFT of `modes` rows of `samples` (forward x 1/N)
+// stream-ordered DFT of `modes` rows of `samples` (forward x 1/N).  The
+// two-pass lengths need a work buffer of the field's size: `work` when given
+// (it must hold modes x samples values), else the context's buffer, grown on
+// demand.
 template <typename R>
-int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data) {
+int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data, void* work = nullptr) {
     int log_n = ilog2_int(samples);
     if (log_n < 1 || log_n > 20) return fail(FOCSS_E_ARG, "fft: samples must be a power of two in [2, 2^20]");
     if (modes < 1) return fail(FOCSS_E_ARG, "fft: modes must be >= 1");
@@ -485,7 +513,7 @@ int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data) {
         hipLaunchKernelGGL(r.whole[log_n][si], dim3(modes), dim3(NT), 0, c->stream, d, scale, tb);
     } else {
         size_t bytes = (size_t)modes * samples * sizeof(cx<R>);
-        if (c->scratch_bytes < bytes) {
+        if (!work && c->scratch_bytes < bytes) {
             HIP_TRY(hipStreamSynchronize(c->stream));  // earlier work may still use the old buffer
             if (c->scratch) HIP_TRY(hipFree(c->scratch));
             c->scratch = nullptr;
@@ -493,22 +521,23 @@ int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data) {
             HIP_TRY(hipMalloc(&c->scratch, bytes));
             c->scratch_bytes = bytes;
         }
+        cx<R>* w = (cx<R>*)(work ? work : c->scratch);
         int n1 = 1 << (log_n / 2);
         int n2 = samples / n1;
         int tc = n2 / r.ncols[log_n];
         int tr = n1 / r.nrows[log_n];
-        hipLaunchKernelGGL(r.cols[log_n][si], dim3(modes * tc), dim3(NT), 0, c->stream, (const cx<R>*)d,
-                           (cx<R>*)c->scratch, log_n, tc, tb);
-        hipLaunchKernelGGL(r.rows[log_n][si], dim3(modes * tr), dim3(NT), 0, c->stream,
-                           (const cx<R>*)c->scratch, d, log_n, tr, scale, tb);
+        hipLaunchKernelGGL(r.cols[log_n][si], dim3(modes * tc), dim3(NT), 0, c->stream, (const cx<R>*)d, w, log_n,
+                           tc, tb);
+        hipLaunchKernelGGL(r.rows[log_n][si], dim3(modes * tr), dim3(NT), 0, c->stream, (const cx<R>*)w, d, log_n,
+                           tr, scale, tb);
     }
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 template <typename R>
-int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data) {
-    if (int rc = fft_enqueue<R>(c, sign, modes, samples, data)) return rc;
+int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data, void* work = nullptr) {
+    if (int rc = fft_enqueue<R>(c, sign, modes, samples, data, work)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1002,6 +1031,7 @@ int focss_ctx_create(int device, focss_ctx** out) {
     }
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    if (int rc = scratch_address(device, &c->noalloc_scratch)) return rc;
     std::vector<cx<double>> b64, f64;
     std::vector<cx<float>> b32, f32;
     host_tables<double>(b64, f64);
@@ -1173,6 +1203,22 @@ int focss_fft(focss_ctx* c, int precision, int sign, int modes, int samples, voi
     if (precision == FOCSS_F64) return fft_impl<double>(c, sign, modes, samples, data);
     if (precision == FOCSS_F32) return fft_impl<float>(c, sign, modes, samples, data);
     return fail(FOCSS_E_ARG, "bad precision");
+}
+
+int focss_fft_noalloc(focss_ctx* c, int precision, int sign, int modes, int samples, void* data) {
+    if (!c || !data) return fail(FOCSS_E_ARG, "null argument");
+    if (sign != -1 && sign != 1) return fail(FOCSS_E_ARG, "fft: sign must be -1 or +1");
+    if (precision != FOCSS_F64 && precision != FOCSS_F32) return fail(FOCSS_E_ARG, "bad precision");
+    if (modes < 1 || samples < 1) return fail(FOCSS_E_ARG, "fft: modes and samples must be >= 1");
+    const size_t bytes = (size_t)modes * (size_t)samples * (precision == FOCSS_F64 ? 16 : 8);
+    if (bytes > kNoAllocScratchBytes)
+        return fail(FOCSS_E_ARG, "focss_fft_noalloc: the field is " + std::to_string(bytes) +
+                                     " bytes, above its " + std::to_string(kNoAllocScratchBytes) +
+                                     "-byte work buffer (use focss_fft)");
+    if (!c->noalloc_scratch) return fail(FOCSS_E_HIP, "focss_fft_noalloc: no work buffer on this device");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == FOCSS_F64) return fft_impl<double>(c, sign, modes, samples, data, c->noalloc_scratch);
+    return fft_impl<float>(c, sign, modes, samples, data, c->noalloc_scratch);
 }
 
 int focss_add_noise(focss_ctx* c, int precision, int modes, int samples, double sigma, uint64_t seed,

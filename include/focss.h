@@ -138,6 +138,16 @@ int focss_link_run_host(focss_ctx* ctx, int precision, const focss_link* link,
  * (Field::ifft_inplace, src/focss/field.cpp:349-352). */
 int focss_fft(focss_ctx* ctx, int precision, int sign, int modes, int samples, void* data);
 
+/* focss_fft without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as focss_fft; the intermediate of the two-pass lengths
+ * (above 2^13) lives in a fixed 32 MiB buffer of the library, one per device,
+ * held until the process ends, so modes x samples x element size must not
+ * exceed 32 MiB (FOCSS_E_ARG otherwise).  Calls that overlap on one device
+ * (several contexts or host threads) race on that buffer. */
+int focss_fft_noalloc(focss_ctx* ctx, int precision, int sign, int modes, int samples, void* data);
+
 /* Amplifier::add_noise (src/focss/module/amplifier.cpp:20-28) on a device
  * field: element i (flat, mode-major) += sigma * (z0 + i z1), (z0, z1) the
  * Box-Muller pair of Philox4x32-10 block (i, stream) under key seed. */
