@@ -76,6 +76,7 @@ _sig = {
     "focss_tile_variants": (C.c_int, [C.c_int, C.c_int]),
     "focss_build_info": (C.c_char_p, []),
     "focss_link_profile": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, C.c_int, _P]),
+    "focss_link_profile_graph": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, C.c_int, _P]),
     "focss_device_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "focss_ctx_stream": (_P, [_P]),
     "focss_link_validate": (C.c_int, [C.POINTER(Link)]),
@@ -219,6 +220,17 @@ class Context:
         _check(_lib.focss_link_profile(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
                                        n_pairs, out.ctypes.data))
         return out
+
+    def link_profile_graph(self, links, batch, offsets, n_replays, precision=F64):
+        """ms per (spectral, time) pass pair over n_replays replays of the
+        link's pass-pair graph, one event pair around them all."""
+        n = len(links)
+        arr = (Link * n)(*links)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        out = C.c_double(0)
+        _check(_lib.focss_link_profile_graph(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
+                                             n_replays, C.byref(out)))
+        return out.value
 
     def device_copy(self, dst, src, nbytes):
         _check(_lib.focss_device_copy(self.handle, dst.ptr, src.ptr, nbytes))

@@ -59,7 +59,7 @@ __device__ __forceinline__ void load_tw_split(cx<R>* lo, cx<R>* hi, const cx<R>*
     for (int i = threadIdx.x; i < L / (1 << LB); i += NT) hi[i] = src[i << LB];
 }
 
-// radix-2/4/8 DFT in registers, exponent sign SIGN (-1 forward, +1 inverse)
+// radix-2/4/8/16 DFT in registers, exponent sign SIGN (-1 forward, +1 inverse)
 template <typename R, int RADIX, int SIGN>
 struct butterfly;
 
@@ -107,6 +107,42 @@ struct butterfly<R, 8, SIGN> {
         for (int k = 0; k < 4; ++k) {
             v[2 * k] = a[k];
             v[2 * k + 1] = b[k];
+        }
+    }
+};
+
+template <typename R, int SIGN>
+struct butterfly<R, 16, SIGN> {
+    // 16 = 4 x 4: length-4 DFTs over n1 (stride 4), twiddles W16^{n2 k1},
+    // length-4 DFTs over n2; output k1 + 4 k2
+    static __device__ __forceinline__ void run(cx<R>* v) {
+        const R h = (R)0.70710678118654752440084436210484904;
+        const R c1 = (R)0.92387953251128675612818318939678829;  // cos(pi/8)
+        const R s1 = (R)0.38268343236508977172845998403039887;  // sin(pi/8)
+        cx<R> a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = v[n2 + 4 * n1];
+            butterfly<R, 4, SIGN>::run(a[n2]);
+        }
+        // a[n2][k1] *= W16^{n2 k1}, W16 = exp(SIGN 2 pi i / 16)
+        const cx<R> w1 = mk<R>(c1, SIGN * s1), w2 = mk<R>(h, SIGN * h), w3 = mk<R>(s1, SIGN * c1);
+        a[1][1] = a[1][1] * w1;
+        a[1][2] = a[1][2] * w2;
+        a[1][3] = a[1][3] * w3;
+        a[2][1] = a[2][1] * w2;
+        a[2][2] = mul_si<R, SIGN>(a[2][2]);
+        a[2][3] = mul_si<R, SIGN>(a[2][3] * w2);
+        a[3][1] = a[3][1] * w3;
+        a[3][2] = mul_si<R, SIGN>(a[3][2] * w2);
+        a[3][3] = (a[3][3] * w1) * (R)-1;  // W16^9 = -W16
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cx<R> b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            butterfly<R, 4, SIGN>::run(b);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
         }
     }
 };
@@ -182,8 +218,12 @@ template <int L, bool REV>
 struct Sched {
     static constexpr int P = ilog2<L>::value;
     static constexpr int REM = P % 3;
-    static constexpr int N = P / 3 + (REM ? 1 : 0);
-    static constexpr int fwd_radix(int i) { return (REM && i == 1) ? (1 << REM) : 8; }
+    // log2 L = 3k + 1 with k >= 3: the remainder joins a radix-8 stage as one
+    // radix-16 middle stage, (8, 16, 8, ...), one LDS round trip fewer than
+    // (8, 2, 8, 8, ...)
+    static constexpr bool R16 = REM == 1 && P >= 10;
+    static constexpr int N = R16 ? (P - 1) / 3 : P / 3 + (REM ? 1 : 0);
+    static constexpr int fwd_radix(int i) { return i == 1 ? (R16 ? 16 : (REM ? (1 << REM) : 8)) : 8; }
     static constexpr int radix(int i) { return REV ? fwd_radix(N - 1 - i) : fwd_radix(i); }
     static constexpr int ns(int i) {
         int p = 1;

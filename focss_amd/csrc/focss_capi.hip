@@ -93,6 +93,7 @@ struct focss_ctx {
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
     long long max_split_steps = 20000000;
     bool use_graph = true;  // FOCSS_NO_GRAPH=1: plain launches (profilers)
+    bool natural_layout = false;  // FOCSS_NATURAL_LAYOUT=1: no row-pair interleave between passes
     int n_cu = 256;          // compute units of the device
     void* ldbp_frag = nullptr;  // learned-DBP inference: matrix-core operands of every step
     size_t ldbp_frag_bytes = 0;
@@ -201,6 +202,7 @@ struct LengthClass {
     int log_n, modes, resident;
     int variant;       // time pass tile variant
     int spec_variant;  // spectral pass tile variant
+    int layout_g;      // intermediate layout of the two-pass programs (Interleave)
     int first;  // index into the waves array
     int count;
 };
@@ -209,10 +211,11 @@ template <typename R>
 void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
     const Registry<R>& r = reg<R>();
     int cols = r.time_cols[lc.log_n][lc.modes - 1][lc.variant];
-    int n2 = 1 << (lc.log_n - lc.log_n / 2);
+    int n2 = 1 << (lc.log_n - pass_log_n1(lc.log_n));
     a.tiles_per_wave = n2 / cols;
     a.waves += lc.first;
     a.log_n = lc.log_n;
+    a.layout_g = lc.layout_g;
     KernelFn<R> k = r.time_k[lc.log_n][lc.modes - 1][lc.variant];
     hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]),
                        0, c->stream, a, tb);
@@ -222,13 +225,27 @@ template <typename R>
 void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
     const Registry<R>& r = reg<R>();
     int rows = r.spec_rows[lc.log_n][lc.spec_variant];
-    int n1 = 1 << (lc.log_n / 2);
+    int n1 = 1 << pass_log_n1(lc.log_n);
     a.tiles_per_wave = lc.modes * n1 / rows;
     a.waves += lc.first;
     a.log_n = lc.log_n;
+    a.layout_g = lc.layout_g;
     KernelFn<R> k = r.spec_k[lc.log_n][lc.spec_variant];
     hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0,
                        c->stream, a, tb);
+}
+
+// natural order back after the programs of an interleaved class end
+template <typename R>
+void launch_deinterleave(focss_ctx* c, const LengthClass& lc, PassArgs a) {
+    const int n1 = 1 << pass_log_n1(lc.log_n);
+    const int n2 = 1 << (lc.log_n - pass_log_n1(lc.log_n));
+    a.tiles_per_wave = lc.modes * n1 / 2;
+    a.waves += lc.first;
+    a.log_n = lc.log_n;
+    a.layout_g = lc.layout_g;
+    hipLaunchKernelGGL(k_deinterleave<R>, dim3(lc.count * a.tiles_per_wave), dim3(256),
+                       (size_t)2 * n2 * sizeof(cx<R>), c->stream, a, n2);
 }
 
 template <typename R>
@@ -291,7 +308,8 @@ int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
 
 template <typename R>
 int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void* batch,
-              const int64_t* offsets, int* total_steps, int profile_pairs = 0, double* prof_ms = nullptr) {
+              const int64_t* offsets, int* total_steps, int profile_pairs = 0, double* prof_ms = nullptr,
+              int graph_replays = 0) {
     std::vector<WaveStatic> st(n);
     for (int i = 0; i < n; ++i) {
         int rc = fill_static(links[i], offsets[i], st[i]);
@@ -311,6 +329,10 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         lc.spec_variant = usable_variant(false, precision, lc.log_n, lc.modes, c->tile_variant[1][precision][lc.log_n]);
         lc.first = (int)order.size();
         lc.count = (int)kv.second.size();
+        lc.layout_g = 1;
+        if (lc.spec_variant >= 0 && !c->natural_layout &&
+            reg<R>().spec_rows[lc.log_n][lc.spec_variant] % 2 == 0)
+            lc.layout_g = 2;
         for (int i : kv.second) order.push_back(i);
         if (resident_ok(precision, lc.log_n, lc.modes)) {
             lc.resident = 1;
@@ -341,6 +363,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     a.waves = c->d_waves;
     a.n_waves = n;
     a.counter = 0;
+    a.layout_g = 1;
     if (profile_pairs > 0) {
         // measurement mode: the first profile_pairs split steps launched one
         // kernel at a time, each bracketed by events on the context stream
@@ -442,6 +465,24 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         // entry time pass (counter 0), then replays of the graph
         for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
         HIP_TRY(hipGetLastError());
+        if (graph_replays > 0) {
+            // measurement mode: `graph_replays` replays of the pass-pair
+            // graph between one event pair; the program is left unfinished
+            hipEvent_t e0, e1;
+            HIP_TRY(hipEventCreate(&e0));
+            HIP_TRY(hipEventCreate(&e1));
+            HIP_TRY(hipEventRecord(e0, c->stream));
+            for (int rep = 0; rep < graph_replays; ++rep) HIP_TRY(hipGraphLaunch(c->graph.exec, c->stream));
+            HIP_TRY(hipEventRecord(e1, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (!resident.empty()) HIP_TRY(hipStreamSynchronize(c->side));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            prof_ms[0] = ms / ((double)graph_replays * c->graph_pairs);
+            return 0;
+        }
         // two replays in flight: replay k+1 is queued before the done
         // counter read after replay k is inspected, so the GPU never waits
         // on the host; surplus passes after completion exit at once
@@ -470,6 +511,9 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                 return fail(FOCSS_E_NUMERIC, "link program did not terminate within the split-step limit");
         }
     }
+    for (const LengthClass& lc : two_pass)
+        if (lc.layout_g == 2) launch_deinterleave<R>(c, lc, a);
+    HIP_TRY(hipGetLastError());
     if (!resident.empty()) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (total_steps) {
         std::vector<WaveState> fin(2 * (size_t)n);
@@ -1016,6 +1060,8 @@ int focss_ctx_create(int device, focss_ctx** out) {
     c->device = device;
     const char* ng = getenv("FOCSS_NO_GRAPH");
     c->use_graph = !(ng && ng[0] == '1');
+    const char* nl = getenv("FOCSS_NATURAL_LAYOUT");
+    c->natural_layout = nl && nl[0] == '1';
     for (int p = 0; p <= 20; ++p) {
         // measured defaults at 2^18 samples: time pass 2048 (complex128) /
         // 4096 (complex64) elements per tile, spectral pass 1024
@@ -1174,6 +1220,19 @@ int focss_link_profile(focss_ctx* c, int precision, int n_waves, const focss_lin
         return run_links<double>(c, precision, n_waves, links, batch, offsets, nullptr, n_pairs, avg_ms);
     if (precision == FOCSS_F32)
         return run_links<float>(c, precision, n_waves, links, batch, offsets, nullptr, n_pairs, avg_ms);
+    return fail(FOCSS_E_ARG, "bad precision");
+}
+
+int focss_link_profile_graph(focss_ctx* c, int precision, int n_waves, const focss_link* links, void* batch,
+                             const int64_t* offsets, int n_replays, double* ms_per_pair) {
+    if (!c || !links || !batch || !offsets || !ms_per_pair || n_waves < 1 || n_replays < 1)
+        return fail(FOCSS_E_ARG, "null argument");
+    if (!c->use_graph) return fail(FOCSS_E_UNSUPPORTED, "graph replay timing with FOCSS_NO_GRAPH set");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == FOCSS_F64)
+        return run_links<double>(c, precision, n_waves, links, batch, offsets, nullptr, 0, ms_per_pair, n_replays);
+    if (precision == FOCSS_F32)
+        return run_links<float>(c, precision, n_waves, links, batch, offsets, nullptr, 0, ms_per_pair, n_replays);
     return fail(FOCSS_E_ARG, "bad precision");
 }
 

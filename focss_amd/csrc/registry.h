@@ -12,6 +12,14 @@ template <typename R>
 using KernelFn = void (*)(PassArgs, Tables<R>);
 
 constexpr int kTileVariants = 3;
+
+#ifndef FOCSS_SPLIT_SHIFT
+#define FOCSS_SPLIT_SHIFT 0
+#endif
+// log2 N1 of the two-pass split N = N1 x N2 of the SSFM passes: the time
+// pass runs length-N1 DFTs down columns, the spectral pass length-N2 DFTs
+// along rows.
+constexpr int pass_log_n1(int log_n) { return log_n / 2 - (log_n >= 16 ? FOCSS_SPLIT_SHIFT : 0); }
 constexpr int kTileElems[kTileVariants] = {4096, 2048, 1024};
 
 template <typename R>
@@ -34,8 +42,8 @@ void reg_resident(Registry<R>& r) {
 
 template <typename R, int LOGN, int M, int V>
 void reg_time(Registry<R>& r) {
-    constexpr int N1 = 1 << (LOGN / 2);
-    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int N1 = 1 << pass_log_n1(LOGN);
+    constexpr int N2 = 1 << (LOGN - pass_log_n1(LOGN));
     constexpr int C0 = kTileElems[V] / (N1 * M);
     constexpr int C = C0 < 1 ? 0 : (C0 > N2 ? N2 : C0);
     if constexpr (C >= 1) {
@@ -47,7 +55,7 @@ void reg_time(Registry<R>& r) {
 
 template <typename R, int LOGN, int V>
 void reg_spec(Registry<R>& r) {
-    constexpr int N2 = 1 << (LOGN - LOGN / 2);
+    constexpr int N2 = 1 << (LOGN - pass_log_n1(LOGN));
     constexpr int S0 = kTileElems[V] / N2;
     constexpr int S = S0 < 1 ? 0 : S0;
     if constexpr (S >= 1) {

@@ -4,6 +4,12 @@
 workload (a batch of 5-channel WDM fields, 2^18 samples, adaptive 80 km
 spans) with one kernel launch per pass, then one plain 16-byte copy kernel
 over `copy_mib` MiB, whose counters calibrate HBM bytes.
+--workload roofline: for the benchmark workload at each batch of
+`--batches`, the mean time-pass and spectral-pass durations (one kernel per
+launch, HIP events), the time per pass pair when the same passes run as
+replays of the captured graph (one event pair around `--replays` replays),
+and the plain 16-byte copy kernel over the field's bytes (host clock over
+`reps` back-to-back copies), all printed as GB/s of algorithmic traffic.
 --workload side: the other hot kernels at their measured sizes, `reps`
 launches each --
   learned DBP matrix-core inference, one polarisation: 6 steps/span x 20
@@ -33,11 +39,17 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
     ap.add_argument("--variant", type=int, default=-1, help="tile variant (default: the library's choice)")
-    ap.add_argument("--workload", choices=["ssfm", "side"], default="ssfm")
+    ap.add_argument("--workload", choices=["ssfm", "side", "roofline"], default="ssfm")
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--batches", default="64,128")
+    ap.add_argument("--replays", type=int, default=4)
+    ap.add_argument("--time-variant", type=int, default=-1)
+    ap.add_argument("--spec-variant", type=int, default=-1)
     args = ap.parse_args()
     if args.workload == "side":
         return side(args)
+    if args.workload == "roofline":
+        return roofline(args)
     sys.path.insert(0, ROOT)
     import bench
     import focss_amd as F
@@ -65,6 +77,59 @@ def main():
         ctx.device_copy(b, a, nbytes)
         print(f"probe {args.precision} batch {args.batch} variant {args.variant}: time pass {ms[0] * 1e3:.2f} us, spectral pass {ms[1] * 1e3:.2f} us, "
               f"field bytes {host.nbytes}, copy bytes {nbytes}", flush=True)
+
+
+def roofline(args):
+    sys.path.insert(0, ROOT)
+    import time
+    import bench
+    import focss_amd as F
+    from focss_amd.transmitter import wdm_signal
+
+    prec = F.F64 if args.precision == "f64" else F.F32
+    esz = 16 if prec == F.F64 else 8
+    batches = [int(b) for b in args.batches.split(",")]
+    base = []
+    for b in range(4):
+        f, dt, _ = wdm_signal(bench.SEED + 7919 * b, bench.N_CH, bench.N_SAMPLES // bench.OSF, bench.OSF,
+                              bench.BAUD, bench.SPACING, bench.POWER_DBM)
+        base.append(f)
+    with F.Context(0) as ctx:
+        log_n = int(np.log2(bench.N_SAMPLES))
+        if args.time_variant >= 0:
+            ctx.set_tile(prec, log_n, args.time_variant, pass_=0)
+        if args.spec_variant >= 0:
+            ctx.set_tile(prec, log_n, args.spec_variant, pass_=1)
+        for batch in batches:
+            host = np.concatenate([base[b % 4] for b in range(batch)])
+            if prec == F.F32:
+                host = host.astype(np.complex64)
+            offsets = np.arange(batch, dtype=np.int64) * bench.N_SAMPLES
+            links = bench.make_links(F, dt, batch)
+            buf = ctx.alloc(host.nbytes)
+            buf.upload(host)
+            ms = ctx.link_profile(links, buf, offsets, args.pairs, precision=prec)
+            buf.upload(host)
+            pair = ctx.link_profile_graph(links, buf, offsets, args.replays, precision=prec)
+            nb = 2.0 * host.nbytes  # one read and one write of the field per pass
+            other = ctx.alloc(host.nbytes)
+            ctx.device_copy(other, buf, host.nbytes)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.reps):
+                ctx.device_copy(other, buf, host.nbytes)
+            ctx.synchronize()
+            t_copy = (time.perf_counter() - t0) / args.reps
+            print(f"probe roofline {args.precision} batch {batch} ({host.nbytes / 2**20:.0f} MiB) "
+                  f"variants {args.time_variant}/{args.spec_variant}: "
+                  f"time pass {ms[0] * 1e3:.2f} us ({nb / ms[0] / 1e6:.0f} GB/s), "
+                  f"spectral pass {ms[1] * 1e3:.2f} us ({nb / ms[1] / 1e6:.0f} GB/s), "
+                  f"graph-replayed pair {pair * 1e3:.2f} us ({2 * nb / pair / 1e6:.0f} GB/s, "
+                  f"sum of separate passes {(ms[0] + ms[1]) * 1e3:.2f} us), "
+                  f"copy {t_copy * 1e6:.1f} us ({nb / t_copy / 1e9:.0f} GB/s)", flush=True)
+            other.free()
+            buf.free()
+    return 0
 
 
 def side(args):

@@ -115,6 +115,15 @@ int focss_link_run(focss_ctx* ctx, int precision, int n_waves, const focss_link*
 int focss_link_profile(focss_ctx* ctx, int precision, int n_waves, const focss_link* links,
                        void* batch, const int64_t* offsets, int n_pairs, double* avg_ms);
 
+/* Measurement: the link programs run as focss_link_run runs them (entry
+ * pass, then replays of the captured graph of pass pairs), but stop after
+ * n_replays replays; *ms_per_pair = the time between one event before the
+ * first replay and one after the last, divided by the number of (spectral,
+ * time) pass pairs replayed.  The batch is left part-way through its
+ * programs. */
+int focss_link_profile_graph(focss_ctx* ctx, int precision, int n_waves, const focss_link* links,
+                             void* batch, const int64_t* offsets, int n_replays, double* ms_per_pair);
+
 /* Host-only argument check of one link program (no device needed): the
  * guards the reference leaves to assert or to an endless loop -- h_max = 0
  * on the ADAPTIVE grid (ssfm.cpp:99), alpha = 0 on the LOGARITHMIC grid
