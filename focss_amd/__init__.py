@@ -48,6 +48,7 @@ class Link(C.Structure):
 _P = C.c_void_p
 _sig = {
     "focss_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "focss_inject_fault": (C.c_int, [C.c_int, C.c_int]),
     "focss_ctx_destroy": (C.c_int, [_P]),
     "focss_last_error": (C.c_char_p, []),
     "focss_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
@@ -587,6 +588,14 @@ class TripletMLP:
 def validate(link):
     """Host-side argument check; raises FocssError with the reason."""
     _check(_lib.focss_link_validate(C.byref(link)))
+
+
+FAULT_ALLOC = 1
+
+
+def inject_fault(kind, count):
+    """The next `count` device allocations fail (kind FAULT_ALLOC); 0 clears."""
+    _check(_lib.focss_inject_fault(kind, count))
 
 
 def tile_variants(precision, log2_samples):

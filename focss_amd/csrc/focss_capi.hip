@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -49,6 +50,17 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                      \
             return fail(FOCSS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Fault injection (focss_inject_fault): the next g_failing_allocs device
+// allocations of the library fail as an exhausted device would.
+std::atomic<int> g_failing_allocs{0};
+
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    for (int n = g_failing_allocs.load(); n > 0;)
+        if (g_failing_allocs.compare_exchange_weak(n, n - 1)) return hipErrorOutOfMemory;
+    return hipMalloc(p, bytes);
+}
 
 int ilog2_int(int n) {
     int p = 0;
@@ -143,19 +155,32 @@ void host_tables(std::vector<cx<R>>& bank, std::vector<cx<R>>& fine) {
     }
 }
 
+void free_control(focss_ctx* c) {
+    for (void* p : {(void*)c->d_statics, (void*)c->d_states, (void*)c->d_red, (void*)c->d_waves})
+        if (p) (void)hipFree(p);
+    c->d_statics = nullptr;
+    c->d_states = nullptr;
+    c->d_red = nullptr;
+    c->d_waves = nullptr;
+    c->cap_waves = 0;
+}
+
+// control arrays for n_waves waveforms; on failure the context holds none
+// and the next call tries again
 int ensure_capacity(focss_ctx* c, size_t n_waves) {
     if (n_waves <= c->cap_waves) return 0;
-    if (c->d_statics) {
-        (void)hipFree(c->d_statics);
-        (void)hipFree(c->d_states);
-        (void)hipFree(c->d_red);
-        (void)hipFree(c->d_waves);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->side));
+    free_control(c);
+    const size_t cap = n_waves < 64 ? 64 : n_waves;
+    hipError_t e = dev_malloc(&c->d_statics, cap * sizeof(WaveStatic));
+    if (e == hipSuccess) e = dev_malloc(&c->d_states, 2 * cap * sizeof(WaveState));
+    if (e == hipSuccess) e = dev_malloc(&c->d_red, 3 * cap * sizeof(Reduction));
+    if (e == hipSuccess) e = dev_malloc(&c->d_waves, cap * sizeof(int));
+    if (e != hipSuccess) {
+        free_control(c);
+        return fail(FOCSS_E_HIP, std::string("link control arrays: ") + hipGetErrorString(e));
     }
-    size_t cap = n_waves < 64 ? 64 : n_waves;
-    HIP_TRY(hipMalloc(&c->d_statics, cap * sizeof(WaveStatic)));
-    HIP_TRY(hipMalloc(&c->d_states, 2 * cap * sizeof(WaveState)));
-    HIP_TRY(hipMalloc(&c->d_red, 3 * cap * sizeof(Reduction)));
-    HIP_TRY(hipMalloc(&c->d_waves, cap * sizeof(int)));
     c->cap_waves = cap;
     return 0;
 }
@@ -515,19 +540,21 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         if (lc.layout_g == 2) launch_deinterleave<R>(c, lc, a);
     HIP_TRY(hipGetLastError());
     if (!resident.empty()) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    if (total_steps) {
-        std::vector<WaveState> fin(2 * (size_t)n);
-        HIP_TRY(hipMemcpyAsync(fin.data(), c->d_states, 2 * n * sizeof(WaveState), hipMemcpyDeviceToHost,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        for (int i = 0; i < n; ++i) {
-            // the larger count of the two buffers is the final one
-            int a0 = fin[i].total_steps, a1 = fin[n + i].total_steps;
-            total_steps[i] = a0 > a1 ? a0 : a1;
-        }
-    } else {
-        HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<WaveState> fin(2 * (size_t)n);
+    HIP_TRY(hipMemcpyAsync(fin.data(), c->d_states, 2 * n * sizeof(WaveState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::string faults;
+    for (int i = 0; i < n; ++i) {
+        // the larger count of the two buffers is the final one
+        const WaveState& s0 = fin[i];
+        const WaveState& s1 = fin[n + i];
+        const WaveState& last = s0.total_steps >= s1.total_steps ? s0 : s1;
+        if (total_steps) total_steps[i] = last.total_steps;
+        if ((s0.fault || s1.fault) && faults.size() < 200)
+            faults += " waveform " + std::to_string(i) + " (span " + std::to_string(last.span) + ", after " +
+                      std::to_string(last.total_steps) + " split steps);";
     }
+    if (!faults.empty()) return fail(FOCSS_E_NUMERIC, "non-finite field power, program stopped:" + faults);
     return 0;
 }
 
@@ -562,7 +589,7 @@ int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data, void
             if (c->scratch) HIP_TRY(hipFree(c->scratch));
             c->scratch = nullptr;
             c->scratch_bytes = 0;
-            HIP_TRY(hipMalloc(&c->scratch, bytes));
+            HIP_TRY(dev_malloc(&c->scratch, bytes));
             c->scratch_bytes = bytes;
         }
         cx<R>* w = (cx<R>*)(work ? work : c->scratch);
@@ -610,7 +637,7 @@ template <typename R>
 int upload_weights(focss_ctx* c, const std::vector<cx<double>>& d, void** dev) {
     std::vector<cx<R>> h(d.size());
     for (size_t i = 0; i < d.size(); ++i) h[i] = mk<R>((R)d[i].x, (R)d[i].y);
-    HIP_TRY(hipMalloc(dev, std::max<size_t>(1, h.size()) * sizeof(cx<R>)));
+    HIP_TRY(dev_malloc(dev, std::max<size_t>(1, h.size()) * sizeof(cx<R>)));
     HIP_TRY(hipMemcpyAsync(*dev, h.data(), h.size() * sizeof(cx<R>), hipMemcpyHostToDevice, c->stream));
     return 0;
 }
@@ -704,9 +731,9 @@ int zscore_normal_solve(focss_ctx* c, const cx<double>* F, const cx<double>* y, 
     cx<double>* G = nullptr;
     cx<double>* sums = nullptr;
     cx<double>* fy = nullptr;
-    hipError_t e = hipMalloc(&G, (size_t)nf * nf * sizeof(cx<double>));
-    if (e == hipSuccess) e = hipMalloc(&sums, (size_t)(nf + 1) * sizeof(cx<double>));
-    if (e == hipSuccess) e = hipMalloc(&fy, (size_t)nf * sizeof(cx<double>));
+    hipError_t e = dev_malloc(&G, (size_t)nf * nf * sizeof(cx<double>));
+    if (e == hipSuccess) e = dev_malloc(&sums, (size_t)(nf + 1) * sizeof(cx<double>));
+    if (e == hipSuccess) e = dev_malloc(&fy, (size_t)nf * sizeof(cx<double>));
     std::vector<cx<double>> hG((size_t)nf * nf), hs(nf + 1), hfy(nf);
     if (e == hipSuccess) e = hipMemsetAsync(G, 0, (size_t)nf * nf * sizeof(cx<double>), c->stream);
     if (e == hipSuccess) {
@@ -775,7 +802,7 @@ int dsp_workspace(focss_ctx* c, size_t bytes, char** out) {
         if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
         c->dsp_tmp = nullptr;
         c->dsp_tmp_bytes = 0;
-        HIP_TRY(hipMalloc(&c->dsp_tmp, bytes));
+        HIP_TRY(dev_malloc(&c->dsp_tmp, bytes));
         c->dsp_tmp_bytes = bytes;
     }
     *out = (char*)c->dsp_tmp;
@@ -1054,6 +1081,12 @@ const char* focss_build_info(void) {
            "two-pass four-step SSFM (2^12..2^20); FP64 and FP32 (FP64 phase generation)";
 }
 
+}  // extern "C"
+namespace {
+int ctx_init(focss_ctx* c);
+}
+extern "C" {
+
 int focss_ctx_create(int device, focss_ctx** out) {
     if (!out) return fail(FOCSS_E_ARG, "null out");
     focss_ctx* c = new focss_ctx();
@@ -1075,6 +1108,22 @@ int focss_ctx_create(int device, focss_ctx** out) {
         delete c;
         return fail(FOCSS_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
     }
+    if (int rc = ctx_init(c)) {
+        const std::string msg = g_err;
+        focss_ctx_destroy(c);
+        return fail(rc, msg);
+    }
+    *out = c;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// device resources of a new context; on failure focss_ctx_destroy releases
+// what was made
+int ctx_init(focss_ctx* c) {
+    const int device = c->device;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     if (int rc = scratch_address(device, &c->noalloc_scratch)) return rc;
@@ -1082,29 +1131,31 @@ int focss_ctx_create(int device, focss_ctx** out) {
     std::vector<cx<float>> b32, f32;
     host_tables<double>(b64, f64);
     host_tables<float>(b32, f32);
-    HIP_TRY(hipMalloc(&c->bank[0], b64.size() * sizeof(cx<double>)));
-    HIP_TRY(hipMalloc(&c->fine[0], f64.size() * sizeof(cx<double>)));
-    HIP_TRY(hipMalloc(&c->bank[1], b32.size() * sizeof(cx<float>)));
-    HIP_TRY(hipMalloc(&c->fine[1], f32.size() * sizeof(cx<float>)));
+    HIP_TRY(dev_malloc(&c->bank[0], b64.size() * sizeof(cx<double>)));
+    HIP_TRY(dev_malloc(&c->fine[0], f64.size() * sizeof(cx<double>)));
+    HIP_TRY(dev_malloc(&c->bank[1], b32.size() * sizeof(cx<float>)));
+    HIP_TRY(dev_malloc(&c->fine[1], f32.size() * sizeof(cx<float>)));
     HIP_TRY(hipMemcpy(c->bank[0], b64.data(), b64.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->fine[0], f64.data(), f64.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->bank[1], b32.data(), b32.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->fine[1], f32.data(), f32.size() * sizeof(cx<float>), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&c->d_done, sizeof(int)));
+    HIP_TRY(dev_malloc(&c->d_done, sizeof(int)));
     HIP_TRY(hipHostMalloc(&c->h_done, 2 * sizeof(int), hipHostMallocDefault));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_done[0], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_done[1], hipEventDisableTiming));
     HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-    *out = c;
     return 0;
 }
+}  // namespace
+
+extern "C" {
 
 int focss_ctx_destroy(focss_ctx* c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
     drop_graph(c);
     for (int i = 0; i < 2; ++i) {
@@ -1139,7 +1190,7 @@ int focss_ctx_destroy(focss_ctx* c) {
 int focss_device_alloc(focss_ctx* c, size_t bytes, void** dptr) {
     if (!c || !dptr) return fail(FOCSS_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipMalloc(dptr, bytes));
+    HIP_TRY(dev_malloc(dptr, bytes));
     return 0;
 }
 
@@ -1185,7 +1236,7 @@ int focss_link_run_host(focss_ctx* c, int precision, const focss_link* link, dou
     size_t esz = precision == FOCSS_F64 ? 16 : 8;
     void* d = nullptr;
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipMalloc(&d, n * esz));
+    HIP_TRY(dev_malloc(&d, n * esz));
     std::vector<float> tmp;
     if (precision == FOCSS_F64) {
         HIP_TRY(hipMemcpyAsync(d, field, n * esz, hipMemcpyHostToDevice, c->stream));
@@ -1264,6 +1315,12 @@ int focss_fft(focss_ctx* c, int precision, int sign, int modes, int samples, voi
     return fail(FOCSS_E_ARG, "bad precision");
 }
 
+int focss_inject_fault(int kind, int count) {
+    if (kind != FOCSS_FAULT_ALLOC || count < 0) return fail(FOCSS_E_ARG, "inject_fault: unknown kind or count < 0");
+    g_failing_allocs.store(count);
+    return 0;
+}
+
 int focss_fft_noalloc(focss_ctx* c, int precision, int sign, int modes, int samples, void* data) {
     if (!c || !data) return fail(FOCSS_E_ARG, "null argument");
     if (sign != -1 && sign != 1) return fail(FOCSS_E_ARG, "fft: sign must be -1 or +1");
@@ -1321,8 +1378,8 @@ int focss_sino_train(focss_ctx* c, int radius, int type, const double* linear_we
     if (rows < nf) return fail(FOCSS_E_ARG, "sino_train: fewer rows than features");
     cx<double>* F = nullptr;
     cx<double>* y = nullptr;
-    HIP_TRY(hipMalloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
-    if (hipMalloc(&y, (size_t)rows * sizeof(cx<double>)) != hipSuccess) {
+    HIP_TRY(dev_malloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
+    if (dev_malloc(&y, (size_t)rows * sizeof(cx<double>)) != hipSuccess) {
         (void)hipFree(F);
         return fail(FOCSS_E_HIP, "sino_train: out of device memory");
     }
@@ -1368,8 +1425,8 @@ int focss_wdm_ppe_train(focss_ctx* c, int c_radius, int d_radius, int samples, c
     if (rows < nf) return fail(FOCSS_E_ARG, "wdm_ppe_train: fewer rows than features");
     cx<double>* F = nullptr;
     cx<double>* y = nullptr;
-    HIP_TRY(hipMalloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
-    if (hipMalloc(&y, (size_t)rows * sizeof(cx<double>)) != hipSuccess) {
+    HIP_TRY(dev_malloc(&F, (size_t)rows * nf * sizeof(cx<double>)));
+    if (dev_malloc(&y, (size_t)rows * sizeof(cx<double>)) != hipSuccess) {
         (void)hipFree(F);
         return fail(FOCSS_E_HIP, "wdm_ppe_train: out of device memory");
     }
@@ -1454,7 +1511,7 @@ int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int mod
     ldbp_kernels(taps, kerr_taps, modes, &fwd, &bwd);
     double* dloss = nullptr;
     if (target) {
-        HIP_TRY(hipMalloc(&dloss, sizeof(double)));
+        HIP_TRY(dev_malloc(&dloss, sizeof(double)));
         HIP_TRY(hipMemsetAsync(dloss, 0, sizeof(double), c->stream));
     }
     LdbpArgs a = {};
@@ -1506,7 +1563,7 @@ int focss_ldbp_infer(focss_ctx* c, int steps, int taps, int kerr_taps, int modes
         if (c->ldbp_frag) (void)hipFree(c->ldbp_frag);
         c->ldbp_frag = nullptr;
         c->ldbp_frag_bytes = 0;
-        HIP_TRY(hipMalloc(&c->ldbp_frag, frag_bytes));
+        HIP_TRY(dev_malloc(&c->ldbp_frag, frag_bytes));
         c->ldbp_frag_bytes = frag_bytes;
     }
     ldbp_h8* afrag = (ldbp_h8*)c->ldbp_frag;
@@ -1581,7 +1638,7 @@ int focss_ldbp_train_step(focss_ctx* c, int steps, int taps, int kerr_taps, int 
         if (c->ldbp_acc) (void)hipFree(c->ldbp_acc);
         c->ldbp_acc = nullptr;
         c->ldbp_acc_bytes = 0;
-        HIP_TRY(hipMalloc(&c->ldbp_acc, acc_bytes));
+        HIP_TRY(dev_malloc(&c->ldbp_acc, acc_bytes));
         c->ldbp_acc_bytes = acc_bytes;
     }
     double* acc = c->ldbp_acc;
@@ -1627,7 +1684,7 @@ size_t tmlp_part_offset(size_t rows) {
 
 // the bf16 copy of W1 in the context (fixed size, allocated on first use)
 int tmlp_pack(focss_ctx* c, const float* params) {
-    if (!c->tmlp_w1b) HIP_TRY(hipMalloc(&c->tmlp_w1b, (size_t)TM_H * TM_KF * sizeof(unsigned short)));
+    if (!c->tmlp_w1b) HIP_TRY(dev_malloc(&c->tmlp_w1b, (size_t)TM_H * TM_KF * sizeof(unsigned short)));
     const int n = TM_H * TM_KF;
     hipLaunchKernelGGL(k_tmlp_pack, dim3((n + 255) / 256), dim3(256), 0, c->stream, params, c->tmlp_w1b, n);
     HIP_TRY(hipGetLastError());
@@ -1681,7 +1738,7 @@ int focss_tmlp_forward(focss_ctx* c, int modes, int symbols, int batch, const fl
     }
     HIP_TRY(hipGetLastError());
     if (!target) return 0;
-    if (!c->tmlp_acc) HIP_TRY(hipMalloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));
+    if (!c->tmlp_acc) HIP_TRY(dev_malloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));
     HIP_TRY(hipMemsetAsync(c->tmlp_acc, 0, sizeof(double), c->stream));
     const long long n = (long long)a.rows;
     const int grid = (int)std::min<long long>((n + TM_NT - 1) / TM_NT, 4096);
@@ -1731,7 +1788,7 @@ int focss_tmlp_train_step(focss_ctx* c, int modes, int symbols, int batch, float
     if (!c || !params || !adam_m || !adam_v || t < 1) return fail(FOCSS_E_ARG, "triplet MLP: bad argument");
     if (int rc = tmlp_check(modes, symbols, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
-    if (!c->tmlp_acc) HIP_TRY(hipMalloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));
+    if (!c->tmlp_acc) HIP_TRY(dev_malloc(&c->tmlp_acc, (size_t)(TM_NP + 1) * sizeof(double)));
     double* acc = c->tmlp_acc;
     HIP_TRY(hipMemsetAsync(acc, 0, (size_t)(TM_NP + 1) * sizeof(double), c->stream));
     int rc = focss_tmlp_gradients(c, modes, symbols, batch, params, in, target, workspace, workspace_bytes, acc + 1,
@@ -1815,7 +1872,7 @@ int linear_stats(focss_ctx* c, int precision, int modes, int samples, const void
     if (precision != FOCSS_F64 && precision != FOCSS_F32)
         return fail(FOCSS_E_ARG, "precision must be FOCSS_F64 or FOCSS_F32");
     HIP_TRY(hipSetDevice(c->device));
-    if (!c->lin_stats) HIP_TRY(hipMalloc(&c->lin_stats, 4 * sizeof(double)));
+    if (!c->lin_stats) HIP_TRY(dev_malloc(&c->lin_stats, 4 * sizeof(double)));
     HIP_TRY(hipMemsetAsync(c->lin_stats, 0, 4 * sizeof(double), c->stream));
     const long long n = (long long)modes * samples;
     const int grid = (int)std::min<long long>((n + LS_NT - 1) / LS_NT, 1024);

@@ -57,7 +57,7 @@ struct WaveState {
     int top;             // time-domain op to run next
     int noise_stream;    // ASE stream (span index) of that op, -1 = none
     int total_steps;     // split steps done over all spans
-    int pad;
+    int fault;           // 1: the field's power was not finite; the program stopped
     double d;            // distance done in the current span (plain double sum, ssfm.cpp:107)
     double h;            // |step| of the last planned step
     double pending;      // signed linear distance not yet applied
@@ -142,6 +142,15 @@ FOCSS_HD inline SpectralOp plan_spectral(const WaveStatic& w, WaveState& st, dou
     op.s = 0;
     op.gain = 1;
     if (st.phase == PH_DONE) {
+        st.top = TOP_NONE;
+        return op;
+    }
+    // a NaN or infinite sample makes the power sum non-finite (the reference
+    // propagates it silently; std::min(h_max, NaN) even keeps stepping,
+    // ssfm.cpp:139): stop the program and report it
+    if (!(sum <= 1.7976931348623157e308)) {
+        st.fault = 1;
+        st.phase = PH_DONE;
         st.top = TOP_NONE;
         return op;
     }
@@ -242,7 +251,7 @@ FOCSS_HD inline WaveState initial_state(const WaveStatic& w) {
     st.top = TOP_ENTRY;
     st.noise_stream = -1;
     st.total_steps = 0;
-    st.pad = 0;
+    st.fault = 0;
     st.d = 0;
     st.h = 0;
     st.pending = 0;

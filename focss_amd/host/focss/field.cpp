@@ -1,5 +1,6 @@
 #include "focss/field.h"
 
+#include <algorithm>
 #include <cassert>
 #include <cmath>
 
@@ -74,13 +75,20 @@ RealVector Field::spectral_power() const {
     return g;
 }
 
-// upsample / downsample act on the flat mode-major array     field.cpp:272-297
+// upsample / downsample act on the flat mode-major array, as the reference
+// does (field.cpp:272-297): upsampling puts value i at flat position
+// factor i (zeros between), downsampling keeps every factor-th value of the
+// flat array
 Field Field::upsample(const int& factor) const {
     Domain d = domain();
     d.samples *= factor;
     d.time_step /= factor;
     Field out(d);
-    for (int i = 0; i < size(); ++i) out.data_[(size_t)factor * i] = data_[i];
+    complex_t* o = out.raw();
+    for (const complex_t& v : data_) {
+        *o = v;
+        o += factor;
+    }
     return out;
 }
 
@@ -90,31 +98,37 @@ Field Field::downsample(const int& factor) const {
     d.samples /= factor;
     d.time_step *= factor;
     Field out(d);
-    for (int i = 0; i < out.size(); ++i) out.data_[i] = data_[(size_t)i * factor];
+    const complex_t* src = raw();
+    for (complex_t& v : out.data_) {
+        v = *src;
+        src += factor;
+    }
     return out;
 }
 
-// decimate: keep the lowest N/factor bins of the (1/N) spectrum   field.cpp:299-320
+// decimate (field.cpp:299-320): the lowest N/factor bins of the spectrum
+// (the first N/(2 factor) non-negative ones and the rest from the top), back
+// to the time domain at the lower rate; on the GPU (focss_decimate)
 Field Field::decimate(const int& factor) const {
     assert(samples_ % factor == 0);
     Domain d = domain();
     d.samples /= factor;
     d.time_step *= factor;
     Field out(d);
-    Field spectrum = fft();
-    int cutoff = samples_ / factor / 2;
-    int shift = samples_ - out.samples_;
-    for (int m = 0; m < modes_; ++m) {
-        for (int s = 0; s < cutoff; ++s) out(m, s) = spectrum(m, s);
-        for (int s = cutoff; s < out.samples_; ++s) out(m, s) = spectrum(m, s + shift);
-    }
-    return out.ifft_inplace();
+    DeviceBuffer in(raw(), sizeof(complex_t) * (size_t)size());
+    DeviceBuffer res(sizeof(complex_t) * (size_t)out.size());
+    check(focss_decimate(runtime(), FOCSS_F64, modes_, samples_, factor, in.ptr, res.ptr), "focss_decimate");
+    res.download(out.raw());
+    return out;
 }
 
+// swap the two halves of every mode (field.cpp:322-330)
 Field& Field::fft_shift() {
-    int half = samples_ / 2;
-    for (int m = 0; m < modes_; ++m)
-        for (int s = 0; s < half; ++s) std::swap(data_[s + m * samples_], data_[s + m * samples_ + half]);
+    const int half = samples_ / 2;
+    for (int m = 0; m < modes_; ++m) {
+        complex_t* row = raw() + (size_t)m * samples_;
+        std::swap_ranges(row, row + half, row + half);
+    }
     return *this;
 }
 
@@ -143,18 +157,15 @@ Field& Field::ifft_inplace() {
 Field Field::fft() const { return Field(*this).fft_inplace(); }
 Field Field::ifft() const { return Field(*this).ifft_inplace(); }
 
-// circular FFT filter, impulse response centred at sample 0   field.cpp:358-372
+// circular FFT filter with the impulse response centred at sample 0
+// (field.cpp:358-372; an odd-length response loses its last tap, as in the
+// reference); on the GPU (focss_filter)
 Field& Field::filter_inplace(const ComplexVector& h) {
-    ComplexVector zp(samples_, complex_t(0, 0));
-    int half = h.size() / 2;
-    int shift = samples_ - half;
-    for (int i = 0; i < h.size() / 2; ++i) {
-        zp[i] = h[i + half];
-        zp[shift + i] = h[i];
-    }
-    fft_inplace();
-    (*this) *= focss::fft(zp);
-    ifft_inplace();
+    DeviceBuffer dev(raw(), sizeof(complex_t) * (size_t)size());
+    check(focss_filter(runtime(), FOCSS_F64, modes_, samples_, h.size(), reinterpret_cast<const double*>(h.raw()),
+                       dev.ptr),
+          "focss_filter");
+    dev.download(raw());
     return *this;
 }
 

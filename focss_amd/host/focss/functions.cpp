@@ -6,6 +6,9 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "focss/runtime.h"
 
@@ -19,36 +22,46 @@ void reseed_global_urng() {
 
 void reseed_global_urng(uint64_t seed) { set_noise_seed(seed); }
 
-// evm2 / q2                                          functions.cpp:27-80
+// evm2 / q2 (functions.cpp:27-80): the error energy sum |tx - rx|^2 over
+// the reference energy sum |tx|^2, on every element or, with an edge cut,
+// on samples [cut, samples - cut) of every mode
+namespace {
+struct Energies {
+    double error = 0, reference = 0;
+    void add(const complex_t& t, const complex_t& r) {
+        error += std::norm(t - r);
+        reference += std::norm(t);
+    }
+    double ratio() const { return error / reference; }
+};
+}  // namespace
+
 double evm2_factor(const ComplexVector& tx, const ComplexVector& rx) {
     assert(tx.size() == rx.size());
-    double num = 0, den = 0;
-    for (int i = 0; i < tx.size(); ++i) {
-        num += std::norm(tx[i] - rx[i]);
-        den += std::norm(tx[i]);
-    }
-    return num / den;
+    Energies e;
+    const complex_t* t = tx.raw();
+    const complex_t* r = rx.raw();
+    for (int i = 0; i < tx.size(); ++i) e.add(t[i], r[i]);
+    return e.ratio();
 }
 
 double evm2_factor(const Field& tx, const Field& rx) {
     assert(tx.size() == rx.size());
-    double num = 0, den = 0;
-    for (int i = 0; i < tx.size(); ++i) {
-        num += std::norm(tx[i] - rx[i]);
-        den += std::norm(tx[i]);
-    }
-    return num / den;
+    Energies e;
+    for (int i = 0; i < tx.size(); ++i) e.add(tx.raw()[i], rx.raw()[i]);
+    return e.ratio();
 }
 
 double evm2_factor(const Field& tx, const Field& rx, const int& cut) {
     assert(tx.modes() == rx.modes() && tx.samples() == rx.samples());
-    double num = 0, den = 0;
-    for (int m = 0; m < tx.modes(); ++m)
-        for (int s = cut; s < tx.samples() - cut; ++s) {
-            num += std::norm(tx(m, s) - rx(m, s));
-            den += std::norm(tx(m, s));
-        }
-    return num / den;
+    Energies e;
+    const int n = tx.samples();
+    for (int m = 0; m < tx.modes(); ++m) {
+        const complex_t* t = tx.raw() + (size_t)m * n;
+        const complex_t* r = rx.raw() + (size_t)m * n;
+        for (int s = cut; s < n - cut; ++s) e.add(t[s], r[s]);
+    }
+    return e.ratio();
 }
 
 double q2_factor(const ComplexVector& tx, const ComplexVector& rx) { return -10 * std::log10(evm2_factor(tx, rx)); }
@@ -94,32 +107,39 @@ ComplexVector ifft(const ComplexVector& d) {
     return c;
 }
 
-// CSV transmission files: per sample tx modes then rx modes as re,im
-// (functions.cpp:174-221).  Written with 17 significant digits so doubles
-// round-trip (the reference's precision(15) does not).
+// CSV transmission files (functions.cpp:174-221): one line per sample,
+// the real and imaginary parts of tx mode 0 .. M-1, then of rx mode
+// 0 .. M-1, separated by commas.  Written with 17 significant digits so
+// doubles round-trip (the reference's precision(15) does not).
 void save_transmission(std::ostream& out, const Field& tx, const Field& rx) {
     assert(tx.modes() == rx.modes() && tx.samples() == rx.samples());
-    for (int i = 0; i < tx.samples(); ++i) {
-        for (int m = 0; m < tx.modes(); ++m) out << tx(m, i).real() << ',' << tx(m, i).imag() << ',';
-        for (int m = 0; m < rx.modes() - 1; ++m) out << rx(m, i).real() << ',' << rx(m, i).imag() << ',';
-        out << rx(rx.modes() - 1, i).real() << ',' << rx(rx.modes() - 1, i).imag() << '\n';
+    const int modes = tx.modes();
+    for (int s = 0; s < tx.samples(); ++s) {
+        for (int k = 0; k < 2 * modes; ++k) {
+            const complex_t v = k < modes ? tx(k, s) : rx(k - modes, s);
+            out << v.real() << ',' << v.imag() << (k + 1 < 2 * modes ? ',' : '\n');
+        }
     }
 }
 
+// reads tx->samples() lines of 4 M comma-separated numbers
 void load_transmission(std::istream& in, Field* tx, Field* rx) {
     assert(tx->modes() == rx->modes() && tx->samples() == rx->samples());
-    double re, im;
-    char sep = ',';
-    for (int i = 0; i < tx->samples(); ++i) {
-        in >> re >> sep >> im;
-        (*tx)(0, i) = complex_t(re, im);
-        for (int m = 1; m < tx->modes(); ++m) {
-            in >> sep >> re >> sep >> im;
-            (*tx)(m, i) = complex_t(re, im);
+    const int modes = tx->modes();
+    std::vector<double> row(4 * (size_t)modes);
+    std::string line;
+    for (int s = 0; s < tx->samples(); ++s) {
+        if (!std::getline(in, line)) throw std::runtime_error("load_transmission: file ends early");
+        const char* p = line.c_str();
+        for (double& v : row) {
+            char* end = nullptr;
+            v = std::strtod(p, &end);
+            if (end == p) throw std::runtime_error("load_transmission: malformed line");
+            p = (*end == ',') ? end + 1 : end;
         }
-        for (int m = 0; m < tx->modes(); ++m) {
-            in >> sep >> re >> sep >> im;
-            (*rx)(m, i) = complex_t(re, im);
+        for (int m = 0; m < modes; ++m) {
+            (*tx)(m, s) = complex_t(row[2 * m], row[2 * m + 1]);
+            (*rx)(m, s) = complex_t(row[2 * (modes + m)], row[2 * (modes + m) + 1]);
         }
     }
 }

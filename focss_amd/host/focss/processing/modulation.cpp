@@ -38,8 +38,10 @@ unsigned bit_distance(unsigned a, unsigned b) { return (unsigned)__builtin_popco
 }  // namespace
 
 complex_t gray_16qam(const unsigned int& code) {
-    static const double norm = 1.0 / std::sqrt(10.0);
-    return complex_t(kAxisLevel[code & 3u], kAxisLevel[(code >> 2) & 3u]) * norm;
+    // divided (not multiplied by 1/sqrt(10)) so that the points, and with
+    // them the decisions on boundaries, round as the reference's table does
+    const double root10 = std::sqrt(10.0);
+    return complex_t(kAxisLevel[code & 3u] / root10, kAxisLevel[(code >> 2) & 3u] / root10);
 }
 
 // pulse::sech (modulation.cpp:19-31): 1/cosh(t / width) / width on a grid
@@ -68,8 +70,8 @@ ComplexVector pulse::lorentzian(const int& n_samples, const double& fwhm, const 
 
 // pulse::rrc (modulation.cpp:47-74): the root-raised-cosine impulse
 // response at t = i / osf - n/2 symbol periods, with the two removable
-// singularities (t = 0 and |t| = 1 / (4 beta)) at their limits, scaled so
-// that sum |h|^2 equals the number of samples
+// singularities (t = 0 and |t| = 1 / (4 beta)) at their limits, multiplied
+// by n / sum h^2 (n the number of samples), as the reference scales it
 ComplexVector pulse::rrc(const int& n_symbols, const int& oversampling, const double& roll_off) {
     const int n = n_symbols * oversampling;
     const double b = roll_off;
@@ -164,10 +166,9 @@ unsigned int demodulate_16qam(const complex_t& symbol) {
 // hard_decision_16qam (modulation.cpp:112-120): decide at unit power per
 // mode, return the decided points at the signal's power
 Field hard_decision_16qam(const Field& signal) {
-    const double p = power_per_mode(signal);
-    const double to_unit = 1.0 / std::sqrt(p), back = std::sqrt(p);
+    const double amp = std::sqrt(power_per_mode(signal));
     Field out(signal.domain());
-    for (int i = 0; i < signal.size(); ++i) out[i] = back * gray_16qam(demodulate_16qam(to_unit * signal[i]));
+    for (int i = 0; i < signal.size(); ++i) out[i] = amp * gray_16qam(demodulate_16qam(signal[i] / amp));
     return out;
 }
 
@@ -175,8 +176,7 @@ Field hard_decision_16qam(const Field& signal) {
 // that an axis within sqrt(10) * uncertainty of a decision boundary (0 or
 // +-2/sqrt(10) at unit power) keeps the received coordinate
 Field soft_decision_16qam(const Field& signal, const double& uncertainty) {
-    const double p = power_per_mode(signal);
-    const double to_unit = 1.0 / std::sqrt(p), back = std::sqrt(p);
+    const double amp = std::sqrt(power_per_mode(signal));
     const double boundary = 2.0 / std::sqrt(10.0);
     const double margin = std::sqrt(10.0) * uncertainty;
     auto near_boundary = [&](double v) {
@@ -184,11 +184,11 @@ Field soft_decision_16qam(const Field& signal, const double& uncertainty) {
     };
     Field out(signal.domain());
     for (int i = 0; i < signal.size(); ++i) {
-        const complex_t u = to_unit * signal[i];
+        const complex_t u = signal[i] / amp;
         complex_t g = gray_16qam(demodulate_16qam(u));
         const double re = near_boundary(u.real()) ? u.real() : g.real();
         const double im = near_boundary(u.imag()) ? u.imag() : g.imag();
-        out[i] = back * complex_t(re, im);
+        out[i] = amp * complex_t(re, im);
     }
     return out;
 }

@@ -17,6 +17,14 @@
 //   focss_tasks save-field <in> <set> <tag> <index> <channels> <modes> <samples>
 //                       (in: the channels' fields one after another)
 //   focss_tasks load-field <set> <tag> <index> <out>   (prints the header)
+//   focss_tasks modulation <in> <out> <samples> <modes> <dt> <osf> <channel_bw>
+//                       (out, one after another: decimate(osf), filter by
+//                        pulse::rrc(samples/osf, osf, 0.25), rrc_sampling(osf),
+//                        rrc_shaping(rrc_sampling), wdm_select(1, bw),
+//                        frequency_shift(bw/3), bandpass_filter(bw/2),
+//                        upsample(2), downsample(2), fft_shift,
+//                        hard_decision_16qam(rrc_sampling), wdm_mux of
+//                        {f, f/2, f/4} at bw)
 //   focss_tasks save-transmission <tx> <rx> <csv> <modes> <samples>
 //   focss_tasks load-transmission <csv> <tx> <rx> <modes> <samples>
 // <in>/<out>/<tx>/<rx> are raw little-endian complex<double> files in the
@@ -35,6 +43,7 @@
 #include "focss/equalizer/wdm_sino.h"
 #include "focss/field.h"
 #include "focss/functions.h"
+#include "focss/processing/modulation.h"
 #include "focss/runtime.h"
 #include "tasks/field_io.h"
 #include "tasks/soliton_check.h"
@@ -189,6 +198,32 @@ int main(int argc, char** argv) {
         }
         std::cout << h.channels << ' ' << h.samples << ' ' << h.modes << ' ' << h.flags << std::endl;
         write_field(argv[5], f);
+        return 0;
+    }
+    if (cmd == "modulation" && argc == 9) {
+        int samples = std::atoi(argv[4]), modes = std::atoi(argv[5]), osf = std::atoi(argv[7]);
+        double dt = std::atof(argv[6]), bw = std::atof(argv[8]);
+        focss::Field f = read_field(argv[2], samples, modes, dt, 0.0);
+        std::ofstream out(argv[3], std::ios::binary);
+        auto put = [&](const focss::Field& g) {
+            out.write(reinterpret_cast<const char*>(g.raw()), (std::streamsize)(g.size() * sizeof(focss::complex_t)));
+        };
+        focss::Field sampled = focss::rrc_sampling(f, osf);
+        put(f.decimate(osf));
+        put(f.filter(focss::pulse::rrc(samples / osf, osf, 0.25)));
+        put(sampled);
+        put(focss::rrc_shaping(sampled, osf));
+        put(focss::wdm_select(f, 1, bw));
+        put(focss::frequency_shift(f, bw / 3));
+        put(focss::bandpass_filter(f, bw / 2));
+        put(f.upsample(2));
+        put(f.downsample(2));
+        put(focss::Field(f).fft_shift());
+        put(focss::hard_decision_16qam(sampled));
+        focss::Vector<focss::Field> chans(3, f);
+        chans[1] = f * 0.5;
+        chans[2] = f * 0.25;
+        put(focss::wdm_mux(chans, bw));
         return 0;
     }
     if ((cmd == "save-transmission" || cmd == "load-transmission") && argc == 7) {

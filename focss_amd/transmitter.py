@@ -104,5 +104,8 @@ def phase_align(tx, rx):
 def demodulate_16qam(rx):
     """demodulate_16qam (modulation.cpp:98-110): index of the nearest point of
     the unit-power Gray constellation, ties to the lower code."""
-    d = np.abs(np.asarray(rx)[:, None] - GRAY16[None, :]) ** 2
+    rx = np.asarray(rx)
+    # squared distance as re^2 + im^2 of the difference (std::norm), so that
+    # ties and near-ties resolve as in the reference
+    d = (rx.real[:, None] - GRAY16.real[None, :]) ** 2 + (rx.imag[:, None] - GRAY16.imag[None, :]) ** 2
     return np.argmin(d, axis=1)  # argmin keeps the first (lowest) index on ties

@@ -30,6 +30,8 @@ extern "C" {
 #define FOCSS_E_UNSUPPORTED -3
 #define FOCSS_E_NUMERIC -4  /* non-finite result detected */
 
+#define FOCSS_FAULT_ALLOC 1 /* focss_inject_fault: device allocations */
+
 #define FOCSS_F64 0
 #define FOCSS_F32 1
 
@@ -82,6 +84,13 @@ typedef struct {
     uint64_t noise_seed;
 } focss_link;
 
+/* Fault injection for tests of the error paths (no counterpart in the
+ * reference, which has none): with kind FOCSS_FAULT_ALLOC the next `count`
+ * device allocations the library makes fail as an exhausted device would
+ * (FOCSS_E_HIP); count 0 clears it.  The failed call leaves its context
+ * usable. */
+int focss_inject_fault(int kind, int count);
+
 /* ---- context / memory ---- */
 int focss_ctx_create(int device, focss_ctx** out);
 int focss_ctx_destroy(focss_ctx* ctx);
@@ -93,6 +102,10 @@ int focss_memcpy_d2h(focss_ctx* ctx, void* dst, const void* src, size_t bytes);
 int focss_synchronize(focss_ctx* ctx);
 
 /* ---- the hot path ----
+ * A program whose field power stops being finite (a NaN or infinite sample
+ * in the input, or an overflow on the way) stops at that point; the call
+ * then returns FOCSS_E_NUMERIC naming the waveforms, after the others have
+ * finished.
  * Runs the link programs of n_waves waveforms resident in device memory.
  * Waveform i occupies modes*samples complex elements at
  * (complex*)batch + offsets[i] (element offsets).  Waveforms may differ in

@@ -105,3 +105,29 @@ def test_tmlp_and_equaliser_guards():
     assert lib.focss_pse_estimate(None, F.F64, 1, 16, None, None, 3600, ctypes.byref(a)) == -1
     assert b"linear equaliser" in lib.focss_last_error()
     assert lib.focss_scalar_estimate(None, F.F64, 1, 16, None, None, None) == -1
+
+
+def test_fault_injection_arguments():
+    """The injection hook is host-only: it validates its arguments without a
+    device."""
+    import focss_amd as F
+    F.inject_fault(F.FAULT_ALLOC, 0)
+    with pytest.raises(F.FocssError, match="inject_fault"):
+        F.inject_fault(7, 1)
+    with pytest.raises(F.FocssError, match="inject_fault"):
+        F.inject_fault(F.FAULT_ALLOC, -1)
+
+
+def test_buffer_size_guard_refuses_small_buffers():
+    """Wrappers refuse device buffers smaller than the call touches (the
+    C-ABI takes no sizes)."""
+    import focss_amd as F
+
+    class Fake:
+        nbytes = 100
+
+    F._need(Fake(), 100, "x")
+    with pytest.raises(F.FocssError, match="needed"):
+        F._need(Fake(), 101, "x")
+    with pytest.raises(F.FocssError, match="needed"):
+        F._need(None, 1, "x")

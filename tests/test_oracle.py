@@ -381,6 +381,46 @@ def test_sino_train_recovers_planted_model(modes):
     assert np.allclose(W_hat, W_hat.T)  # symmetric in (m, n)
 
 
+def _planted_weights(R, seed):
+    r = rng(seed)
+    W = np.zeros((2 * R + 1, 2 * R + 1), dtype=np.complex128)
+    for m in range(-(R // 2), R // 2 + 1):
+        for nn in range(abs(m), R - abs(m) + 1):
+            v = complex(r.standard_normal(), r.standard_normal()) * 0.1
+            for (a, b) in {(m, nn), (nn, m), (-m, -nn), (-nn, -m)}:
+                W[R + a, R + b] = v
+    return W
+
+
+@pytest.mark.parametrize("type_", [0, 1, 2], ids=["ORDINARY", "FIXED", "HOMOGENEOUS"])
+def test_sino_train_types_recover_planted_model_and_intercept(type_):
+    """The three SINO types (sino.cpp:53-65,150-153), in the reference's enum
+    order ORDINARY = 0, FIXED = 1, HOMOGENEOUS = 2, on data generated by
+    their own model plus a constant offset c0:
+      ORDINARY:    rx = w tx + cubic(tx; W) + c0  -> (W, c0, w) from the fit;
+      FIXED:       rx = w tx + cubic(tx; W) + c0, w given -> (W, c0) fitted
+                   to rx - w tx, w returned unchanged;
+      HOMOGENEOUS: rx = cubic(tx; W) + c0  -> (W, c0), w = 0.
+    A FIXED fit given the wrong w cannot reproduce W (the linear term has no
+    feature), and a HOMOGENEOUS fit of data with a linear term neither."""
+    R, n, modes = 4, 300, 2
+    tx = crandn(rng(20), modes * n) * 0.5
+    W = _planted_weights(R, 21)
+    w_lin, c0 = 0.9 + 0.1j, 0.02 - 0.03j
+    cubic = O.sino_cubic(W.ravel(), R, tx, modes=modes)
+    rx = (w_lin * tx if type_ != 2 else 0) + cubic + c0
+    W_hat, ic, w_hat = O.sino_train(R, type_, tx, rx, modes=modes, linear_weight=w_lin if type_ == 1 else 0j)
+    assert np.max(np.abs(W_hat - W)) < 1e-9
+    assert abs(ic - c0) < 1e-10
+    assert abs(w_hat - (w_lin if type_ != 2 else 0)) < 1e-10
+    if type_ == 1:
+        W_bad, _, _ = O.sino_train(R, 1, tx, rx, modes=modes, linear_weight=w_lin * 0.9)
+        assert np.max(np.abs(W_bad - W)) > 1e-3
+    if type_ == 2:
+        W_bad, _, _ = O.sino_train(R, 2, tx, rx + w_lin * tx, modes=modes)
+        assert np.max(np.abs(W_bad - W)) > 1e-3
+
+
 def test_wdmppe_delta_weights_closed_form():
     c = 2
     n = 30
