@@ -8,12 +8,18 @@ One bench step = the whole 20-span link for the batch (default 64 independent
 realisations).
 Metric: GSample-steps/s = sum over waveforms of samples x split steps / s.
 Prints one JSON line.  FP64 (complex128) is the headline; the FP32 path
-(complex64 arithmetic, FP64 phase generation) is reported beside it.
+(complex64 arithmetic, FP64 phase generation) is reported beside it, and so
+are the dual-polarisation 11-channel 2^20-sample link (`c5`), learned DBP
+against plain FP64 DBP, the triplet MLP, and the CPU baseline.
 """
 import argparse
+import concurrent.futures as cf
+import ctypes
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -98,6 +104,9 @@ def run_gpu(F, ctx, fields, dt, precision, steps, warmup, batch):
     prof_buf = ctx.alloc(nbytes)
     prof_buf.upload(host)
     prof = ctx.link_profile(links, prof_buf, offsets, 48, precision=precision)
+    # the same passes replayed from the captured graph, as link_run runs them
+    prof_buf.upload(host)
+    pair_ms = ctx.link_profile_graph(links, prof_buf, offsets, 4, precision=precision)
     prof_buf.free()
     out = bufs[-1].download(np.empty_like(host))
     for b in bufs:
@@ -110,14 +119,89 @@ def run_gpu(F, ctx, fields, dt, precision, steps, warmup, batch):
     return dict(value=value, ms_per_step=elapsed / steps * 1e3, split_steps=split_steps, tuned=tuned,
                 kernel=dominant, kernel_ms=t_dom, time_pass_ms=t_time, spec_pass_ms=t_spec,
                 achieved_gbs=achieved, bytes_per_launch=bytes_per_launch,
+                graph_pair_ms=pair_ms, graph_pair_gbs=2 * bytes_per_launch / (pair_ms * 1e-3) / 1e9,
                 finite=bool(np.all(np.isfinite(out))), output_power=float(np.mean(np.abs(out) ** 2)))
+
+
+def kernel_fractions(F, ctx, fields, dt, precision, batch):
+    """Per-kernel and graph-replayed pair HBM rates at another batch (128:
+    twice the 256 MiB Infinity Cache in FP64), with the tiles tuned above."""
+    esz = 16 if precision == F.F64 else 8
+    host = np.concatenate([fields[b % len(fields)] for b in range(batch)])
+    if precision == F.F32:
+        host = host.astype(np.complex64)
+    offsets = np.arange(batch, dtype=np.int64) * N_SAMPLES
+    links = make_links(F, dt, batch)
+    buf = ctx.alloc(host.nbytes)
+    buf.upload(host)
+    ms = ctx.link_profile(links, buf, offsets, 24, precision=precision)
+    buf.upload(host)
+    pair = ctx.link_profile_graph(links, buf, offsets, 4, precision=precision)
+    buf.free()
+    nb = 2.0 * esz * N_SAMPLES * batch
+    return {"batch": batch, "field_mib": host.nbytes / 2 ** 20,
+            "time_pass_us": round(ms[0] * 1e3, 2), "time_pass_gbs": round(nb / (ms[0] * 1e-3) / 1e9, 1),
+            "spec_pass_us": round(ms[1] * 1e3, 2), "spec_pass_gbs": round(nb / (ms[1] * 1e-3) / 1e9, 1),
+            "graph_pair_us": round(pair * 1e3, 2), "graph_pair_gbs": round(2 * nb / (pair * 1e-3) / 1e9, 1),
+            "graph_pair_frac": round(2 * nb / (pair * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def run_c5(F, ctx, batch, log):
+    """The dual-polarisation link: 11 x 32 GBd DP-16QAM on a 37.5 GHz grid,
+    2^20 samples per polarisation (32 sps), 30 x 80 km SSMF, Manakov
+    kappa 8/9, ADAPTIVE phi <= 1e-3 rad h <= 1 km, EDFA 16 dB + ASE NF 5 dB,
+    0 dBm per channel; `batch` independent realisations (symbols and ASE
+    seeds) per call, one full link each, tiles tuned per precision."""
+    from focss_amd.autotune import tune
+    from focss_amd.manakov import c5_field, c5_links
+    base = [c5_field(SEED + 31 * b) for b in range(min(batch, 2))]
+    dt = base[0][1]
+    n_per = base[0][0].size
+    out = {"workload": f"{batch} waveforms ({len(base)} symbol realisations, one ASE seed each) of 11x32 GBd "
+                       "DP-16QAM, 37.5 GHz grid, 2^20 samples per polarisation, "
+                       "30x80 km SSMF, Manakov kappa 8/9, ADAPTIVE phi<=1e-3 rad h<=1 km, EDFA 16 dB + ASE NF 5 dB, "
+                       "0 dBm per channel", "batch": batch}
+    for prec, name in ((F.F64, "f64"), (F.F32, "f32")):
+        esz = 16 if prec == F.F64 else 8
+        host = np.concatenate([base[b % len(base)][0] for b in range(batch)])
+        if prec == F.F32:
+            host = host.astype(np.complex64)
+        offsets = np.arange(batch, dtype=np.int64) * n_per
+        links = c5_links(F, dt, batch, SEED)
+        buf = ctx.alloc(host.nbytes)
+        tuned = tune(ctx, links, buf, offsets, host, precision=prec, log=log)
+        buf.upload(host)
+        ms = ctx.link_profile(links, buf, offsets, 24, precision=prec)
+        buf.upload(host)
+        pair = ctx.link_profile_graph(links, buf, offsets, 4, precision=prec)
+        buf.upload(host)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        steps = ctx.link_run(links, buf, offsets, precision=prec)
+        ctx.synchronize()
+        t = time.perf_counter() - t0
+        y = buf.download(np.empty_like(host))
+        buf.free()
+        nb = 2.0 * esz * n_per * batch
+        total = float(np.sum(steps))
+        out[name] = {"gsample_steps_per_s": round(n_per * total / t / 1e9, 3), "seconds": round(t, 3),
+                     "split_steps_per_waveform": int(total // batch),
+                     "time_pass_us": round(ms[0] * 1e3, 2), "time_pass_gbs": round(nb / (ms[0] * 1e-3) / 1e9, 1),
+                     "spec_pass_us": round(ms[1] * 1e3, 2), "spec_pass_gbs": round(nb / (ms[1] * 1e-3) / 1e9, 1),
+                     "graph_pair_gbs": round(2 * nb / (pair * 1e-3) / 1e9, 1),
+                     "link_frac": round(n_per * total / t * 4 * esz / 1e9 / HBM_PEAK_GBS, 4),
+                     "tiles": {str(k): v[0] for k, v in tuned.items()},
+                     "finite": bool(np.all(np.isfinite(y)))}
+        log(f"c5 {name}: {t:.2f} s, {out[name]['gsample_steps_per_s']} GSample-steps/s")
+    return out
 
 
 def f32_error_per_span(F, ctx, field, dt, grid):
     """Relative L2 error of the FP32 path against the FP64 path after each
     span, one realisation carried span by span through both precisions with
     identical ASE (seed per span).  The FP64 device path is the reference:
-    the tests hold it to the CPU oracle within 1e-10."""
+    tests/test_ssfm_twopass_gpu.py holds it to the CPU oracle within 1e-10
+    on the first span of this field at this size."""
     gain = 10 ** (ALPHA_DB_KM * SPAN_LEN * 1e-3 / 10)
     sigma = np.sqrt(PLANCK * F_C * 10 ** (NF_DB / 10) * (gain - 1) / 2 / dt / 2)
     b64, b32 = ctx.alloc(field.size * 16), ctx.alloc(field.size * 8)
@@ -149,10 +233,41 @@ def cpu_model():
     return "unknown"
 
 
+def native_oracle():
+    """The oracle's C source built for this host (gcc -O3 -march=native) in a
+    temporary folder; the in-tree -O3 build when gcc is unavailable."""
+    out = os.path.join(tempfile.gettempdir(), f"focss_oracle_native_{os.getpid()}.so")
+    try:
+        subprocess.run(["gcc", "-O3", "-march=native", "-std=gnu99", "-fPIC", "-shared", "-o", out,
+                        os.path.join(ROOT, "oracle", "focss_oracle.c"), "-lm"], check=True, capture_output=True,
+                       timeout=120)
+        return out, "gcc -O3 -march=native"
+    except (OSError, subprocess.SubprocessError):
+        return None, "in-tree build (gcc -O3)"
+
+
+def fftw_probe():
+    """Whether the host has FFTW3 (double) to load; the reference's CPU path
+    would use it."""
+    for name in ("libfftw3.so.3", "libfftw3.so"):
+        try:
+            ctypes.CDLL(name)
+            return name
+        except OSError:
+            continue
+    return None
+
+
 def cpu_baseline(field, dt, km, runs=3):
     """The CPU FP64 oracle (a restatement of the reference path, single
-    thread) on the first `km` km of span 1 of the same field; the median of
-    `runs` runs."""
+    thread, built for the host) on the first `km` km of span 1 of the same
+    field, the median of `runs` runs; and, on all host cores with one
+    waveform per thread, plain DBP at 16 steps/span of single-channel
+    16QAM waveforms of 2^14 samples over 30 spans (the batched-DBP
+    configuration's work unit)."""
+    lib, how = native_oracle()
+    if lib:
+        os.environ["FOCSS_ORACLE_LIB"] = lib
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     fd = fiber_dict()
@@ -165,10 +280,33 @@ def cpu_baseline(field, dt, km, runs=3):
         _, st = O.ssfm_run(fib, grd, 1.0, field, dt)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
+    # all cores, one waveform per thread at a time (the oracle's calls
+    # release the GIL)
+    threads = min(16, os.cpu_count() or 1)
+    n = 1 << 14
+    gain = 10 ** (ALPHA_DB_KM * SPAN_LEN * 1e-3 / 10)
+    wf = []
+    for k in range(4 * threads):
+        syms, _ = O.qam16_symbols(0xF0C55003 + k, n // 2)
+        wf.append(O.rrc_shaping(syms * np.sqrt(1e-3 * 10 ** (0.4 * (k % 8)) / 10 ** 0.2), 2, 0.1))
+    fib = O.fiber(**fiber_dict())
+    dt2 = 1.0 / (BAUD * 2)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda x: O.backward_propagation(fib, 30, 16, gain, x, dt2), wf))
+    t_all = time.perf_counter() - t0
+    all_rate = len(wf) * n * 30 * 16 / t_all / 1e9
+    if lib:
+        os.remove(lib)
     return dict(value=N_SAMPLES * st / t / 1e9, unit="GSample-steps/s", cores=1, kind="port",
-                sample=f"FP64 oracle (own Stockham FFT, 1 thread), first {km:g} km of span 1 "
+                sample=f"FP64 oracle ({how}, own FFT, 1 thread), first {km:g} km of span 1 of the bench field "
                        f"({st} adaptive steps, 2^18 samples), median of {runs} runs {t:.2f} s; "
-                       f"host {cpu_model()}, {os.cpu_count()} logical CPUs")
+                       f"host {cpu_model()}, {os.cpu_count()} logical CPUs; FFTW3 on the host: "
+                       f"{fftw_probe() or 'not found (the reference FFTW path cannot run here)'}",
+                all_cores={"value": round(all_rate, 5), "unit": "GSample-steps/s", "cores": threads,
+                           "sample": f"{threads} threads, {len(wf)} waveforms, one at a time per thread: plain DBP "
+                          f"16 steps/span x 30 spans "
+                                     f"of 2^14-sample 16QAM at 2 sps, {t_all:.2f} s"})
 
 
 def main():
@@ -181,6 +319,10 @@ def main():
     ap.add_argument("--cpu-km", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-batch", type=int, default=4)
+    ap.add_argument("--literal-iters", type=int, default=1500,
+                    help="Adam steps for the 2 steps/span 15-tap scalar-Kerr learned model (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: the link batch would shard "
@@ -197,11 +339,18 @@ def main():
     field = fields[0]
     res = {}
     span_err = None
+    big = {}
+    c5 = None
+    log = lambda m: print(m, file=sys.stderr, flush=True)
     with F.Context(0) as ctx:
         if args.precision in ("f64", "both"):
             res["f64"] = run_gpu(F, ctx, fields, dt, F.F64, args.steps, args.warmup, args.batch)
+            big["f64"] = kernel_fractions(F, ctx, fields, dt, F.F64, 2 * args.batch)
         if args.precision in ("f32", "both"):
             res["f32"] = run_gpu(F, ctx, fields, dt, F.F32, args.steps, args.warmup, args.batch)
+            big["f32"] = kernel_fractions(F, ctx, fields, dt, F.F32, 4 * args.batch)
+        if not args.no_c5:
+            c5 = run_c5(F, ctx, args.c5_batch, log)
         if args.precision == "both":
             span_err = {"adaptive": f32_error_per_span(F, ctx, field, dt, F.Grid(F.ADAPTIVE, 0, PHI_MAX, H_MAX)),
                         "uniform_100_steps_per_span": f32_error_per_span(F, ctx, field, dt,
@@ -235,7 +384,13 @@ def main():
                      "bytes_per_launch": head["bytes_per_launch"],
                      "kernel_ms": round(head["kernel_ms"], 5),
                      "time_pass_ms": round(head["time_pass_ms"], 5),
-                     "spec_pass_ms": round(head["spec_pass_ms"], 5)},
+                     "spec_pass_ms": round(head["spec_pass_ms"], 5),
+                     "note": "achieved = one pass's algorithmic bytes over the dominant kernel's HIP-event-timed "
+                             "duration, one kernel per launch; graph_pair = the same passes as link_run replays "
+                             "them from the captured graph, one event pair around 4 replays of 48 pass pairs",
+                     "graph_pair_gbs": round(head["graph_pair_gbs"], 1),
+                     "graph_pair_frac": round(head["graph_pair_gbs"] / HBM_PEAK_GBS, 4),
+                     "at_other_batch": big.get(head_key)},
     }
     if "f32" in res and head_key != "f32":
         r = res["f32"]
@@ -243,17 +398,22 @@ def main():
                        "kernel": r["kernel"], "achieved_gbs": round(r["achieved_gbs"], 1),
                        "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
                        "time_pass_ms": round(r["time_pass_ms"], 5),
-                       "spec_pass_ms": round(r["spec_pass_ms"], 5)}
+                       "spec_pass_ms": round(r["spec_pass_ms"], 5),
+                       "graph_pair_gbs": round(r["graph_pair_gbs"], 1),
+                       "graph_pair_frac": round(r["graph_pair_gbs"] / HBM_PEAK_GBS, 4),
+                       "at_other_batch": big.get("f32")}
         if span_err:
             # the ADAPTIVE schedule follows each precision's own peak power, so
             # the two runs take different steps; the UNIFORM run isolates the
             # arithmetic
             line["f32"]["rel_l2_vs_f64_per_span"] = {k: [float(f"{e:.3g}") for e in v]
                                                      for k, v in span_err.items()}
+    if c5:
+        line["c5"] = c5
     if not args.no_learned:
         # the metric's second half: Q of learned few-step DBP vs plain FP64 DBP
         from focss_amd.learned import parser as learned_parser, run as learned_run
-        la = learned_parser().parse_args([])
+        la = learned_parser().parse_args(["--literal-iters", str(args.literal_iters)])
         rep = learned_run(la, log=lambda m: print(m, file=sys.stderr, flush=True))
         line["learned_dbp"] = {
             "workload": f"{rep['waveforms']} test waveforms x 2^14 samples (16QAM 32 GBd, 2 sps), 20x80 km, "
@@ -267,6 +427,15 @@ def main():
             "waveforms_per_s": {k: round(v, 1) for k, v in rep["waveforms_per_s"].items()},
             "speedup_learned_vs_dbp16": round(rep["speedup_learned_vs_dbp16"], 2),
             "train_seconds": rep["train_seconds"]}
+        if "literal_model" in rep:
+            lm = rep["literal_model"]
+            line["learned_dbp"]["literal_model"] = {
+                "model": lm["model"], "iters": lm["iters"], "train_seconds": lm["train_seconds"],
+                "loss_first": lm["loss_first"], "loss_last_finite": lm["loss_last_finite"],
+                "diverged": lm["diverged"],
+                "q_db": round(lm["q_db"], 3), "q_gap_vs_dbp16_db": round(lm["q_gap_vs_dbp16_db"], 3),
+                "waveforms_per_s_fp16_mfma": round(lm["waveforms_per_s_fp16_mfma"], 1),
+                "speedup_vs_dbp16": round(lm["speedup_vs_dbp16"], 2)}
         # the learned configuration's optional network: the triplet MLP after
         # 1-step-per-span DBP, trained on 10^6 simulator symbols
         from focss_amd.triplet_mlp import parser as tmlp_parser, run as tmlp_run

@@ -88,6 +88,7 @@ struct focss_ctx {
     WaveStatic* d_statics = nullptr;
     WaveState* d_states = nullptr;
     Reduction* d_red = nullptr;
+    double* d_partials = nullptr;  // [3][cap][kMaxTimeTiles] per-tile power sums
     int* d_waves = nullptr;
     int* d_done = nullptr;
     int* h_done = nullptr;  // pinned, [2]
@@ -155,9 +156,13 @@ void host_tables(std::vector<cx<R>>& bank, std::vector<cx<R>>& fine) {
     }
 }
 
+constexpr int kMaxTimeTiles = 1024;  // time-pass tiles per waveform (N2 <= 1024)
+
 void free_control(focss_ctx* c) {
-    for (void* p : {(void*)c->d_statics, (void*)c->d_states, (void*)c->d_red, (void*)c->d_waves})
+    for (void* p : {(void*)c->d_statics, (void*)c->d_states, (void*)c->d_red, (void*)c->d_waves,
+                    (void*)c->d_partials})
         if (p) (void)hipFree(p);
+    c->d_partials = nullptr;
     c->d_statics = nullptr;
     c->d_states = nullptr;
     c->d_red = nullptr;
@@ -177,6 +182,7 @@ int ensure_capacity(focss_ctx* c, size_t n_waves) {
     if (e == hipSuccess) e = dev_malloc(&c->d_states, 2 * cap * sizeof(WaveState));
     if (e == hipSuccess) e = dev_malloc(&c->d_red, 3 * cap * sizeof(Reduction));
     if (e == hipSuccess) e = dev_malloc(&c->d_waves, cap * sizeof(int));
+    if (e == hipSuccess) e = dev_malloc(&c->d_partials, 3 * cap * kMaxTimeTiles * sizeof(double));
     if (e != hipSuccess) {
         free_control(c);
         return fail(FOCSS_E_HIP, std::string("link control arrays: ") + hipGetErrorString(e));
@@ -255,6 +261,7 @@ void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     a.waves += lc.first;
     a.log_n = lc.log_n;
     a.layout_g = lc.layout_g;
+    a.time_tiles = (1 << (lc.log_n - pass_log_n1(lc.log_n))) / r.time_cols[lc.log_n][lc.modes - 1][lc.variant];
     KernelFn<R> k = r.spec_k[lc.log_n][lc.spec_variant];
     hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0,
                        c->stream, a, tb);
@@ -389,6 +396,9 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     a.n_waves = n;
     a.counter = 0;
     a.layout_g = 1;
+    a.partials = c->d_partials;
+    a.partial_stride = kMaxTimeTiles;
+    a.time_tiles = 0;
     if (profile_pairs > 0) {
         // measurement mode: the first profile_pairs split steps launched one
         // kernel at a time, each bracketed by events on the context stream
@@ -451,6 +461,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         put(&c->d_states, sizeof(void*));
         put(&c->d_red, sizeof(void*));
         put(&c->d_waves, sizeof(void*));
+        put(&c->d_partials, sizeof(void*));
         for (const LengthClass& lc : two_pass) put(&lc, sizeof(lc));
         if (c->use_graph && (c->graph.key != key || !c->graph.exec)) {
             drop_graph(c);
@@ -1168,12 +1179,7 @@ int focss_ctx_destroy(focss_ctx* c) {
     if (c->tmlp_acc) (void)hipFree(c->tmlp_acc);
     if (c->lin_stats) (void)hipFree(c->lin_stats);
     if (c->dsp_tmp) (void)hipFree(c->dsp_tmp);
-    if (c->d_statics) {
-        (void)hipFree(c->d_statics);
-        (void)hipFree(c->d_states);
-        (void)hipFree(c->d_red);
-        (void)hipFree(c->d_waves);
-    }
+    free_control(c);
     if (c->d_done) (void)hipFree(c->d_done);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->h_done) (void)hipHostFree(c->h_done);

@@ -67,6 +67,9 @@ struct PassArgs {
     int counter;         // pass counter (time pass c or spectral pass r)
     int log_n;           // log2 of samples for this class
     int layout_g;        // 2: row-pair interleaved intermediate layout, 1: natural
+    double* partials;    // [3][n_waves][partial_stride] per-tile power sums (LOGARITHMIC grid)
+    int partial_stride;
+    int time_tiles;      // time-pass tiles per waveform of this class (spectral pass)
 };
 
 // Intermediate layout of the two-pass programs.  The field is an N1 x N2
@@ -337,6 +340,11 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
         Reduction* r = a.red + (a.counter % 3) * a.n_waves + w_id;
         atomicMax(&r->peak_bits, double_to_bits(pk));
         atomicAdd(&r->sum, sm);
+        // the logarithmic grid's step count depends on the sum; keep the
+        // tile sums so that it is formed in tile order, whatever order the
+        // workgroups finish in
+        if (w.grid == GRID_LOGARITHMIC)
+            a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + tile] = sm;
     }
     if (forward) {
 #pragma unroll
@@ -435,7 +443,13 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
     const bool il_out = a.layout_g == 2;
     const WaveStatic w = a.statics[w_id];
     const Reduction rd = a.red[(a.counter % 3) * a.n_waves + w_id];
-    SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), rd.sum);
+    double sum = rd.sum;
+    if (w.grid == GRID_LOGARITHMIC && st.phase == PH_SPAN_START) {
+        const double* p = a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride;
+        sum = 0;
+        for (int t = 0; t < a.time_tiles; ++t) sum += p[t];
+    }
+    SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
     if (tile == 0 && threadIdx.x == 0) {
         a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
         Reduction z;

@@ -31,6 +31,27 @@ BAUD, SPACING, OSF, ROLL_OFF = 32e9, 37.5e9, 32, 0.1
 KAPPA = 8.0 / 9.0
 
 
+def c5_field(seed, channels=11, n_sym=1 << 15, power_dbm=0.0):
+    """The dual-polarisation WDM field [2][2^20] of this link: independent
+    16QAM symbols per polarisation, half the launch power in each.  Returns
+    (field, dt, per-polarisation per-channel symbols)."""
+    from .transmitter import wdm_signal
+    pols = [wdm_signal(seed + 101 * p, channels, n_sym, OSF, BAUD, SPACING, power_dbm - 3.0103, ROLL_OFF)
+            for p in range(2)]
+    return np.concatenate([pols[0][0], pols[1][0]]), pols[0][1], [pols[p][2] for p in range(2)]
+
+
+def c5_links(F, dt, batch, seed, spans=30, n=1 << 20):
+    """`batch` forward links over this link's fibre and amplifiers, one ASE
+    seed per realisation."""
+    fib = F.Fiber(ALPHA, BETA2, GAMMA, SPAN_LEN, WAVELENGTH, KAPPA)
+    gain = np.exp(ALPHA * SPAN_LEN)
+    sigma = np.sqrt(PLANCK * F_C * 10 ** (NF_DB / 10) * (gain - 1) / 2 / dt / 2)
+    return [F.make_link(fib, F.Grid(F.ADAPTIVE, 0, 1e-3, 1e3), F.FORWARD, spans, n, dt, modes=2,
+                        center_frequency=F_C, post_scale=np.sqrt(gain), noise_sigma=sigma, noise_seed=seed + b)
+            for b in range(batch)]
+
+
 def run(args, log=print):
     import focss_amd as F
     from .transmitter import phase_align, q2_factor, rrc_sampling, wdm_signal

@@ -10,6 +10,9 @@ launch, HIP events), the time per pass pair when the same passes run as
 replays of the captured graph (one event pair around `--replays` replays),
 and the plain 16-byte copy kernel over the field's bytes (host clock over
 `reps` back-to-back copies), all printed as GB/s of algorithmic traffic.
+--workload c5: the same figures for the dual-polarisation link (11 x 32 GBd
+DP-16QAM on a 37.5 GHz grid, 2^20 samples per polarisation, ADAPTIVE, ASE)
+at each batch of `--batches` realisations.
 --workload side: the other hot kernels at their measured sizes, `reps`
 launches each --
   learned DBP matrix-core inference, one polarisation: 6 steps/span x 20
@@ -39,7 +42,7 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
     ap.add_argument("--variant", type=int, default=-1, help="tile variant (default: the library's choice)")
-    ap.add_argument("--workload", choices=["ssfm", "side", "roofline"], default="ssfm")
+    ap.add_argument("--workload", choices=["ssfm", "side", "roofline", "c5"], default="ssfm")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--batches", default="64,128")
     ap.add_argument("--replays", type=int, default=4)
@@ -48,7 +51,7 @@ def main():
     args = ap.parse_args()
     if args.workload == "side":
         return side(args)
-    if args.workload == "roofline":
+    if args.workload in ("roofline", "c5"):
         return roofline(args)
     sys.path.insert(0, ROOT)
     import bench
@@ -87,25 +90,35 @@ def roofline(args):
     from focss_amd.transmitter import wdm_signal
 
     prec = F.F64 if args.precision == "f64" else F.F32
-    esz = 16 if prec == F.F64 else 8
     batches = [int(b) for b in args.batches.split(",")]
     base = []
-    for b in range(4):
-        f, dt, _ = wdm_signal(bench.SEED + 7919 * b, bench.N_CH, bench.N_SAMPLES // bench.OSF, bench.OSF,
-                              bench.BAUD, bench.SPACING, bench.POWER_DBM)
-        base.append(f)
-    with F.Context(0) as ctx:
+    if args.workload == "c5":
+        from .manakov import c5_field, c5_links
+        for b in range(2):
+            f, dt, _ = c5_field(bench.SEED + 31 * b)
+            base.append(f)
+        n_per = base[0].size
+        make_links = lambda batch: c5_links(F, dt, batch, bench.SEED)
+        log_n = int(np.log2(n_per // 2))
+    else:
+        for b in range(4):
+            f, dt, _ = wdm_signal(bench.SEED + 7919 * b, bench.N_CH, bench.N_SAMPLES // bench.OSF, bench.OSF,
+                                  bench.BAUD, bench.SPACING, bench.POWER_DBM)
+            base.append(f)
+        n_per = bench.N_SAMPLES
+        make_links = lambda batch: bench.make_links(F, dt, batch)
         log_n = int(np.log2(bench.N_SAMPLES))
+    with F.Context(0) as ctx:
         if args.time_variant >= 0:
             ctx.set_tile(prec, log_n, args.time_variant, pass_=0)
         if args.spec_variant >= 0:
             ctx.set_tile(prec, log_n, args.spec_variant, pass_=1)
         for batch in batches:
-            host = np.concatenate([base[b % 4] for b in range(batch)])
+            host = np.concatenate([base[b % len(base)] for b in range(batch)])
             if prec == F.F32:
                 host = host.astype(np.complex64)
-            offsets = np.arange(batch, dtype=np.int64) * bench.N_SAMPLES
-            links = bench.make_links(F, dt, batch)
+            offsets = np.arange(batch, dtype=np.int64) * n_per
+            links = make_links(batch)
             buf = ctx.alloc(host.nbytes)
             buf.upload(host)
             ms = ctx.link_profile(links, buf, offsets, args.pairs, precision=prec)
@@ -120,7 +133,7 @@ def roofline(args):
                 ctx.device_copy(other, buf, host.nbytes)
             ctx.synchronize()
             t_copy = (time.perf_counter() - t0) / args.reps
-            print(f"probe roofline {args.precision} batch {batch} ({host.nbytes / 2**20:.0f} MiB) "
+            print(f"probe {args.workload} {args.precision} batch {batch} ({host.nbytes / 2**20:.0f} MiB) "
                   f"variants {args.time_variant}/{args.spec_variant}: "
                   f"time pass {ms[0] * 1e3:.2f} us ({nb / ms[0] / 1e6:.0f} GB/s), "
                   f"spectral pass {ms[1] * 1e3:.2f} us ({nb / ms[1] / 1e6:.0f} GB/s), "

@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libfocss_oracle.so")
+# FOCSS_ORACLE_LIB: another build of the same source (the CPU baseline
+# builds one for its host with -O3 -march=native)
+_LIB_PATH = os.environ.get("FOCSS_ORACLE_LIB") or os.path.join(_HERE, "libfocss_oracle.so")
 
 UNIFORM, LOGARITHMIC, ADAPTIVE = 0, 1, 2
 FORWARD, BACKWARD = 1.0, -1.0

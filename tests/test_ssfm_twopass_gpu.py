@@ -201,3 +201,24 @@ def test_wdm_span_f64_and_f32(ctx):
     assert e64 <= TOL64
     assert e32 <= 5e-4
     assert differ == 0 or np.max(np.abs(q32 - q_ref)) <= 0.05
+
+
+@pytest.mark.parametrize("logn", [16, 18])
+def test_logarithmic_grid_deterministic(ctx, logn):
+    """The logarithmic grid's step count is an integer part of a ratio of
+    the field's power sum (ssfm.cpp:120-134); the device forms that sum from
+    per-tile sums in tile order, so repeated runs agree bit for bit, and
+    with the oracle's step count and field to 1e-10."""
+    import focss_amd as F
+    n = 1 << logn
+    x, _, dt = qam_waveform(2000 + logn, n, sps=4, power_dbm=4.0)
+    pre = float(1.0 / np.sqrt(GAIN))
+    link = F.make_link(F.Fiber(**ssmf()), F.Grid(F.LOGARITHMIC, 0, 1e-2, 1e4), F.BACKWARD, 2, n, dt,
+                       center_frequency=F_C, pre_scale=pre)
+    runs = [ctx.link_run_host(link, x, precision=F.F64) for _ in range(4)]
+    for y, s in runs[1:]:
+        assert s == runs[0][1]
+        assert np.array_equal(y, runs[0][0])
+    ref = O.backward_propagation(O.fiber(**ssmf()), 2, 0, GAIN, x, dt, max_phase_shift=1e-2, max_step_size=1e4)
+    print(f"LOG grid 2^{logn}: {runs[0][1]} steps, rel L2 vs oracle {rel_l2(runs[0][0], ref):.3e}", flush=True)
+    assert rel_l2(runs[0][0], ref) <= TOL64
