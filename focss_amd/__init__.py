@@ -240,6 +240,10 @@ class Context:
         return _lib.focss_ctx_stream(self.handle)
 
     def fft(self, buf, modes, samples, sign, precision=F64):
+        """DFT of `modes` rows of `samples` values in place: forward (sign -1)
+        scaled by 1/samples, inverse unscaled.  Any length in [2, 2^19], and
+        powers of two up to 2^20."""
+        _need(buf, modes * samples * (16 if precision == F64 else 8), "fft")
         _check(_lib.focss_fft(self.handle, precision, sign, modes, samples, buf.ptr))
 
     def fft_noalloc(self, buf, modes, samples, sign, precision=F64):

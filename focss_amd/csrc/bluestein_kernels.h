@@ -63,14 +63,15 @@ __global__ __launch_bounds__(256) void k_gen_in(GenArgs<R> g) {
     gen_locate(g, k, m, e0);
     const int w_id = g.p.waves[k];
     const WaveState st = g.p.states[(g.p.counter & 1) * g.p.n_waves + w_id];
-    if (st.phase == PH_DONE) return;
     const GenWave<R> gw = g.gw[w_id];
     const WaveStatic w = g.p.statics[w_id];
     const cx<R>* x = reinterpret_cast<const cx<R>*>(g.p.field) + w.offset + (long long)m * gw.n;
     cx<R>* a = g.work + (gw.row0 + m) * (1LL << g.log_m);
     const int M = 1 << g.log_m;
+    // rows of finished programs are zeroed: the class's DFTs still run over them
+    const int live = st.phase != PH_DONE;
     for (int e = e0 + threadIdx.x; e < e0 + kGenChunk && e < M; e += blockDim.x)
-        a[e] = e < gw.n ? x[e] * gw.chirp[e] : mk<R>(0, 0);
+        a[e] = (live && e < gw.n) ? x[e] * gw.chirp[e] : mk<R>(0, 0);
 }
 
 // a *= Bhat

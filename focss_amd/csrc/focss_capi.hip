@@ -19,6 +19,7 @@
 #include "ldbp_kernels.h"
 #include "tmlp_kernels.h"
 #include "dsp_kernels.h"
+#include "bluestein_kernels.h"
 
 using namespace focss_hip;
 
@@ -79,6 +80,12 @@ struct GraphCache {
     std::vector<unsigned char> key;
 };
 
+struct BlueTable {
+    void* chirp = nullptr;
+    void* bhat = nullptr;
+    int log_m = 0;
+};
+
 struct focss_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -103,6 +110,15 @@ struct focss_ctx {
     void* scratch = nullptr;  // out-of-place buffer of the two-pass DFT
     size_t scratch_bytes = 0;
     void* noalloc_scratch = nullptr;  // device address of `scratch` on this context's device
+    std::map<std::pair<int, int>, BlueTable> blue;  // (length, precision) -> Bluestein tables
+    void* blu_work = nullptr;  // standalone Bluestein DFT rows
+    size_t blu_work_bytes = 0;
+    void* gen_work = nullptr;  // link programs of generic length: rows of the Bluestein convolutions
+    size_t gen_work_bytes = 0;
+    void* gen_tmp = nullptr;   // their power-of-two DFTs' intermediate
+    size_t gen_tmp_bytes = 0;
+    void* d_genwave = nullptr;  // [cap] GenWave
+    int* d_active = nullptr;    // [cap]
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
     long long max_split_steps = 20000000;
     bool use_graph = true;  // FOCSS_NO_GRAPH=1: plain launches (profilers)
@@ -157,12 +173,18 @@ void host_tables(std::vector<cx<R>>& bank, std::vector<cx<R>>& fine) {
 }
 
 constexpr int kMaxTimeTiles = 1024;  // time-pass tiles per waveform (N2 <= 1024)
+// lengths that are not powers of two (Bluestein DFTs of length 2^20 at most);
+// their time op's chunks (kGenTimeChunk samples) also fit kMaxTimeTiles
+constexpr int kMaxGenericSamples = 1 << 19;
+static_assert(kMaxGenericSamples / kGenTimeChunk <= kMaxTimeTiles, "generic time chunks per waveform");
 
 void free_control(focss_ctx* c) {
     for (void* p : {(void*)c->d_statics, (void*)c->d_states, (void*)c->d_red, (void*)c->d_waves,
-                    (void*)c->d_partials})
+                    (void*)c->d_partials, c->d_genwave, (void*)c->d_active})
         if (p) (void)hipFree(p);
     c->d_partials = nullptr;
+    c->d_genwave = nullptr;
+    c->d_active = nullptr;
     c->d_statics = nullptr;
     c->d_states = nullptr;
     c->d_red = nullptr;
@@ -183,6 +205,8 @@ int ensure_capacity(focss_ctx* c, size_t n_waves) {
     if (e == hipSuccess) e = dev_malloc(&c->d_red, 3 * cap * sizeof(Reduction));
     if (e == hipSuccess) e = dev_malloc(&c->d_waves, cap * sizeof(int));
     if (e == hipSuccess) e = dev_malloc(&c->d_partials, 3 * cap * kMaxTimeTiles * sizeof(double));
+    if (e == hipSuccess) e = dev_malloc(&c->d_genwave, cap * sizeof(GenWave<double>));
+    if (e == hipSuccess) e = dev_malloc(&c->d_active, cap * sizeof(int));
     if (e != hipSuccess) {
         free_control(c);
         return fail(FOCSS_E_HIP, std::string("link control arrays: ") + hipGetErrorString(e));
@@ -291,8 +315,9 @@ void launch_resident(hipStream_t s, const LengthClass& lc, PassArgs a, Tables<R>
 
 int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
     if (L.modes < 1 || L.modes > 2) return fail(FOCSS_E_ARG, "modes must be 1 or 2");
-    int log_n = ilog2_int(L.samples);
-    if (log_n < 6 || log_n > 20) return fail(FOCSS_E_ARG, "samples must be a power of two in [2^6, 2^20]");
+    const int log_n = ilog2_int(L.samples);
+    if (!((log_n >= 6 && log_n <= 20) || (L.samples >= 2 && L.samples <= kMaxGenericSamples)))
+        return fail(FOCSS_E_ARG, "samples must be a power of two in [2^6, 2^20] or any length in [2, 2^19]");
     if (!(L.dt > 0)) return fail(FOCSS_E_ARG, "dt must be > 0");
     if (L.n_spans < 1) return fail(FOCSS_E_ARG, "n_spans must be >= 1");
     if (!(L.fiber.length > 0)) return fail(FOCSS_E_ARG, "fibre length must be > 0");
@@ -338,6 +363,266 @@ int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
     return 0;
 }
 
+// ------------------------------------------------------------------
+// DFTs of rows (power-of-two lengths), Bluestein tables and the
+// standalone DFT of any length
+// ------------------------------------------------------------------
+template <typename R>
+Tables<R> tables_for(focss_ctx* c) {
+    Tables<R> tb;
+    const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
+    tb.bank = (const cx<R>*)c->bank[prec];
+    tb.fine = (const cx<R>*)c->fine[prec];
+    tb.bank64 = (const cx<double>*)c->bank[FOCSS_F64];
+    tb.fine64 = (const cx<double>*)c->fine[FOCSS_F64];
+    return tb;
+}
+
+// a context buffer of at least `need` bytes, grown after the stream drains
+int grow(focss_ctx* c, void** buf, size_t* have, size_t need) {
+    if (*have >= need) return 0;
+    HIP_TRY(hipStreamSynchronize(c->stream));  // earlier work may still use the old buffer
+    if (*buf) HIP_TRY(hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+    HIP_TRY(dev_malloc(buf, need));
+    *have = need;
+    return 0;
+}
+
+// unscaled-or-scaled DFT (exponent sign `sign`, result x `scale`) of `rows`
+// contiguous rows of 2^log_m values; lengths above 2^13 go through `tmp`
+// (rows x 2^log_m values)
+template <typename R>
+void fft_rows_launch(focss_ctx* c, const Tables<R>& tb, int sign, int rows, int log_m, cx<R>* data, cx<R>* tmp,
+                     double scale) {
+    const FftRegistry<R>& r = fft_reg<R>();
+    const int si = sign < 0 ? 0 : 1;
+    if (r.whole[log_m][si]) {
+        hipLaunchKernelGGL(r.whole[log_m][si], dim3(rows), dim3(NT), 0, c->stream, data, scale, tb);
+        return;
+    }
+    const int n = 1 << log_m;
+    const int n1 = 1 << (log_m / 2);
+    const int n2 = n / n1;
+    const int tc = n2 / r.ncols[log_m];
+    const int tr = n1 / r.nrows[log_m];
+    hipLaunchKernelGGL(r.cols[log_m][si], dim3(rows * tc), dim3(NT), 0, c->stream, (const cx<R>*)data, tmp, log_m, tc,
+                       tb);
+    hipLaunchKernelGGL(r.rows[log_m][si], dim3(rows * tr), dim3(NT), 0, c->stream, (const cx<R>*)tmp, data, log_m, tr,
+                       scale, tb);
+}
+
+int ceil_log2(long long v) {
+    int p = 0;
+    while ((1LL << p) < v) ++p;
+    return p;
+}
+
+int fft_enqueue_pow2_f64(focss_ctx* c, int modes, int log_n, void* data);
+
+// The chirp c_j = exp(-i pi j^2 / N) (j^2 reduced mod 2N, in long double)
+// and Bhat = DFT_M(b) / M of one length N at precision R, built once per
+// context and kept.
+template <typename R>
+int blue_table(focss_ctx* c, int n, const cx<R>** chirp, const cx<R>** bhat, int* log_m) {
+    const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
+    auto key = std::make_pair(n, prec);
+    auto it = c->blue.find(key);
+    if (it == c->blue.end()) {
+        if (n < 2 || n > kMaxGenericSamples)
+            return fail(FOCSS_E_ARG, "samples must be in [2, 2^19] when not a power of two");
+        const int lm = ceil_log2(2LL * n - 1);
+        const int M = 1 << lm;
+        const long double pi = 3.141592653589793238462643383279502884L;
+        std::vector<cx<double>> ch(n), b((size_t)M, mk<double>(0, 0));
+        for (long long j = 0; j < n; ++j) {
+            const long long q = (j * j) % (2LL * n);
+            const long double ang = pi * (long double)q / (long double)n;
+            ch[j] = mk<double>((double)cosl(ang), (double)-sinl(ang));
+        }
+        b[0] = conj(ch[0]);
+        for (int j = 1; j < n; ++j) b[j] = b[M - j] = conj(ch[j]);
+        BlueTable t;
+        t.log_m = lm;
+        void *d_ch = nullptr, *d_b = nullptr;  // FP64 tables
+        auto drop = [&](hipError_t e) {
+            (void)hipStreamSynchronize(c->stream);
+            for (void* p : {d_ch, d_b, t.chirp, t.bhat})
+                if (p) (void)hipFree(p);
+            return fail(FOCSS_E_HIP, std::string("Bluestein table: ") + hipGetErrorString(e));
+        };
+        hipError_t e = dev_malloc(&d_ch, (size_t)n * sizeof(cx<double>));
+        if (e == hipSuccess) e = dev_malloc(&d_b, (size_t)M * sizeof(cx<double>));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_ch, ch.data(), (size_t)n * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_b, b.data(), (size_t)M * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return drop(e);
+        if (int rc = fft_enqueue_pow2_f64(c, 1, lm, d_b)) {  // DFT(b) / M
+            drop(hipSuccess);
+            return rc;
+        }
+        if (prec == FOCSS_F32) {
+            e = dev_malloc(&t.chirp, (size_t)n * sizeof(cx<float>));
+            if (e == hipSuccess) e = dev_malloc(&t.bhat, (size_t)M * sizeof(cx<float>));
+            if (e != hipSuccess) return drop(e);
+            hipLaunchKernelGGL(k_from_c128<float>, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                               (const cx<double>*)d_ch, (cx<float>*)t.chirp, (long long)n);
+            hipLaunchKernelGGL(k_from_c128<float>, dim3((M + 255) / 256), dim3(256), 0, c->stream,
+                               (const cx<double>*)d_b, (cx<float>*)t.bhat, (long long)M);
+        }
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return drop(e);
+        if (prec == FOCSS_F64) {
+            t.chirp = d_ch;
+            t.bhat = d_b;
+        } else {
+            (void)hipFree(d_ch);
+            (void)hipFree(d_b);
+        }
+        it = c->blue.emplace(key, t).first;
+    }
+    *chirp = (const cx<R>*)it->second.chirp;
+    *bhat = (const cx<R>*)it->second.bhat;
+    *log_m = it->second.log_m;
+    return 0;
+}
+
+// stream-ordered DFT of `modes` rows of `samples` (forward x 1/N, inverse
+// unscaled).  Power-of-two lengths above 2^13 need a work buffer of the
+// field's size: `work` when given (it must hold modes x samples values),
+// else the context's buffer, grown on demand.  Other lengths (up to 2^19)
+// take the Bluestein path through the context's buffers.
+template <typename R>
+int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data, void* work = nullptr) {
+    if (modes < 1) return fail(FOCSS_E_ARG, "fft: modes must be >= 1");
+    const int log_n = ilog2_int(samples);
+    const Tables<R> tb = tables_for<R>(c);
+    cx<R>* d = (cx<R>*)data;
+    if (log_n >= 1 && log_n <= 20) {
+        const size_t bytes = (size_t)modes * samples * sizeof(cx<R>);
+        if (!work && !fft_reg<R>().whole[log_n][0])
+            if (int rc = grow(c, &c->scratch, &c->scratch_bytes, bytes)) return rc;
+        fft_rows_launch<R>(c, tb, sign, modes, log_n, d, (cx<R>*)(work ? work : c->scratch),
+                           sign < 0 ? 1.0 / samples : 1.0);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    if (samples < 2 || samples > kMaxGenericSamples)
+        return fail(FOCSS_E_ARG, "fft: samples must be a power of two up to 2^20, or any length in [2, 2^19]");
+    const cx<R>* chirp;
+    const cx<R>* bhat;
+    int lm;
+    if (int rc = blue_table<R>(c, samples, &chirp, &bhat, &lm)) return rc;
+    const size_t mbytes = (size_t)modes * ((size_t)1 << lm) * sizeof(cx<R>);
+    if (int rc = grow(c, &c->blu_work, &c->blu_work_bytes, mbytes)) return rc;
+    if (int rc = grow(c, &c->scratch, &c->scratch_bytes, mbytes)) return rc;
+    cx<R>* wk = (cx<R>*)c->blu_work;
+    const int inverse = sign > 0;
+    const int chunks = ((1 << lm) + kGenChunk - 1) / kGenChunk;
+    hipLaunchKernelGGL(k_blu_in<R>, dim3(modes * chunks), dim3(256), 0, c->stream, (const cx<R>*)d, wk, chirp, samples,
+                       lm, inverse);
+    fft_rows_launch<R>(c, tb, -1, modes, lm, wk, (cx<R>*)c->scratch, 1.0);
+    hipLaunchKernelGGL(k_blu_mul<R>, dim3(modes * chunks), dim3(256), 0, c->stream, wk, bhat, lm);
+    fft_rows_launch<R>(c, tb, +1, modes, lm, wk, (cx<R>*)c->scratch, 1.0);
+    hipLaunchKernelGGL(k_blu_out<R>, dim3(modes * chunks), dim3(256), 0, c->stream, (const cx<R>*)wk, d, chirp,
+                       samples, lm, inverse, inverse ? 1.0 : 1.0 / samples);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int fft_enqueue_pow2_f64(focss_ctx* c, int modes, int log_n, void* data) {
+    return fft_enqueue<double>(c, -1, modes, 1 << log_n, data);
+}
+
+template <typename R>
+int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data, void* work = nullptr) {
+    if (int rc = fft_enqueue<R>(c, sign, modes, samples, data, work)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Waveforms whose length is not a power of two (or below 2^6), grouped by
+// the power-of-two length 2^log_m of their Bluestein convolutions
+struct GenClass {
+    int log_m, modes;
+    int first, count;  // into the waves array
+    int max_n;         // longest waveform of the class
+    long long work_off;  // its rows' first value in gen_work / gen_tmp
+};
+
+template <typename R>
+GenArgs<R> gen_args(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
+    GenArgs<R> g;
+    g.p = a;
+    g.p.waves += gc.first;
+    g.gw = (const GenWave<R>*)c->d_genwave;
+    g.work = (cx<R>*)c->gen_work + gc.work_off;
+    g.active = c->d_active;
+    g.log_m = gc.log_m;
+    g.modes = gc.modes;
+    g.chunks_per_row = ((1 << gc.log_m) + kGenChunk - 1) / kGenChunk;
+    return g;
+}
+
+// the spectral op of counter a.counter: two Bluestein DFTs around the
+// dispersion multiply
+template <typename R>
+void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb) {
+    const GenArgs<R> g = gen_args<R>(c, gc, a);
+    const int rows = gc.count * gc.modes;
+    const dim3 grid(rows * g.chunks_per_row), block(256);
+    cx<R>* tmp = (cx<R>*)c->gen_tmp + gc.work_off;
+    hipLaunchKernelGGL(k_gen_in<R>, grid, block, 0, c->stream, g);
+    fft_rows_launch<R>(c, tb, -1, rows, gc.log_m, g.work, tmp, 1.0);
+    hipLaunchKernelGGL(k_gen_mul<R>, grid, block, 0, c->stream, g);
+    fft_rows_launch<R>(c, tb, +1, rows, gc.log_m, g.work, tmp, 1.0);
+    hipLaunchKernelGGL(k_gen_mid<R>, grid, block, 0, c->stream, g);
+    fft_rows_launch<R>(c, tb, -1, rows, gc.log_m, g.work, tmp, 1.0);
+    hipLaunchKernelGGL(k_gen_mul<R>, grid, block, 0, c->stream, g);
+    fft_rows_launch<R>(c, tb, +1, rows, gc.log_m, g.work, tmp, 1.0);
+    hipLaunchKernelGGL(k_gen_out<R>, grid, block, 0, c->stream, g);
+}
+
+template <typename R>
+void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
+    GenArgs<R> g = gen_args<R>(c, gc, a);
+    g.p.tiles_per_wave = (gc.max_n + kGenTimeChunk - 1) / kGenTimeChunk;
+    void (*k)(GenArgs<R>) = gc.modes == 1 ? &k_gen_time<R, 1> : &k_gen_time<R, 2>;
+    hipLaunchKernelGGL(k, dim3(gc.count * g.p.tiles_per_wave), dim3(256), 0, c->stream, g);
+}
+
+// Bluestein tables, GenWave records and work rows of the generic classes
+template <typename R>
+int prepare_generic(focss_ctx* c, int n, const focss_link* links, const std::vector<int>& order,
+                    std::vector<GenClass>& gen) {
+    if (gen.empty()) return 0;
+    std::vector<GenWave<R>> gw(n);
+    long long values = 0;
+    for (GenClass& gc : gen) {
+        gc.work_off = values;
+        gc.max_n = 0;
+        for (int k = 0; k < gc.count; ++k) {
+            const int i = order[gc.first + k];
+            GenWave<R>& w = gw[i];
+            int lm = 0;
+            if (int rc = blue_table<R>(c, links[i].samples, &w.chirp, &w.bhat, &lm)) return rc;
+            w.row0 = (long long)k * gc.modes;
+            w.n = links[i].samples;
+            w.pad = 0;
+            gc.max_n = std::max(gc.max_n, w.n);
+        }
+        values += (long long)gc.count * gc.modes << gc.log_m;
+    }
+    const size_t bytes = (size_t)values * sizeof(cx<R>);
+    if (int rc = grow(c, &c->gen_work, &c->gen_work_bytes, bytes)) return rc;
+    if (int rc = grow(c, &c->gen_tmp, &c->gen_tmp_bytes, bytes)) return rc;
+    HIP_TRY(hipMemcpyAsync(c->d_genwave, gw.data(), (size_t)n * sizeof(GenWave<R>), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // gw is released on return
+    return 0;
+}
+
 template <typename R>
 int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void* batch,
               const int64_t* offsets, int* total_steps, int profile_pairs = 0, double* prof_ms = nullptr,
@@ -348,12 +633,30 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         if (rc) return rc;
     }
     if (int rc = ensure_capacity(c, n)) return rc;
-    // group by length class
+    // group by length class; lengths without a kernel of their own (not a
+    // power of two in [2^6, 2^20]) by their Bluestein length, keyed 100 + log_m
     std::map<std::pair<int, int>, std::vector<int>> groups;
-    for (int i = 0; i < n; ++i) groups[{ilog2_int(links[i].samples), links[i].modes}].push_back(i);
+    for (int i = 0; i < n; ++i) {
+        const int l = ilog2_int(links[i].samples);
+        const int key = (l >= 6 && l <= 20) ? l : 100 + ceil_log2(2LL * links[i].samples - 1);
+        groups[{key, links[i].modes}].push_back(i);
+    }
     std::vector<int> order;
     std::vector<LengthClass> resident, two_pass;
+    std::vector<GenClass> gen;
     for (auto& kv : groups) {
+        if (kv.first.first >= 100) {
+            GenClass gc;
+            gc.log_m = kv.first.first - 100;
+            gc.modes = kv.first.second;
+            gc.first = (int)order.size();
+            gc.count = (int)kv.second.size();
+            gc.max_n = 0;
+            gc.work_off = 0;
+            for (int i : kv.second) order.push_back(i);
+            gen.push_back(gc);
+            continue;
+        }
         LengthClass lc;
         lc.log_n = kv.first.first;
         lc.modes = kv.first.second;
@@ -377,6 +680,9 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                                                  " modes=" + std::to_string(lc.modes));
         }
     }
+    if ((profile_pairs > 0 || graph_replays > 0) && !gen.empty())
+        return fail(FOCSS_E_UNSUPPORTED, "pass timing takes power-of-two lengths in [2^6, 2^20] only");
+    if (int rc = prepare_generic<R>(c, n, links, order, gen)) return rc;
     std::vector<WaveState> init(2 * (size_t)n);
     for (int i = 0; i < n; ++i) init[i] = initial_state(st[i]);
     std::vector<Reduction> red0(3 * (size_t)n);
@@ -446,7 +752,8 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         prof_ms[2] = t_res;
         return 0;
     }
-    if (!two_pass.empty()) {
+    const bool stepped = !two_pass.empty() || !gen.empty();  // programs driven from the host loop
+    if (stepped) {
         // graph of [spectral pass r, time pass r+1] x K, captured before any
         // work is queued; its key is everything baked into the kernel args
         std::vector<unsigned char> key;
@@ -463,6 +770,11 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         put(&c->d_waves, sizeof(void*));
         put(&c->d_partials, sizeof(void*));
         for (const LengthClass& lc : two_pass) put(&lc, sizeof(lc));
+        for (const GenClass& gc : gen) put(&gc, sizeof(gc));
+        put(&c->gen_work, sizeof(void*));
+        put(&c->gen_tmp, sizeof(void*));
+        put(&c->d_genwave, sizeof(void*));
+        put(&c->d_active, sizeof(void*));
         if (c->use_graph && (c->graph.key != key || !c->graph.exec)) {
             drop_graph(c);
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -470,8 +782,10 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                 PassArgs ak = a;
                 ak.counter = k;
                 for (const LengthClass& lc : two_pass) launch_spec<R>(c, lc, ak, tb);
+                for (const GenClass& gc : gen) launch_gen_spec<R>(c, gc, ak, tb);
                 ak.counter = k + 1;
                 for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
+                for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak);
             }
             HIP_TRY(hipStreamEndCapture(c->stream, &c->graph.graph));
             HIP_TRY(hipGraphInstantiate(&c->graph.exec, c->graph.graph, nullptr, nullptr, 0));
@@ -497,9 +811,10 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev_join, c->side));
     }
-    if (!two_pass.empty()) {
+    if (stepped) {
         // entry time pass (counter 0), then replays of the graph
         for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
+        for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, a);
         HIP_TRY(hipGetLastError());
         if (graph_replays > 0) {
             // measurement mode: `graph_replays` replays of the pass-pair
@@ -531,8 +846,10 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                     PassArgs ak = a;
                     ak.counter = k;
                     for (const LengthClass& lc : two_pass) launch_spec<R>(c, lc, ak, tb);
+                    for (const GenClass& gc : gen) launch_gen_spec<R>(c, gc, ak, tb);
                     ak.counter = k + 1;
                     for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
+                    for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak);
                 }
                 HIP_TRY(hipGetLastError());
             }
@@ -566,61 +883,6 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                       std::to_string(last.total_steps) + " split steps);";
     }
     if (!faults.empty()) return fail(FOCSS_E_NUMERIC, "non-finite field power, program stopped:" + faults);
-    return 0;
-}
-
-}  // namespace
-
-namespace {
-// stream-ordered DFT of `modes` rows of `samples` (forward x 1/N).  The
-// two-pass lengths need a work buffer of the field's size: `work` when given
-// (it must hold modes x samples values), else the context's buffer, grown on
-// demand.
-template <typename R>
-int fft_enqueue(focss_ctx* c, int sign, int modes, int samples, void* data, void* work = nullptr) {
-    int log_n = ilog2_int(samples);
-    if (log_n < 1 || log_n > 20) return fail(FOCSS_E_ARG, "fft: samples must be a power of two in [2, 2^20]");
-    if (modes < 1) return fail(FOCSS_E_ARG, "fft: modes must be >= 1");
-    const int si = sign < 0 ? 0 : 1;
-    const double scale = sign < 0 ? 1.0 / samples : 1.0;
-    Tables<R> tb;
-    const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
-    tb.bank = (const cx<R>*)c->bank[prec];
-    tb.fine = (const cx<R>*)c->fine[prec];
-    tb.bank64 = (const cx<double>*)c->bank[FOCSS_F64];
-    tb.fine64 = (const cx<double>*)c->fine[FOCSS_F64];
-    const FftRegistry<R>& r = fft_reg<R>();
-    cx<R>* d = (cx<R>*)data;
-    if (r.whole[log_n][si]) {
-        hipLaunchKernelGGL(r.whole[log_n][si], dim3(modes), dim3(NT), 0, c->stream, d, scale, tb);
-    } else {
-        size_t bytes = (size_t)modes * samples * sizeof(cx<R>);
-        if (!work && c->scratch_bytes < bytes) {
-            HIP_TRY(hipStreamSynchronize(c->stream));  // earlier work may still use the old buffer
-            if (c->scratch) HIP_TRY(hipFree(c->scratch));
-            c->scratch = nullptr;
-            c->scratch_bytes = 0;
-            HIP_TRY(dev_malloc(&c->scratch, bytes));
-            c->scratch_bytes = bytes;
-        }
-        cx<R>* w = (cx<R>*)(work ? work : c->scratch);
-        int n1 = 1 << (log_n / 2);
-        int n2 = samples / n1;
-        int tc = n2 / r.ncols[log_n];
-        int tr = n1 / r.nrows[log_n];
-        hipLaunchKernelGGL(r.cols[log_n][si], dim3(modes * tc), dim3(NT), 0, c->stream, (const cx<R>*)d, w, log_n,
-                           tc, tb);
-        hipLaunchKernelGGL(r.rows[log_n][si], dim3(modes * tr), dim3(NT), 0, c->stream, (const cx<R>*)w, d, log_n,
-                           tr, scale, tb);
-    }
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-
-template <typename R>
-int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data, void* work = nullptr) {
-    if (int rc = fft_enqueue<R>(c, sign, modes, samples, data, work)) return rc;
-    HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1182,6 +1444,13 @@ int focss_ctx_destroy(focss_ctx* c) {
     free_control(c);
     if (c->d_done) (void)hipFree(c->d_done);
     if (c->scratch) (void)hipFree(c->scratch);
+    for (auto& kv : c->blue) {
+        if (kv.second.chirp) (void)hipFree(kv.second.chirp);
+        if (kv.second.bhat) (void)hipFree(kv.second.bhat);
+    }
+    c->blue.clear();
+    for (void* p : {c->blu_work, c->gen_work, c->gen_tmp})
+        if (p) (void)hipFree(p);
     if (c->h_done) (void)hipHostFree(c->h_done);
     for (int i = 0; i < 2; ++i)
         if (c->ev_done[i]) (void)hipEventDestroy(c->ev_done[i]);
@@ -1332,6 +1601,9 @@ int focss_fft_noalloc(focss_ctx* c, int precision, int sign, int modes, int samp
     if (sign != -1 && sign != 1) return fail(FOCSS_E_ARG, "fft: sign must be -1 or +1");
     if (precision != FOCSS_F64 && precision != FOCSS_F32) return fail(FOCSS_E_ARG, "bad precision");
     if (modes < 1 || samples < 1) return fail(FOCSS_E_ARG, "fft: modes and samples must be >= 1");
+    if (ilog2_int(samples) < 1)
+        return fail(FOCSS_E_ARG, "focss_fft_noalloc: samples must be a power of two in [2, 2^20] (other lengths "
+                                 "build Bluestein tables on the device: use focss_fft)");
     const size_t bytes = (size_t)modes * (size_t)samples * (precision == FOCSS_F64 ? 16 : 8);
     if (bytes > kNoAllocScratchBytes)
         return fail(FOCSS_E_ARG, "focss_fft_noalloc: the field is " + std::to_string(bytes) +

@@ -49,11 +49,20 @@ def test_validate_accepts_good_link():
     F.validate(_link(F))
 
 
+@pytest.mark.parametrize("n", [2, 16, 3000, 3 << 12, 997, (1 << 19) - 1])
+def test_validate_accepts_any_length(n):
+    """Lengths without a power-of-two kernel run through Bluestein DFTs; the
+    reference's FFTW plans take any length (field.cpp:530-541)."""
+    import focss_amd as F
+    F.validate(_link(F, n=n))
+
+
 @pytest.mark.parametrize("case,msg", [
     ("hmax0", "max_step_size"),
     ("log_alpha0", "alpha"),
-    ("npot", "power of two"),
-    ("small", "power of two"),
+    ("npot_long", "any length in \\[2, 2\\^19\\]"),
+    ("one", "power of two"),
+    ("pow2_long", "power of two"),
     ("modes3", "modes"),
     ("dir", "direction"),
     ("steps0", "total_steps"),
@@ -66,10 +75,12 @@ def test_validate_rejects(case, msg):
     elif case == "log_alpha0":
         link = _link(F, grid=F.Grid(F.LOGARITHMIC, 0, 1e-3, 1e3),
                      fiber=F.Fiber(0.0, -2.17e-26, 1.3e-3, 80e3, 1.55e-6, 1.0))
-    elif case == "npot":
-        link = _link(F, n=3000)
-    elif case == "small":
-        link = _link(F, n=16)
+    elif case == "npot_long":
+        link = _link(F, n=(1 << 19) + 3)
+    elif case == "one":
+        link = _link(F, n=1)
+    elif case == "pow2_long":
+        link = _link(F, n=1 << 21)
     elif case == "modes3":
         link = _link(F, modes=3)
     elif case == "dir":
