@@ -8,7 +8,14 @@ from . import F32, F64, tile_variants
 
 
 def tune(ctx, links, batch, offsets, host_fields, precision=F64, n_pairs=24, log=None):
-    """host_fields: the batch contents to restore between trials (numpy)."""
+    """host_fields: the batch contents to restore between trials (numpy).
+    Only the power-of-two waveforms are timed (the other lengths run the
+    Bluestein kernels, which have one shape)."""
+    keep = [i for i, l in enumerate(links) if l.samples & (l.samples - 1) == 0 and l.samples >= 64]
+    if not keep:
+        return {}
+    links = [links[i] for i in keep]
+    offsets = np.asarray(offsets, dtype=np.int64)[keep]
     log_ns = sorted({int(np.log2(l.samples)) for l in links})
     best = {}
     for log_n in log_ns:

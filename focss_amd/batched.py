@@ -4,12 +4,14 @@ programs, tile shapes from the autotuner).
 
     python -m focss_amd.batched [--waves 256] [--seed 5]
 
-Workload: 16QAM at 32 GBd, RRC 0.1; per waveform the length is drawn
-uniformly from the five powers of two 2^12 .. 2^16 samples at 2 samples/symbol
-(the FFT is radix-2 based, so lengths are powers of two), the span count
-uniformly from 10 .. 30 (80 km SSMF, EDFA 16 dB + ASE NF 5 dB) and the launch
-power uniformly from -2 .. 6 dBm.  The received field comes from the device
-link program (ADAPTIVE grid, FP64) at `sim_sps` samples/symbol, decimated to 2.
+Workload: 16QAM at 32 GBd, RRC 0.1; per waveform the symbol count n_sym is
+drawn uniformly from the integers 2^11 .. 2^15 (N = 2 n_sym samples at 2
+samples/symbol, almost never a power of two: those programs run through the
+Bluestein DFTs), or with `--lengths pow2` N from the five powers of two
+2^12 .. 2^16; the span count uniformly from 10 .. 30 (80 km SSMF, EDFA 16 dB +
+ASE NF 5 dB) and the launch power uniformly from -2 .. 6 dBm.  The received
+field comes from the device link program (ADAPTIVE grid, FP64) at `sim_sps`
+samples/symbol, decimated to 2.
 DBP (backward_propagation, src/tasks/systems.cpp:73-91) runs at 1, 4 and 16
 steps per span; hard decisions (demodulate_16qam after RRC matched filter and
 phase alignment) of every setting are compared with FP64 DBP at 16 steps.
@@ -24,11 +26,15 @@ import numpy as np
 from .learned import ALPHA, BAUD, F_C, NF_DB, PLANCK, ROLL_OFF, SPAN_LEN, fiber
 
 LOG2_LENGTHS = (12, 13, 14, 15, 16)
+SYMBOLS = (1 << 11, 1 << 15)  # n_sym range of the uniform-length workload
 
 
 def make_workload(F, ctx, n_waves, seed, sim_sps=8, log2_lengths=LOG2_LENGTHS, spans=(10, 30),
-                  power_dbm=(-2.0, 6.0)):
-    """Returns a list of dicts {rx (2 sps), syms, spans, power_dbm, samples}."""
+                  power_dbm=(-2.0, 6.0), symbols=None, chunk=64, log=None):
+    """Returns a list of dicts {rx (2 sps), syms, spans, power_dbm, samples}.
+    With `symbols` = (lo, hi) the symbol count is uniform in [lo, hi] and
+    `log2_lengths` is not used.  The forward links run `chunk` waveforms per
+    device call (`log` gets a line after each)."""
     from .transmitter import decimate, qam16, rrc_shaping
     rng = np.random.Generator(np.random.PCG64(seed))
     gain = np.exp(ALPHA * SPAN_LEN)
@@ -37,7 +43,10 @@ def make_workload(F, ctx, n_waves, seed, sim_sps=8, log2_lengths=LOG2_LENGTHS, s
     sigma = np.sqrt(PLANCK * F_C * nf * (gain - 1) / 2 / dt_sim / 2)
     waves, links, fields = [], [], []
     for i in range(n_waves):
-        n = 1 << int(rng.choice(log2_lengths))
+        if symbols is not None:
+            n = 2 * int(rng.integers(symbols[0], symbols[1] + 1))
+        else:
+            n = 1 << int(rng.choice(log2_lengths))
         n_spans = int(rng.integers(spans[0], spans[1] + 1))
         p_dbm = float(rng.uniform(power_dbm[0], power_dbm[1]))
         s, _ = qam16(rng, n // 2)
@@ -47,15 +56,20 @@ def make_workload(F, ctx, n_waves, seed, sim_sps=8, log2_lengths=LOG2_LENGTHS, s
                                  dt_sim, center_frequency=F_C, post_scale=np.sqrt(gain), noise_sigma=sigma,
                                  noise_seed=seed * 100000 + i))
         waves.append({"syms": s, "spans": n_spans, "power_dbm": p_dbm, "samples": n})
-    offs = np.cumsum([0] + [f.size for f in fields[:-1]]).astype(np.int64)
-    host = np.concatenate(fields)
-    buf = ctx.alloc(host.nbytes)
-    buf.upload(host)
-    ctx.link_run(links, buf, offs, precision=F.F64)
-    rx = buf.download(np.empty_like(host))
-    buf.free()
-    for w, o, f in zip(waves, offs, fields):
-        w["rx"] = decimate(rx[o:o + f.size], sim_sps // 2)
+    for c0 in range(0, n_waves, chunk):
+        part = slice(c0, min(n_waves, c0 + chunk))
+        offs = np.cumsum([0] + [f.size for f in fields[part][:-1]]).astype(np.int64)
+        host = np.concatenate(fields[part])
+        buf = ctx.alloc(host.nbytes)
+        buf.upload(host)
+        t0 = time.perf_counter()
+        ctx.link_run(links[part], buf, offs, precision=F.F64)
+        rx = buf.download(np.empty_like(host))
+        buf.free()
+        for w, o, f in zip(waves[part], offs, fields[part]):
+            w["rx"] = decimate(rx[o:o + f.size], sim_sps // 2)
+        if log:
+            log(f"forward links {part.start}..{part.stop - 1}: {time.perf_counter() - t0:.1f} s")
     return waves
 
 
@@ -115,13 +129,19 @@ def run(args, log=print):
     import focss_amd as F
     with F.Context(0) as ctx:
         t0 = time.perf_counter()
-        waves = make_workload(F, ctx, args.waves, args.seed, args.sim_sps)
+        uniform = args.lengths == "uniform"
+        waves = make_workload(F, ctx, args.waves, args.seed, args.sim_sps, symbols=SYMBOLS if uniform else None,
+                              log=log)
         log(f"batched DBP workload: {len(waves)} waveforms, {time.perf_counter() - t0:.1f} s")
         ref, _, _ = run_dbp(F, ctx, waves, 16, F.F64, reps=1, tune=False)
         ref_codes = decisions(waves, ref)
+        lengths = np.array([w["samples"] for w in waves])
+        pow2 = int(np.count_nonzero(lengths & (lengths - 1) == 0))
         rep = {"waveforms": len(waves),
-               "samples_total": int(sum(w["samples"] for w in waves)),
-               "lengths_log2": {str(k): int(sum(1 for w in waves if w["samples"] == 1 << k)) for k in LOG2_LENGTHS},
+               "samples_total": int(lengths.sum()),
+               "lengths": ({"n_sym_range": list(SYMBOLS), "min": int(lengths.min()), "max": int(lengths.max()),
+                            "powers_of_two": pow2} if uniform else
+                           {"log2": {str(k): int(np.count_nonzero(lengths == 1 << k)) for k in LOG2_LENGTHS}}),
                "spans_range": [int(min(w["spans"] for w in waves)), int(max(w["spans"] for w in waves))],
                "q_db_dbp16_f64": round(q_mean(waves, ref), 3), "settings": {}}
         for prec, name in ((F.F32, "f32"), (F.F64, "f64")):
@@ -147,6 +167,8 @@ def parser():
     ap.add_argument("--waves", type=int, default=256)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--sim-sps", type=int, default=8)
+    ap.add_argument("--lengths", choices=["uniform", "pow2"], default="uniform",
+                    help="uniform: N = 2 n_sym, n_sym uniform in [2^11, 2^15]; pow2: N in 2^12 .. 2^16")
     return ap
 
 

@@ -25,10 +25,11 @@ namespace focss_hip {
 // a waveform of a generic-length class
 template <typename R>
 struct GenWave {
-    const cx<R>* chirp;  // c_j, j < N
-    const cx<R>* bhat;   // Bhat_k, k < M
-    long long row0;      // its first row of the work buffer (modes rows of M)
-    int n;               // samples per mode
+    const cx<R>* chirp;      // c_j, j < N
+    const cx<R>* bhat;       // Bhat_k, k < M
+    const cx<R>* bhat_rows;  // Bhat in the row order of the fused kernels (M >= 2^12)
+    long long row0;          // its first row of the work buffer (modes rows of M)
+    int n;                   // samples per mode
     int pad;
 };
 

@@ -83,6 +83,7 @@ struct GraphCache {
 struct BlueTable {
     void* chirp = nullptr;
     void* bhat = nullptr;
+    void* bhat_rows = nullptr;  // row order of the fused link kernels (log_m >= kBluMinLog)
     int log_m = 0;
 };
 
@@ -115,8 +116,6 @@ struct focss_ctx {
     size_t blu_work_bytes = 0;
     void* gen_work = nullptr;  // link programs of generic length: rows of the Bluestein convolutions
     size_t gen_work_bytes = 0;
-    void* gen_tmp = nullptr;   // their power-of-two DFTs' intermediate
-    size_t gen_tmp_bytes = 0;
     void* d_genwave = nullptr;  // [cap] GenWave
     int* d_active = nullptr;    // [cap]
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
@@ -422,10 +421,12 @@ int ceil_log2(long long v) {
 int fft_enqueue_pow2_f64(focss_ctx* c, int modes, int log_n, void* data);
 
 // The chirp c_j = exp(-i pi j^2 / N) (j^2 reduced mod 2N, in long double)
-// and Bhat = DFT_M(b) / M of one length N at precision R, built once per
-// context and kept.
+// and Bhat = DFT_M(b) / M of one length N at precision R (also in the row
+// order of the fused link kernels when M >= 2^kBluMinLog), built in FP64
+// once per context and kept.
 template <typename R>
-int blue_table(focss_ctx* c, int n, const cx<R>** chirp, const cx<R>** bhat, int* log_m) {
+int blue_table(focss_ctx* c, int n, const cx<R>** chirp, const cx<R>** bhat, int* log_m,
+               const cx<R>** bhat_rows = nullptr) {
     const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
     auto key = std::make_pair(n, prec);
     auto it = c->blue.find(key);
@@ -445,15 +446,17 @@ int blue_table(focss_ctx* c, int n, const cx<R>** chirp, const cx<R>** bhat, int
         for (int j = 1; j < n; ++j) b[j] = b[M - j] = conj(ch[j]);
         BlueTable t;
         t.log_m = lm;
-        void *d_ch = nullptr, *d_b = nullptr;  // FP64 tables
+        const bool rows = lm >= kBluMinLog;
+        void *d_ch = nullptr, *d_b = nullptr, *d_br = nullptr;  // FP64 tables
         auto drop = [&](hipError_t e) {
             (void)hipStreamSynchronize(c->stream);
-            for (void* p : {d_ch, d_b, t.chirp, t.bhat})
+            for (void* p : {d_ch, d_b, d_br, t.chirp, t.bhat, t.bhat_rows})
                 if (p) (void)hipFree(p);
             return fail(FOCSS_E_HIP, std::string("Bluestein table: ") + hipGetErrorString(e));
         };
         hipError_t e = dev_malloc(&d_ch, (size_t)n * sizeof(cx<double>));
         if (e == hipSuccess) e = dev_malloc(&d_b, (size_t)M * sizeof(cx<double>));
+        if (e == hipSuccess && rows) e = dev_malloc(&d_br, (size_t)M * sizeof(cx<double>));
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_ch, ch.data(), (size_t)n * sizeof(cx<double>), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
@@ -463,29 +466,39 @@ int blue_table(focss_ctx* c, int n, const cx<R>** chirp, const cx<R>** bhat, int
             drop(hipSuccess);
             return rc;
         }
+        if (rows)
+            hipLaunchKernelGGL(k_blu_row_order<double>, dim3((M + 255) / 256), dim3(256), 0, c->stream,
+                               (const cx<double>*)d_b, (cx<double>*)d_br, lm, blu_log_n1(lm));
         if (prec == FOCSS_F32) {
             e = dev_malloc(&t.chirp, (size_t)n * sizeof(cx<float>));
             if (e == hipSuccess) e = dev_malloc(&t.bhat, (size_t)M * sizeof(cx<float>));
+            if (e == hipSuccess && rows) e = dev_malloc(&t.bhat_rows, (size_t)M * sizeof(cx<float>));
             if (e != hipSuccess) return drop(e);
             hipLaunchKernelGGL(k_from_c128<float>, dim3((n + 255) / 256), dim3(256), 0, c->stream,
                                (const cx<double>*)d_ch, (cx<float>*)t.chirp, (long long)n);
             hipLaunchKernelGGL(k_from_c128<float>, dim3((M + 255) / 256), dim3(256), 0, c->stream,
                                (const cx<double>*)d_b, (cx<float>*)t.bhat, (long long)M);
+            if (rows)
+                hipLaunchKernelGGL(k_from_c128<float>, dim3((M + 255) / 256), dim3(256), 0, c->stream,
+                                   (const cx<double>*)d_br, (cx<float>*)t.bhat_rows, (long long)M);
         }
-        e = hipStreamSynchronize(c->stream);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return drop(e);
         if (prec == FOCSS_F64) {
             t.chirp = d_ch;
             t.bhat = d_b;
+            t.bhat_rows = d_br;
         } else {
-            (void)hipFree(d_ch);
-            (void)hipFree(d_b);
+            for (void* p : {d_ch, d_b, d_br})
+                if (p) (void)hipFree(p);
         }
         it = c->blue.emplace(key, t).first;
     }
     *chirp = (const cx<R>*)it->second.chirp;
     *bhat = (const cx<R>*)it->second.bhat;
     *log_m = it->second.log_m;
+    if (bhat_rows) *bhat_rows = (const cx<R>*)it->second.bhat_rows;
     return 0;
 }
 
@@ -547,9 +560,10 @@ int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data, void* w
 // the power-of-two length 2^log_m of their Bluestein convolutions
 struct GenClass {
     int log_m, modes;
+    int fused;         // log_m >= kBluMinLog: the four-launch k_blu_* kernels
     int first, count;  // into the waves array
     int max_n;         // longest waveform of the class
-    long long work_off;  // its rows' first value in gen_work / gen_tmp
+    long long work_off;  // its rows' first value in gen_work
 };
 
 template <typename R>
@@ -566,27 +580,62 @@ GenArgs<R> gen_args(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
     return g;
 }
 
+template <typename R>
+BluArgs<R> blu_args(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
+    BluArgs<R> g;
+    g.p = a;
+    g.p.waves += gc.first;
+    g.gw = (const GenWave<R>*)c->d_genwave;
+    g.work = (cx<R>*)c->gen_work + gc.work_off;
+    g.active = c->d_active;
+    g.log_m = gc.log_m;
+    g.tiles = 0;
+    return g;
+}
+
 // the spectral op of counter a.counter: two Bluestein DFTs around the
-// dispersion multiply
+// dispersion multiply (whole-row DFT kernels below 2^kBluMinLog, else the
+// fused row / column kernels)
 template <typename R>
 void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb) {
+    if (gc.fused) {
+        const BluRegistry<R>& r = blu_reg<R>();
+        BluArgs<R> g = blu_args<R>(c, gc, a);
+        g.tiles = gc.modes * r.row_tiles[gc.log_m];
+        const dim3 rows_grid(gc.count * g.tiles);
+        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, dim3(NT), 0, c->stream, g, 0, tb);
+        BluArgs<R> gm = g;
+        gm.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
+        hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][BLU_MID], dim3(gc.count * gm.tiles), dim3(NT), 0, c->stream,
+                           gm, tb);
+        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, dim3(NT), 0, c->stream, g, 1, tb);
+        return;
+    }
     const GenArgs<R> g = gen_args<R>(c, gc, a);
     const int rows = gc.count * gc.modes;
     const dim3 grid(rows * g.chunks_per_row), block(256);
-    cx<R>* tmp = (cx<R>*)c->gen_tmp + gc.work_off;
     hipLaunchKernelGGL(k_gen_in<R>, grid, block, 0, c->stream, g);
-    fft_rows_launch<R>(c, tb, -1, rows, gc.log_m, g.work, tmp, 1.0);
+    fft_rows_launch<R>(c, tb, -1, rows, gc.log_m, g.work, nullptr, 1.0);
     hipLaunchKernelGGL(k_gen_mul<R>, grid, block, 0, c->stream, g);
-    fft_rows_launch<R>(c, tb, +1, rows, gc.log_m, g.work, tmp, 1.0);
+    fft_rows_launch<R>(c, tb, +1, rows, gc.log_m, g.work, nullptr, 1.0);
     hipLaunchKernelGGL(k_gen_mid<R>, grid, block, 0, c->stream, g);
-    fft_rows_launch<R>(c, tb, -1, rows, gc.log_m, g.work, tmp, 1.0);
+    fft_rows_launch<R>(c, tb, -1, rows, gc.log_m, g.work, nullptr, 1.0);
     hipLaunchKernelGGL(k_gen_mul<R>, grid, block, 0, c->stream, g);
-    fft_rows_launch<R>(c, tb, +1, rows, gc.log_m, g.work, tmp, 1.0);
+    fft_rows_launch<R>(c, tb, +1, rows, gc.log_m, g.work, nullptr, 1.0);
     hipLaunchKernelGGL(k_gen_out<R>, grid, block, 0, c->stream, g);
 }
 
+// the time op of counter a.counter (entry: the first one)
 template <typename R>
-void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
+void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb, bool entry) {
+    if (gc.fused) {
+        const BluRegistry<R>& r = blu_reg<R>();
+        BluArgs<R> g = blu_args<R>(c, gc, a);
+        g.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
+        hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][entry ? BLU_ENTRY : BLU_OUT], dim3(gc.count * g.tiles),
+                           dim3(NT), 0, c->stream, g, tb);
+        return;
+    }
     GenArgs<R> g = gen_args<R>(c, gc, a);
     g.p.tiles_per_wave = (gc.max_n + kGenTimeChunk - 1) / kGenTimeChunk;
     void (*k)(GenArgs<R>) = gc.modes == 1 ? &k_gen_time<R, 1> : &k_gen_time<R, 2>;
@@ -607,7 +656,7 @@ int prepare_generic(focss_ctx* c, int n, const focss_link* links, const std::vec
             const int i = order[gc.first + k];
             GenWave<R>& w = gw[i];
             int lm = 0;
-            if (int rc = blue_table<R>(c, links[i].samples, &w.chirp, &w.bhat, &lm)) return rc;
+            if (int rc = blue_table<R>(c, links[i].samples, &w.chirp, &w.bhat, &lm, &w.bhat_rows)) return rc;
             w.row0 = (long long)k * gc.modes;
             w.n = links[i].samples;
             w.pad = 0;
@@ -615,9 +664,7 @@ int prepare_generic(focss_ctx* c, int n, const focss_link* links, const std::vec
         }
         values += (long long)gc.count * gc.modes << gc.log_m;
     }
-    const size_t bytes = (size_t)values * sizeof(cx<R>);
-    if (int rc = grow(c, &c->gen_work, &c->gen_work_bytes, bytes)) return rc;
-    if (int rc = grow(c, &c->gen_tmp, &c->gen_tmp_bytes, bytes)) return rc;
+    if (int rc = grow(c, &c->gen_work, &c->gen_work_bytes, (size_t)values * sizeof(cx<R>))) return rc;
     HIP_TRY(hipMemcpyAsync(c->d_genwave, gw.data(), (size_t)n * sizeof(GenWave<R>), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));  // gw is released on return
     return 0;
@@ -649,6 +696,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
             GenClass gc;
             gc.log_m = kv.first.first - 100;
             gc.modes = kv.first.second;
+            gc.fused = gc.log_m >= kBluMinLog;
             gc.first = (int)order.size();
             gc.count = (int)kv.second.size();
             gc.max_n = 0;
@@ -772,7 +820,6 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         for (const LengthClass& lc : two_pass) put(&lc, sizeof(lc));
         for (const GenClass& gc : gen) put(&gc, sizeof(gc));
         put(&c->gen_work, sizeof(void*));
-        put(&c->gen_tmp, sizeof(void*));
         put(&c->d_genwave, sizeof(void*));
         put(&c->d_active, sizeof(void*));
         if (c->use_graph && (c->graph.key != key || !c->graph.exec)) {
@@ -785,7 +832,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                 for (const GenClass& gc : gen) launch_gen_spec<R>(c, gc, ak, tb);
                 ak.counter = k + 1;
                 for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
-                for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak);
+                for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak, tb, false);
             }
             HIP_TRY(hipStreamEndCapture(c->stream, &c->graph.graph));
             HIP_TRY(hipGraphInstantiate(&c->graph.exec, c->graph.graph, nullptr, nullptr, 0));
@@ -814,7 +861,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     if (stepped) {
         // entry time pass (counter 0), then replays of the graph
         for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, a, tb);
-        for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, a);
+        for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, a, tb, true);
         HIP_TRY(hipGetLastError());
         if (graph_replays > 0) {
             // measurement mode: `graph_replays` replays of the pass-pair
@@ -849,7 +896,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                     for (const GenClass& gc : gen) launch_gen_spec<R>(c, gc, ak, tb);
                     ak.counter = k + 1;
                     for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
-                    for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak);
+                    for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak, tb, false);
                 }
                 HIP_TRY(hipGetLastError());
             }
@@ -1444,12 +1491,11 @@ int focss_ctx_destroy(focss_ctx* c) {
     free_control(c);
     if (c->d_done) (void)hipFree(c->d_done);
     if (c->scratch) (void)hipFree(c->scratch);
-    for (auto& kv : c->blue) {
-        if (kv.second.chirp) (void)hipFree(kv.second.chirp);
-        if (kv.second.bhat) (void)hipFree(kv.second.bhat);
-    }
+    for (auto& kv : c->blue)
+        for (void* p : {kv.second.chirp, kv.second.bhat, kv.second.bhat_rows})
+            if (p) (void)hipFree(p);
     c->blue.clear();
-    for (void* p : {c->blu_work, c->gen_work, c->gen_tmp})
+    for (void* p : {c->blu_work, c->gen_work})
         if (p) (void)hipFree(p);
     if (c->h_done) (void)hipHostFree(c->h_done);
     for (int i = 0; i < 2; ++i)

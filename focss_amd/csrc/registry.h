@@ -5,6 +5,7 @@
 #pragma once
 #include "ssfm_kernels.h"
 #include "fft_kernels.h"
+#include "bluestein_fused.h"
 
 namespace focss_hip {
 
@@ -108,6 +109,35 @@ void reg_fft_two(FftRegistry<R>& r) {
     r.nrows[LOGN] = S;
 }
 
+// fused Bluestein link kernels: [log_m], [log_m][modes-1][kind]
+template <typename R>
+struct BluRegistry {
+    void (*rows[21])(BluArgs<R>, int, Tables<R>) = {};
+    void (*cols[21][2][3])(BluArgs<R>, Tables<R>) = {};
+    int row_tiles[21] = {};     // row-kernel workgroups per mode (N1 / S)
+    int col_tiles[21][2] = {};  // column-kernel workgroups per waveform (N2 / C)
+};
+
+template <typename R, int LOGM, int MODES>
+void reg_blu_modes(BluRegistry<R>& r) {
+    using Sh = BluShape<R, LOGM>;
+    r.cols[LOGM][MODES - 1][BLU_ENTRY] = &k_blu_cols<R, LOGM, MODES, BLU_ENTRY>;
+    r.cols[LOGM][MODES - 1][BLU_MID] = &k_blu_cols<R, LOGM, MODES, BLU_MID>;
+    r.cols[LOGM][MODES - 1][BLU_OUT] = &k_blu_cols<R, LOGM, MODES, BLU_OUT>;
+    r.col_tiles[LOGM][MODES - 1] = Sh::N2 / Sh::template C<MODES>;
+}
+
+template <typename R, int LOGM>
+void reg_blu(BluRegistry<R>& r) {
+    using Sh = BluShape<R, LOGM>;
+    static_assert(Sh::N1 % Sh::S == 0 && Sh::N2 % Sh::template C<1> == 0 && Sh::N2 % Sh::template C<2> == 0,
+                  "Bluestein tile shape");
+    r.rows[LOGM] = &k_blu_rows<R, LOGM>;
+    r.row_tiles[LOGM] = Sh::N1 / Sh::S;
+    reg_blu_modes<R, LOGM, 1>(r);
+    reg_blu_modes<R, LOGM, 2>(r);
+}
+
 // the parts, one per translation unit
 void fill_resident(Registry<double>& r);
 void fill_resident(Registry<float>& r);
@@ -117,6 +147,8 @@ void fill_two_pass_large(Registry<double>& r);  // 2^16 .. 2^20
 void fill_two_pass_large(Registry<float>& r);
 void fill_fft(FftRegistry<double>& r);
 void fill_fft(FftRegistry<float>& r);
+void fill_blu(BluRegistry<double>& r);
+void fill_blu(BluRegistry<float>& r);
 
 template <typename R>
 const Registry<R>& reg() {
@@ -135,6 +167,16 @@ const FftRegistry<R>& fft_reg() {
     static const FftRegistry<R> r = [] {
         FftRegistry<R> x;
         fill_fft(x);
+        return x;
+    }();
+    return r;
+}
+
+template <typename R>
+const BluRegistry<R>& blu_reg() {
+    static const BluRegistry<R> r = [] {
+        BluRegistry<R> x;
+        fill_blu(x);
         return x;
     }();
     return r;
