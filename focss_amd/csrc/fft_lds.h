@@ -9,6 +9,7 @@
 // W_{Ns R}^{r k}, radix-R DFT, write to expand(j) + r Ns.  Radix-8 stages
 // first, then one radix-4 or radix-2 stage for the remainder.
 #pragma once
+#include <type_traits>
 #include "cx.h"
 
 namespace focss_hip {
@@ -34,6 +35,7 @@ __host__ __device__ constexpr int padded(int n) { return n + (n >> 3) + 1; }
 // W_L^m from a full length-L table in LDS
 template <typename R>
 struct TwFull {
+    static constexpr bool kPowers = false;
     const cx<R>* t;
     __device__ __forceinline__ cx<R> get(int m) const { return t[m]; }
 };
@@ -41,9 +43,17 @@ struct TwFull {
 // W_L^m = hi[m >> LB] * lo[m & (2^LB - 1)] from two short LDS tables
 template <typename R, int LB>
 struct TwSplit {
+    static constexpr bool kPowers = false;
     const cx<R>* lo;
     const cx<R>* hi;
     __device__ __forceinline__ cx<R> get(int m) const { return hi[m >> LB] * lo[m & ((1 << LB) - 1)]; }
+};
+
+// The same two tables for the split-stage DFTs, whose stages look up one
+// twiddle W per butterfly and form W^r, r < RADIX, in registers
+template <typename R, int LB>
+struct TwSplitPow : TwSplit<R, LB> {
+    static constexpr bool kPowers = true;
 };
 
 // Fill the LDS twiddle tables for length L from the global bank.
@@ -202,6 +212,29 @@ __device__ __forceinline__ void fft_lds(cx<R>* __restrict__ buf, const TW& tw) {
     if constexpr (L > 1) fft_stages<R, L, S, NT, SIGN, 1, TW>(buf, tw);
 }
 
+// Twiddle tables of the split-stage DFTs of length L: FP64 from the two
+// short tables with powers formed in registers (little LDS, two lookups
+// per butterfly); FP32 from the full table (powers of a rounded complex64
+// W drift: 1.1e-3 against 1.7e-4 field error over 1500 split steps)
+template <typename R, int L>
+struct PassTw {
+    static constexpr bool kSplit = sizeof(R) == 8;
+    static constexpr int LB = (ilog2<L>::value + 1) / 2;
+    static constexpr int A = kSplit ? (1 << LB) : L;  // sizes of the two LDS arrays
+    static constexpr int B = kSplit ? (L >> LB) : 1;
+    using type = std::conditional_t<kSplit, TwSplitPow<R, LB>, TwFull<R>>;
+    template <int NT>
+    static __device__ __forceinline__ type load(cx<R>* a, cx<R>* b, const cx<R>* __restrict__ bank) {
+        if constexpr (kSplit) {
+            load_tw_split<R, L, LB, NT>(a, b, bank);
+            return type{{a, b}};
+        } else {
+            load_tw_full<R, L, NT>(a, bank);
+            return type{a};
+        }
+    }
+};
+
 // ------------------------------------------------------------------
 // Split-stage DFTs for the two-pass kernels.  A length-L DFT (L >= 64) runs
 // the mixed-radix schedule P(L) = (8, 2^(log2 L mod 3), 8, ..., 8), or its
@@ -211,8 +244,9 @@ __device__ __forceinline__ void fft_lds(cx<R>* __restrict__ buf, const TW& tw) {
 // operation needs: the first stage is fed from HBM, the last one feeds
 // either HBM or the pointwise operator and the next DFT's first stage
 // (which must use the reversed schedule), with no LDS round trip.
-// Stage twiddles W_{NS R}^{r k} = W_L^{r k L/(NS R)} come from a length-L
-// table in LDS, filled at kernel start.
+// Stage twiddles W_{NS R}^{r k} = W_L^{r k L/(NS R)}: one lookup of
+// W_L^{k L/(NS R)} per butterfly (TwSplitPow: two short LDS tables), its
+// powers by products of depth <= 3 (radix 8) or 4 (radix 16).
 // ------------------------------------------------------------------
 template <int L, bool REV>
 struct Sched {
@@ -233,21 +267,43 @@ struct Sched {
 };
 
 // w[r] = W_L^{r k STRIDE} (SIGN -1) or its conjugate (SIGN +1), 1 <= r < RADIX,
-// read from the length-L table t in LDS
-template <typename R, int RADIX, int SIGN, int STRIDE>
-__device__ __forceinline__ void stage_twiddles(const cx<R>* __restrict__ t, int k, cx<R>* w) {
+// from the table tw: every power looked up, or (kPowers) one lookup and
+// products
+template <typename R, int RADIX, int SIGN, int STRIDE, typename TW>
+__device__ __forceinline__ void stage_twiddles(const TW& tw, int k, cx<R>* w) {
+    if constexpr (TW::kPowers) {
+        cx<R> w1 = tw.get(k * STRIDE);
+        if constexpr (SIGN > 0) w1 = conj(w1);
+        w[1] = w1;
+        if constexpr (RADIX >= 4) {
+            w[2] = w1 * w1;
+            w[3] = w[2] * w1;
+        }
+        if constexpr (RADIX >= 8) {
+            w[4] = w[2] * w[2];
+            w[5] = w[4] * w1;
+            w[6] = w[4] * w[2];
+            w[7] = w[4] * w[3];
+        }
+        if constexpr (RADIX >= 16) {
+            w[8] = w[4] * w[4];
 #pragma unroll
-    for (int r = 1; r < RADIX; ++r) {
-        w[r] = t[r * k * STRIDE];
-        if constexpr (SIGN > 0) w[r] = conj(w[r]);
+            for (int r = 9; r < 16; ++r) w[r] = w[8] * w[r - 8];
+        }
+    } else {
+#pragma unroll
+        for (int r = 1; r < RADIX; ++r) {
+            w[r] = tw.get(r * k * STRIDE);
+            if constexpr (SIGN > 0) w[r] = conj(w[r]);
+        }
     }
 }
 
 // One Stockham stage (radix RADIX, NS = product of earlier radices) over S
 // interleaved sequences in LDS, NT threads; barriers before the writes and
-// after them.  tw is the length-L table W_L^m in LDS.
-template <typename R, int L, int S, int NT, int SIGN, int RADIX, int NS>
-__device__ __forceinline__ void stockham_stage(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw) {
+// after them.  tw gives W_L^m.
+template <typename R, int L, int S, int NT, int SIGN, int RADIX, int NS, typename TW>
+__device__ __forceinline__ void stockham_stage(cx<R>* __restrict__ buf, const TW& tw) {
     constexpr int NB = L / RADIX;
     constexpr int TOTAL = NB * S;
     constexpr int PER = (TOTAL + NT - 1) / NT;
@@ -287,8 +343,8 @@ __device__ __forceinline__ void stockham_stage(cx<R>* __restrict__ buf, const cx
 }
 
 // stages 1 .. N-2 of the schedule (the first and last run in registers)
-template <typename R, int L, int S, int NT, int SIGN, bool REV, int I = 1>
-__device__ __forceinline__ void middle_stages(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw) {
+template <typename R, int L, int S, int NT, int SIGN, bool REV, int I = 1, typename TW>
+__device__ __forceinline__ void middle_stages(cx<R>* __restrict__ buf, const TW& tw) {
     using SC = Sched<L, REV>;
     if constexpr (I < SC::N - 1) {
         stockham_stage<R, L, S, NT, SIGN, SC::radix(I), SC::ns(I)>(buf, tw);
@@ -305,12 +361,11 @@ __device__ __forceinline__ void first_stage_store(cx<R>* __restrict__ buf, cx<R>
     for (int r = 0; r < 8; ++r) buf[pad((j * 8 + r) * S + s)] = v[r];
 }
 
-// last stage: reads x[j + r L/8] of sequence s, twiddles by W_L^{r j} from
-// the length-L table tw in LDS, radix-8 DFT; v[r] is then output element
-// j + r L/8
-template <typename R, int L, int S, int SIGN>
-__device__ __forceinline__ void last_stage_load(const cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw,
-                                                cx<R>* v, int s, int j) {
+// last stage: reads x[j + r L/8] of sequence s, twiddles by W_L^{r j},
+// radix-8 DFT; v[r] is then output element j + r L/8
+template <typename R, int L, int S, int SIGN, typename TW>
+__device__ __forceinline__ void last_stage_load(const cx<R>* __restrict__ buf, const TW& tw, cx<R>* v, int s,
+                                                int j) {
     constexpr int Q = L / 8;
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = buf[pad((j + r * Q) * S + s)];

@@ -271,8 +271,9 @@ void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     a.log_n = lc.log_n;
     a.layout_g = lc.layout_g;
     KernelFn<R> k = r.time_k[lc.log_n][lc.modes - 1][lc.variant];
-    hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]),
-                       0, c->stream, a, tb);
+    const int groups = lc.count * (a.tiles_per_wave / r.time_tpb[lc.log_n][lc.modes - 1][lc.variant]);
+    hipLaunchKernelGGL(k, dim3(groups), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]), 0, c->stream, a,
+                       tb);
 }
 
 template <typename R>
@@ -286,8 +287,8 @@ void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     a.layout_g = lc.layout_g;
     a.time_tiles = (1 << (lc.log_n - pass_log_n1(lc.log_n))) / r.time_cols[lc.log_n][lc.modes - 1][lc.variant];
     KernelFn<R> k = r.spec_k[lc.log_n][lc.spec_variant];
-    hipLaunchKernelGGL(k, dim3(lc.count * a.tiles_per_wave), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0,
-                       c->stream, a, tb);
+    const int groups = lc.count * (a.tiles_per_wave / r.spec_tpb[lc.log_n][lc.spec_variant]);
+    hipLaunchKernelGGL(k, dim3(groups), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0, c->stream, a, tb);
 }
 
 // natural order back after the programs of an interleaved class end
@@ -1416,12 +1417,12 @@ int focss_ctx_create(int device, focss_ctx** out) {
     const char* nl = getenv("FOCSS_NATURAL_LAYOUT");
     c->natural_layout = nl && nl[0] == '1';
     for (int p = 0; p <= 20; ++p) {
-        // measured defaults at 2^18 samples: time pass 2048 (complex128) /
-        // 4096 (complex64) elements per tile, spectral pass 1024
+        // measured defaults at 2^18 samples: time pass 2048 elements per
+        // tile; spectral pass 1024, two tiles per workgroup for complex64
         c->tile_variant[0][FOCSS_F64][p] = 1;
-        c->tile_variant[0][FOCSS_F32][p] = 0;
+        c->tile_variant[0][FOCSS_F32][p] = 1;
         c->tile_variant[1][FOCSS_F64][p] = 2;
-        c->tile_variant[1][FOCSS_F32][p] = 2;
+        c->tile_variant[1][FOCSS_F32][p] = 5;
     }
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
@@ -2324,9 +2325,11 @@ int focss_set_tile_pass(focss_ctx* c, int precision, int log2_samples, int pass,
 }
 
 int focss_tile_variants(int precision, int log2_samples) {
+    // variants are numbered from 0; a pass without variant v runs its first
+    // usable one
     int n = 0;
     for (int v = 0; v < kTileVariants; ++v)
-        if (two_pass_ok(precision, log2_samples, 1, v)) ++n;
+        if (time_ok(precision, log2_samples, 1, v) || spec_ok(precision, log2_samples, v)) n = v + 1;
     return n;
 }
 

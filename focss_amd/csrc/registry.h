@@ -12,7 +12,9 @@ namespace focss_hip {
 template <typename R>
 using KernelFn = void (*)(PassArgs, Tables<R>);
 
-constexpr int kTileVariants = 3;
+// variants 0-2: one tile per workgroup; 3-5: the same tiles, two per
+// workgroup (the second one's loads in flight during the first's DFTs)
+constexpr int kTileVariants = 6;
 
 #ifndef FOCSS_SPLIT_SHIFT
 #define FOCSS_SPLIT_SHIFT 0
@@ -21,7 +23,8 @@ constexpr int kTileVariants = 3;
 // pass runs length-N1 DFTs down columns, the spectral pass length-N2 DFTs
 // along rows.
 constexpr int pass_log_n1(int log_n) { return log_n / 2 - (log_n >= 16 ? FOCSS_SPLIT_SHIFT : 0); }
-constexpr int kTileElems[kTileVariants] = {4096, 2048, 1024};
+constexpr int kTileElems[kTileVariants] = {4096, 2048, 1024, 4096, 2048, 1024};
+constexpr int kTilesPerGroup[kTileVariants] = {1, 1, 1, 2, 2, 2};
 
 template <typename R>
 struct Registry {
@@ -34,6 +37,8 @@ struct Registry {
     int spec_rows[21][kTileVariants] = {};
     int time_threads[21][2][kTileVariants] = {};
     int spec_threads[21][kTileVariants] = {};
+    int time_tpb[21][2][kTileVariants] = {};  // tiles per workgroup
+    int spec_tpb[21][kTileVariants] = {};
 };
 
 template <typename R, int N, int M>
@@ -47,22 +52,28 @@ void reg_time(Registry<R>& r) {
     constexpr int N2 = 1 << (LOGN - pass_log_n1(LOGN));
     constexpr int C0 = kTileElems[V] / (N1 * M);
     constexpr int C = C0 < 1 ? 0 : (C0 > N2 ? N2 : C0);
-    if constexpr (C >= 1) {
-        r.time_k[LOGN][M - 1][V] = &k_time_pass<R, N1, C, M>;
+    constexpr int T = kTilesPerGroup[V];
+    if constexpr (C >= 1 && (N2 / (C < 1 ? 1 : C)) % T == 0) {
+        r.time_k[LOGN][M - 1][V] = &k_time_pass<R, N1, C, M, T>;
         r.time_cols[LOGN][M - 1][V] = C;
         r.time_threads[LOGN][M - 1][V] = pass_threads(C * N1 / 8);
+        r.time_tpb[LOGN][M - 1][V] = T;
     }
 }
 
 template <typename R, int LOGN, int V>
 void reg_spec(Registry<R>& r) {
     constexpr int N2 = 1 << (LOGN - pass_log_n1(LOGN));
+    constexpr int N1 = 1 << pass_log_n1(LOGN);
     constexpr int S0 = kTileElems[V] / N2;
-    constexpr int S = S0 < 1 ? 0 : S0;
-    if constexpr (S >= 1) {
-        r.spec_k[LOGN][V] = &k_spec_pass<R, N2, S>;
+    constexpr int S = S0 < 1 ? 0 : (S0 > N1 ? N1 : S0);
+    constexpr int T = kTilesPerGroup[V];
+    // tiles per waveform are modes N1 / S: a multiple of T for either mode count
+    if constexpr (S >= 1 && (N1 / (S < 1 ? 1 : S)) % T == 0) {
+        r.spec_k[LOGN][V] = &k_spec_pass<R, N2, S, T>;
         r.spec_rows[LOGN][V] = S;
         r.spec_threads[LOGN][V] = pass_threads(S * N2 / 8);
+        r.spec_tpb[LOGN][V] = T;
     }
 }
 
@@ -71,12 +82,21 @@ void reg_two_pass(Registry<R>& r) {
     reg_time<R, LOGN, 1, 0>(r);
     reg_time<R, LOGN, 1, 1>(r);
     reg_time<R, LOGN, 1, 2>(r);
+    reg_time<R, LOGN, 1, 3>(r);
+    reg_time<R, LOGN, 1, 4>(r);
+    reg_time<R, LOGN, 1, 5>(r);
     reg_time<R, LOGN, 2, 0>(r);
     reg_time<R, LOGN, 2, 1>(r);
     reg_time<R, LOGN, 2, 2>(r);
+    reg_time<R, LOGN, 2, 3>(r);
+    reg_time<R, LOGN, 2, 4>(r);
+    reg_time<R, LOGN, 2, 5>(r);
     reg_spec<R, LOGN, 0>(r);
     reg_spec<R, LOGN, 1>(r);
     reg_spec<R, LOGN, 2>(r);
+    reg_spec<R, LOGN, 3>(r);
+    reg_spec<R, LOGN, 4>(r);
+    reg_spec<R, LOGN, 5>(r);
 }
 
 // standalone DFT kernels: [log_n][sign 0 = forward, 1 = inverse]

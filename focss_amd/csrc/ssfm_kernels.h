@@ -16,6 +16,7 @@
 //          -> inverse length-N2 DFT
 //    so one split step reads and writes the field exactly twice.
 #pragma once
+#include <type_traits>
 #include "device_util.h"
 #include "fft_lds.h"
 #include "link.h"
@@ -249,8 +250,14 @@ __device__ __forceinline__ void inter_twiddles(const Tables<R>& tb, unsigned n2,
     for (int r = 0; r < 8; ++r) t[r] = mk<R>((R)d[r].x, (R)d[r].y);
 }
 
-// time pass: C columns x M modes, each column of length N1 with stride N2
-template <typename R, int N1, int C, int M>
+// time pass: C columns x M modes, each column of length N1 with stride N2.
+// A workgroup runs TPB consecutive tiles of one waveform; with TPB = 2 the
+// second tile's loads are issued before the first tile's DFTs (barriers
+// wait on LDS only, so they stay in flight).  The inter-pass twiddles are
+// formed at the load and again at the store rather than held in registers
+// across both DFTs where that pays (124 instead of 150 VGPRs for 512 x 4
+// complex128: four waves per SIMD instead of three).
+template <typename R, int N1, int C, int M, int TPB>
 __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs a, Tables<R> tb) {
     constexpr int S = C * M;
     constexpr int Q = N1 / 8;
@@ -258,18 +265,20 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
     constexpr int TN = pass_threads(ITEMS);
     constexpr int PER = (ITEMS + TN - 1) / TN;
     static_assert(N1 >= 64, "split-stage DFT needs length >= 64");
+    using TW = PassTw<R, N1>;
     __shared__ cx<R> buf[padded(N1 * S)];
-    __shared__ cx<R> tw_t[N1];
+    __shared__ cx<R> tw_a[TW::A];
+    __shared__ cx<R> tw_b[TW::B];
     __shared__ double redbuf[2 * (TN / 64)];
     const int blk = xcd_block();
-    const int w_id = a.waves[blk / a.tiles_per_wave];
-    const int tile = blk % a.tiles_per_wave;
+    const int groups = a.tiles_per_wave / TPB;
+    const int w_id = a.waves[blk / groups];
+    const int tile0 = (blk % groups) * TPB;
     const WaveState st = a.states[(a.counter & 1) * a.n_waves + w_id];
     if (st.top == TOP_NONE) return;
     const WaveStatic w = a.statics[w_id];
     const int N = w.samples;
     const int N2 = N / N1;
-    const int n2_0 = tile * C;
     cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset;
     const bool inverse = st.top != TOP_ENTRY;
     const bool forward = st.top != TOP_EXIT;
@@ -277,105 +286,148 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
     // are r Q N2 apart in either order
     const bool il = a.layout_g == 2 && st.top != TOP_ENTRY;
     const long long rstride = (long long)Q * N2;
-    load_tw_full<R, N1, TN>(tw_t, tb.bank);
-    cx<R> v[PER][M][8];
-    cx<R> itw[PER][8];
+    auto at = [&](int j, int n2) -> long long {
+        return il ? Interleave<R>::at(j, n2, N2) : (long long)j * N2 + n2;
+    };
+    auto load = [&](int tile, cx<R> (&dst)[PER][M][8]) {
 #pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) {
-            const int c = id % C, j = id / C;
-            const int n2 = n2_0 + c;
-            const long long base = il ? Interleave<R>::at(j, n2, N2) : (long long)j * N2 + n2;
-            inter_twiddles<R>(tb, (unsigned)n2, (unsigned)j, (unsigned)Q, a.log_n, itw[b]);
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-#pragma unroll
-                for (int r = 0; r < 8; ++r) v[b][m][r] = g[(long long)m * N + base + r * rstride];
-            }
-            if (inverse) {
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const long long base = at(id / C, tile * C + id % C);
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
 #pragma unroll
-                    for (int r = 0; r < 8; ++r) v[b][m][r] = v[b][m][r] * conj(itw[b][r]);
-                    first_stage_store<R, N1, S, +1>(buf, v[b][m], m * C + c, j);
+                    for (int r = 0; r < 8; ++r) dst[b][m][r] = g[(long long)m * N + base + r * rstride];
                 }
             }
         }
-    }
-    if (inverse) {
-        __syncthreads();
-        middle_stages<R, N1, S, TN, +1, false>(buf, tw_t);
+    };
+    const auto tw_t = TW::template load<TN>(tw_a, tw_b, tb.bank);
+    static_assert(TPB == 1 || TPB == 2, "one or two tiles per workgroup");
+    // the inter-pass twiddles formed again at the store (single-mode columns,
+    // or two tiles) or held from the load (two modes: 88 against 111 us for
+    // 1024 x 2 x 2 complex128)
+    constexpr bool kRecompute = M == 1 || TPB > 1;
+    // (a loop here allocates fewer registers than the compile-time tile
+    // calls of the spectral pass)
+    cx<R> v[PER][M][8];
+#pragma unroll
+    for (int t = 0; t < TPB; ++t) {
+        const int tile = tile0 + t;
+        const int n2_0 = tile * C;
+        cx<R> itw[PER][8];
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+                inter_twiddles<R>(tb, (unsigned)(n2_0 + c), (unsigned)j, (unsigned)Q, a.log_n, itw[b]);
+            }
+        }
+        if (t == 0) load(tile, v);  // later tiles: loaded during the previous one
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+                if (inverse) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+#pragma unroll
+                        for (int r = 0; r < 8; ++r) v[b][m][r] = v[b][m][r] * conj(itw[b][r]);
+                        first_stage_store<R, N1, S, +1>(buf, v[b][m], m * C + c, j);
+                    }
+                }
+            }
+        }
+        cx<R> vn[PER][M][8];
+        if (t + 1 < TPB) load(tile + 1, vn);
+        if (inverse) {
+            __syncthreads();
+            middle_stages<R, N1, S, TN, +1, false>(buf, tw_t);
+#pragma unroll
+            for (int b = 0; b < PER; ++b) {
+                const int id = threadIdx.x + b * TN;
+                if (ITEMS % TN == 0 || id < ITEMS) {
+                    const int c = id % C, j = id / C;
+#pragma unroll
+                    for (int m = 0; m < M; ++m) last_stage_load<R, N1, S, +1>(buf, tw_t, v[b][m], m * C + c, j);
+                }
+            }
+        }
+        double pk = 0, sm = 0;
 #pragma unroll
         for (int b = 0; b < PER; ++b) {
             const int id = threadIdx.x + b * TN;
             if (ITEMS % TN == 0 || id < ITEMS) {
                 const int c = id % C, j = id / C;
 #pragma unroll
-                for (int m = 0; m < M; ++m) last_stage_load<R, N1, S, +1>(buf, tw_t, v[b][m], m * C + c, j);
+                for (int r = 0; r < 8; ++r) {
+                    cx<R> u[M];
+#pragma unroll
+                    for (int m = 0; m < M; ++m) u[m] = v[b][m][r];
+                    time_op<R, M>(u, w, st, (long long)(j + r * Q) * N2 + n2_0 + c, pk, sm);
+#pragma unroll
+                    for (int m = 0; m < M; ++m) v[b][m][r] = u[m];
+                }
             }
         }
-    }
-    double pk = 0, sm = 0;
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) {
-            const int c = id % C, j = id / C;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                cx<R> u[M];
-#pragma unroll
-                for (int m = 0; m < M; ++m) u[m] = v[b][m][r];
-                time_op<R, M>(u, w, st, (long long)(j + r * Q) * N2 + n2_0 + c, pk, sm);
-#pragma unroll
-                for (int m = 0; m < M; ++m) v[b][m][r] = u[m];
-            }
+        // the reduction's barriers also order the last-stage LDS reads above
+        // before the first-stage writes below
+        block_max_sum<TN>(pk, sm, redbuf);
+        if (threadIdx.x == 0) {
+            Reduction* rd = a.red + (a.counter % 3) * a.n_waves + w_id;
+            atomicMax(&rd->peak_bits, double_to_bits(pk));
+            atomicAdd(&rd->sum, sm);
+            // the logarithmic grid's step count depends on the sum; keep the
+            // tile sums so that it is formed in tile order, whatever order
+            // the workgroups finish in
+            if (w.grid == GRID_LOGARITHMIC)
+                a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + tile] = sm;
         }
-    }
-    // the reduction's barriers also order the last-stage LDS reads above
-    // before the first-stage writes below
-    block_max_sum<TN>(pk, sm, redbuf);
-    if (threadIdx.x == 0) {
-        Reduction* r = a.red + (a.counter % 3) * a.n_waves + w_id;
-        atomicMax(&r->peak_bits, double_to_bits(pk));
-        atomicAdd(&r->sum, sm);
-        // the logarithmic grid's step count depends on the sum; keep the
-        // tile sums so that it is formed in tile order, whatever order the
-        // workgroups finish in
-        if (w.grid == GRID_LOGARITHMIC)
-            a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + tile] = sm;
-    }
-    if (forward) {
+        if (forward) {
+#pragma unroll
+            for (int b = 0; b < PER; ++b) {
+                const int id = threadIdx.x + b * TN;
+                if (ITEMS % TN == 0 || id < ITEMS) {
+                    const int c = id % C, j = id / C;
+#pragma unroll
+                    for (int m = 0; m < M; ++m) first_stage_store<R, N1, S, -1>(buf, v[b][m], m * C + c, j);
+                }
+            }
+            __syncthreads();
+            middle_stages<R, N1, S, TN, -1, true>(buf, tw_t);
+        }
 #pragma unroll
         for (int b = 0; b < PER; ++b) {
             const int id = threadIdx.x + b * TN;
             if (ITEMS % TN == 0 || id < ITEMS) {
                 const int c = id % C, j = id / C;
+                const int n2 = n2_0 + c;
+                const long long base = at(j, n2);
+                if (forward && kRecompute) inter_twiddles<R>(tb, (unsigned)n2, (unsigned)j, (unsigned)Q, a.log_n, itw[b]);
 #pragma unroll
-                for (int m = 0; m < M; ++m) first_stage_store<R, N1, S, -1>(buf, v[b][m], m * C + c, j);
+                for (int m = 0; m < M; ++m) {
+                    if (forward) {
+                        last_stage_load<R, N1, S, -1>(buf, tw_t, v[b][m], m * C + c, j);
+#pragma unroll
+                        for (int r = 0; r < 8; ++r) v[b][m][r] = v[b][m][r] * itw[b][r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) g[(long long)m * N + base + r * rstride] = v[b][m][r];
+                }
             }
         }
-        __syncthreads();
-        middle_stages<R, N1, S, TN, -1, true>(buf, tw_t);
-    }
+        if (t + 1 < TPB) {
+            // this tile's last-stage LDS reads before the next tile's writes
+            __syncthreads();
 #pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) {
-            const int c = id % C, j = id / C;
-            const int n2 = n2_0 + c;
-            const long long base = il ? Interleave<R>::at(j, n2, N2) : (long long)j * N2 + n2;
+            for (int b = 0; b < PER; ++b)
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                if (forward) {
-                    last_stage_load<R, N1, S, -1>(buf, tw_t, v[b][m], m * C + c, j);
+                for (int m = 0; m < M; ++m)
 #pragma unroll
-                    for (int r = 0; r < 8; ++r) v[b][m][r] = v[b][m][r] * itw[b][r];
-                }
-#pragma unroll
-                for (int r = 0; r < 8; ++r) g[(long long)m * N + base + r * rstride] = v[b][m][r];
-            }
+                    for (int r = 0; r < 8; ++r) v[b][m][r] = vn[b][m][r];
         }
     }
 }
@@ -420,22 +472,27 @@ __device__ __forceinline__ void spectral_multiply(cx<R>* v, const WaveStatic& w,
     }
 }
 
-// spectral pass: S rows of length N2; row q = m * N1 + k1 is contiguous
-template <typename R, int N2, int S>
+// spectral pass: S rows of length N2; row q = m * N1 + k1 is contiguous.
+// A workgroup runs TPB consecutive tiles of one waveform, the next tile's
+// loads issued before the present tile's DFTs.
+template <typename R, int N2, int S, int TPB>
 __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs a, Tables<R> tb) {
     constexpr int Q = N2 / 8;
     constexpr int ITEMS = S * Q;
     constexpr int TN = pass_threads(ITEMS);
     constexpr int PER = (ITEMS + TN - 1) / TN;
     static_assert(N2 >= 64, "split-stage DFT needs length >= 64");
+    using TW = PassTw<R, N2>;
     __shared__ cx<R> buf[padded(N2 * S)];
-    __shared__ cx<R> tw_t[N2];
+    __shared__ cx<R> tw_a[TW::A];
+    __shared__ cx<R> tw_b[TW::B];
     // plain block order: consecutive tiles are consecutive rows in memory,
     // and spreading them over the XCDs streams best (an XCD-grouped order
     // measured 4.4 against 5.6 TB/s for the same row traffic)
     const int blk = blockIdx.x;
-    const int w_id = a.waves[blk / a.tiles_per_wave];
-    const int tile = blk % a.tiles_per_wave;
+    const int groups = a.tiles_per_wave / TPB;
+    const int w_id = a.waves[blk / groups];
+    const int tile0 = (blk % groups) * TPB;
     WaveState st = a.states[(a.counter & 1) * a.n_waves + w_id];
     const int was_done = st.phase == PH_DONE;
     // the first spectral pass (after the entry op) reads natural order
@@ -450,7 +507,7 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
         for (int t = 0; t < a.time_tiles; ++t) sum += p[t];
     }
     SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
-    if (tile == 0 && threadIdx.x == 0) {
+    if (tile0 == 0 && threadIdx.x == 0) {
         a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
         Reduction z;
         z.peak_bits = 0;
@@ -461,57 +518,77 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
     if (!op.active) return;
     const int N = w.samples;
     const int N1 = N / N2;
-    cx<R>* g = reinterpret_cast<cx<R>*>(a.field) + w.offset + (long long)tile * S * N2;
-    load_tw_full<R, N2, TN>(tw_t, tb.bank);
-    cx<R> v[PER][8];
+    cx<R>* g0 = reinterpret_cast<cx<R>*>(a.field) + w.offset;
+    const auto tw_t = TW::template load<TN>(tw_a, tw_b, tb.bank);
     // offset of element j + r Q of row s of the tile: s N2 + j + r Q in
     // natural order, or Interleave(s, j) + 2 r Q (Q a multiple of W)
     auto row_at = [&](bool il, int s, int j) -> int {
         return il ? (int)Interleave<R>::at(s, j, N2) : s * N2 + j;
     };
+    auto load = [&](int tile, cx<R> (&dst)[PER][8]) {
+        const cx<R>* g = g0 + (long long)tile * S * N2;
 #pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) {
-            const int s = id % S, j = id / S;
-            const int o = row_at(il_in, s, j), rs = il_in ? 2 * Q : Q;
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int o = row_at(il_in, id % S, id / S), rs = il_in ? 2 * Q : Q;
 #pragma unroll
-            for (int r = 0; r < 8; ++r) v[b][r] = g[o + r * rs];
-            first_stage_store<R, N2, S, -1>(buf, v[b], s, j);
+                for (int r = 0; r < 8; ++r) dst[b][r] = g[o + r * rs];
+            }
         }
-    }
-    __syncthreads();
-    middle_stages<R, N2, S, TN, -1, false>(buf, tw_t);
-    const double arg = 0.5 * w.beta2 * op.s;
-    const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
+    };
+    static_assert(TPB == 1 || TPB == 2, "one or two tiles per workgroup");
+    cx<R> va[PER][8], vb[PER][8];
+    // one tile; v holds its values (tile 0 loads them here), vn receives the
+    // next tile's
+    auto run_tile = [&](auto tc, cx<R> (&v)[PER][8], cx<R> (&vn)[PER][8]) {
+        constexpr int t = decltype(tc)::value;
+        const int tile = tile0 + t;
+        cx<R>* g = g0 + (long long)tile * S * N2;
+        if constexpr (t == 0) load(tile, v);
 #pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) {
-            const int s = id % S, j = id / S;
-            last_stage_load<R, N2, S, -1>(buf, tw_t, v[b], s, j);
-            spectral_multiply<R>(v[b], w, (tile * S + s) % N1, j, N1, Q, arg, mag);
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) first_stage_store<R, N2, S, -1>(buf, v[b], id % S, id / S);
         }
-    }
-    __syncthreads();
+        if constexpr (t + 1 < TPB) load(tile + 1, vn);
+        __syncthreads();
+        middle_stages<R, N2, S, TN, -1, false>(buf, tw_t);
+        const double arg = 0.5 * w.beta2 * op.s;
+        const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
 #pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) first_stage_store<R, N2, S, +1>(buf, v[b], id % S, id / S);
-    }
-    __syncthreads();
-    middle_stages<R, N2, S, TN, +1, true>(buf, tw_t);
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int id = threadIdx.x + b * TN;
-        if (ITEMS % TN == 0 || id < ITEMS) {
-            const int s = id % S, j = id / S;
-            last_stage_load<R, N2, S, +1>(buf, tw_t, v[b], s, j);
-            const int o = row_at(il_out, s, j), rs = il_out ? 2 * Q : Q;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) g[o + r * rs] = v[b][r];
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int s = id % S, j = id / S;
+                last_stage_load<R, N2, S, -1>(buf, tw_t, v[b], s, j);
+                spectral_multiply<R>(v[b], w, (tile * S + s) % N1, j, N1, Q, arg, mag);
+            }
         }
-    }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) first_stage_store<R, N2, S, +1>(buf, v[b], id % S, id / S);
+        }
+        __syncthreads();
+        middle_stages<R, N2, S, TN, +1, true>(buf, tw_t);
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int s = id % S, j = id / S;
+                last_stage_load<R, N2, S, +1>(buf, tw_t, v[b], s, j);
+                const int o = row_at(il_out, s, j), rs = il_out ? 2 * Q : Q;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) g[o + r * rs] = v[b][r];
+            }
+        }
+        // this tile's last-stage LDS reads before the next tile's writes
+        if constexpr (t + 1 < TPB) __syncthreads();
+    };
+    run_tile(std::integral_constant<int, 0>{}, va, vb);
+    if constexpr (TPB > 1) run_tile(std::integral_constant<int, 1>{}, vb, va);
 }
 
 // Row-pair interleaved order back to natural order, in place: one workgroup
