@@ -431,6 +431,7 @@ def main():
             lm = rep["literal_model"]
             line["learned_dbp"]["literal_model"] = {
                 "model": lm["model"], "iters": lm["iters"], "train_seconds": lm["train_seconds"],
+                "init_chain_rel_err_in_band": lm["init_chain_rel_err_in_band"],
                 "loss_first": lm["loss_first"], "loss_last_finite": lm["loss_last_finite"],
                 "diverged": lm["diverged"],
                 "q_db": round(lm["q_db"], 3), "q_gap_vs_dbp16_db": round(lm["q_gap_vs_dbp16_db"], 3),

@@ -22,6 +22,10 @@ launches each --
   the save_field record reorder of 11 channels x 2 polarisations x 2^20
     samples, both directions, FP64;
   WdmPPE training, c = 30, d = 2, 2^14 samples (features, Gram, sums).
+--workload generic: plain DBP (UNIFORM, 16 steps per span, 2 spans) of
+`batch` waveforms of N = 2 n_sym samples, n_sym uniform in [2^11, 2^15]
+(seed 5: the Bluestein link kernels), a complex Gaussian field, `reps`
+calls of focss_link_run, each timed on the host.
 Usage:
     python -m focss_amd.probe [--batch 64] [--pairs 12] [--precision f64|f32] [--variant V]
     python -m focss_amd.probe --workload side [--reps 5]
@@ -42,7 +46,7 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
     ap.add_argument("--variant", type=int, default=-1, help="tile variant (default: the library's choice)")
-    ap.add_argument("--workload", choices=["ssfm", "side", "roofline", "c5"], default="ssfm")
+    ap.add_argument("--workload", choices=["ssfm", "side", "roofline", "c5", "generic"], default="ssfm")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--batches", default="64,128")
     ap.add_argument("--replays", type=int, default=4)
@@ -53,6 +57,8 @@ def main():
         return side(args)
     if args.workload in ("roofline", "c5"):
         return roofline(args)
+    if args.workload == "generic":
+        return generic(args)
     sys.path.insert(0, ROOT)
     import bench
     import focss_amd as F
@@ -142,6 +148,35 @@ def roofline(args):
                   f"copy {t_copy * 1e6:.1f} us ({nb / t_copy / 1e9:.0f} GB/s)", flush=True)
             other.free()
             buf.free()
+    return 0
+
+
+def generic(args):
+    sys.path.insert(0, ROOT)
+    import time
+    import focss_amd as F
+    from focss_amd.learned import ALPHA, BAUD, F_C, SPAN_LEN, fiber
+    prec = F.F64 if args.precision == "f64" else F.F32
+    rng = np.random.default_rng(5)
+    ns = 2 * rng.integers(1 << 11, (1 << 15) + 1, size=args.batch)
+    gain = np.exp(ALPHA * SPAN_LEN)
+    links = [F.make_link(fiber(F), F.Grid(F.UNIFORM, 16, 0.0, 0.0), F.BACKWARD, 2, int(n), 1 / (2 * BAUD),
+                         center_frequency=F_C, pre_scale=1 / np.sqrt(gain)) for n in ns]
+    dt = np.complex128 if prec == F.F64 else np.complex64
+    host = ((rng.standard_normal(ns.sum()) + 1j * rng.standard_normal(ns.sum())) * 0.03).astype(dt)
+    offs = np.cumsum([0] + list(ns[:-1])).astype(np.int64)
+    with F.Context(0) as ctx:
+        buf = ctx.alloc(host.nbytes)
+        for rep in range(args.reps):
+            buf.upload(host)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            steps = ctx.link_run(links, buf, offs, precision=prec)
+            ctx.synchronize()
+            t = time.perf_counter() - t0
+            print(f"probe generic {args.precision} batch {args.batch} rep {rep}: {t * 1e3:.1f} ms, "
+                  f"{float(np.sum(ns) * 32) / t / 1e9:.2f} GSample-steps/s, split steps {int(steps[0])}", flush=True)
+        buf.free()
     return 0
 
 
