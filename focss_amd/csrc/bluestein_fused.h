@@ -18,10 +18,13 @@
 // Between launches only the work rows (modes x M values per waveform) move
 // through HBM: about 10 M values per split step, against 2 N for the
 // power-of-two two-pass programs.  The field is read at entry and written
-// once, after the program's last time op.
+// once, after the program's last time op.  Both kernels use the split-stage
+// DFTs of the SSFM passes (first and last radix-8 stage in registers, one
+// LDS round trip per middle stage).
 #pragma once
 #include "bluestein_kernels.h"
 #include "fft_lds.h"
+#include "ssfm_kernels.h"
 
 namespace focss_hip {
 
@@ -39,32 +42,39 @@ struct BluArgs {
     int tiles;             // workgroups per waveform of this launch
 };
 
-// the split of M: N1 = 2^floor((log_m - 1) / 2) (N1 <= N2, columns of at
-// least 4 values per workgroup at every length)
-constexpr int blu_log_n1(int log_m) { return (log_m - 1) / 2; }
-template <typename R>
-constexpr int blu_tile() { return sizeof(R) == 8 ? 4096 : 8192; }  // values per workgroup (64 KiB)
+// the split of M: N1 = 2^max(6, floor((log_m - 1) / 2)) (N1 <= N2, both at
+// least 64 for the split-stage DFTs)
+constexpr int blu_log_n1(int log_m) { return (log_m - 1) / 2 < 6 ? 6 : (log_m - 1) / 2; }
 template <typename R, int LOGM>
 struct BluShape {
     static constexpr int N1 = 1 << blu_log_n1(LOGM);
     static constexpr int N2 = (1 << LOGM) / N1;
-    static constexpr int S = (blu_tile<R>() / N2 < N1) ? blu_tile<R>() / N2 : N1;  // rows per row-kernel tile
+    // values per workgroup: rows 1024 (complex128) / 2048 (complex64), columns 2048
+    static constexpr int ROW_ELEMS = sizeof(R) == 8 ? 1024 : 2048;
+    static constexpr int S = ROW_ELEMS / N2 < 1 ? 1 : (ROW_ELEMS / N2 > N1 ? N1 : ROW_ELEMS / N2);
     template <int MODES>
-    static constexpr int C = (blu_tile<R>() / (N1 * MODES) < N2) ? blu_tile<R>() / (N1 * MODES) : N2;
+    static constexpr int C = 2048 / (N1 * MODES) < 1 ? 1 : (2048 / (N1 * MODES) > N2 ? N2 : 2048 / (N1 * MODES));
 };
 
-// forward row DFT, x Bhat (row order), inverse row DFT, x W_M^{-n2 k1}, on S
-// rows of one mode of one waveform; after_mid: the second convolution of
-// the spectral op (runs when the op is active), else the first (runs while
-// the program has not ended)
+// S rows (k1 = k1_0 + s) of one mode of one waveform, split-stage as the
+// spectral pass: forward row DFT N2 -> x Bhat (row order) -> inverse row
+// DFT N2 -> x W_M^{-n2 k1}.  after_mid: the second convolution of the
+// spectral op (runs when the op is active), else the first (runs while the
+// program has not ended).  A work item (s, j < N2/8) owns the values
+// j + r N2/8 of row s.
 template <typename R, int LOGM>
-__global__ __launch_bounds__(NT) void k_blu_rows(BluArgs<R> g, int after_mid, Tables<R> tb) {
+__global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::S * BluShape<R, LOGM>::N2 / 8)) void k_blu_rows(
+    BluArgs<R> g, int after_mid, Tables<R> tb) {
     using Sh = BluShape<R, LOGM>;
     constexpr int N1 = Sh::N1, N2 = Sh::N2, S = Sh::S, M = 1 << LOGM;
-    constexpr int LB = (ilog2<N2>::value + 1) / 2;
+    constexpr int Q = N2 / 8;
+    constexpr int ITEMS = S * Q;
+    constexpr int TN = pass_threads(ITEMS);
+    constexpr int PER = (ITEMS + TN - 1) / TN;
+    using TW = PassTw<R, N2>;
     __shared__ cx<R> buf[padded(N2 * S)];
-    __shared__ cx<R> tw_lo[1 << LB];
-    __shared__ cx<R> tw_hi[N2 >> LB];
+    __shared__ cx<R> tw_a[TW::A];
+    __shared__ cx<R> tw_b[TW::B];
     const int k = blockIdx.x / g.tiles, t = blockIdx.x % g.tiles;
     const int m = t / (N1 / S);
     const int k1_0 = (t % (N1 / S)) * S;
@@ -78,35 +88,73 @@ __global__ __launch_bounds__(NT) void k_blu_rows(BluArgs<R> g, int after_mid, Ta
     const int modes = g.p.statics[w_id].modes;
     cx<R>* rows = g.work + ((long long)k * modes + m) * M + (long long)k1_0 * N2;
     const cx<R>* bh = gw.bhat_rows + (long long)k1_0 * N2;
-    load_tw_split<R, N2, LB, NT>(tw_lo, tw_hi, tb.bank);
-    const TwSplit<R, LB> tw{tw_lo, tw_hi};
-    for (int i = threadIdx.x; i < N2 * S; i += NT) buf[pad((i % N2) * S + i / N2)] = rows[i];
-    __syncthreads();
-    fft_lds<R, N2, S, NT, -1>(buf, tw);
-    for (int i = threadIdx.x; i < N2 * S; i += NT) {
-        const int p = pad((i % N2) * S + i / N2);
-        buf[p] = buf[p] * bh[i];
+    const auto tw = TW::template load<TN>(tw_a, tw_b, tb.bank);
+    cx<R> v[PER][8];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int s = id % S, j = id / S;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[b][r] = rows[s * N2 + j + r * Q];
+            first_stage_store<R, N2, S, -1>(buf, v[b], s, j);
+        }
     }
     __syncthreads();
-    fft_lds<R, N2, S, NT, +1>(buf, tw);
-    for (int i = threadIdx.x; i < N2 * S; i += NT) {
-        const int s = i / N2, n2 = i % N2;
-        const cx<R> w = tw_big<R>(tb, (unsigned)(n2 * (k1_0 + s)) & (unsigned)(M - 1), LOGM);
-        rows[i] = buf[pad(n2 * S + s)] * conj(w);
+    middle_stages<R, N2, S, TN, -1, false>(buf, tw);
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int s = id % S, j = id / S;
+            last_stage_load<R, N2, S, -1>(buf, tw, v[b], s, j);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[b][r] = v[b][r] * bh[s * N2 + j + r * Q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) first_stage_store<R, N2, S, +1>(buf, v[b], id % S, id / S);
+    }
+    __syncthreads();
+    middle_stages<R, N2, S, TN, +1, true>(buf, tw);
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int s = id % S, j = id / S;
+            last_stage_load<R, N2, S, +1>(buf, tw, v[b], s, j);
+            cx<R> itw[8];  // W_M^{k1 (j + r Q)}
+            inter_twiddles<R>(tb, (unsigned)(k1_0 + s), (unsigned)j, (unsigned)Q, LOGM, itw);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) rows[s * N2 + j + r * Q] = v[b][r] * conj(itw[r]);
+        }
     }
 }
 
-// C columns (all modes) of one waveform: see the header comment for KIND
+// C columns (all modes) of one waveform, split-stage as the time pass:
+// inverse column DFT N1 (not at ENTRY) -> the pointwise operator of KIND on
+// the time-domain values e = n1 N2 + n2 (see the header comment; zero
+// beyond N) -> forward column DFT N1 -> x W_M^{n2 k1}; after the program's
+// last op the field is written instead.  A work item (c, j < N1/8) owns the
+// values j + r N1/8 of column c of every mode.
 template <typename R, int LOGM, int MODES, int KIND>
-__global__ __launch_bounds__(NT) void k_blu_cols(BluArgs<R> g, Tables<R> tb) {
+__global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::template C<MODES> * BluShape<R, LOGM>::N1 / 8)) void
+k_blu_cols(BluArgs<R> g, Tables<R> tb) {
     using Sh = BluShape<R, LOGM>;
     constexpr int N1 = Sh::N1, N2 = Sh::N2, C = Sh::template C<MODES>, M = 1 << LOGM;
     constexpr int SEQ = C * MODES;  // sequence (m, c) at LDS column m C + c
-    constexpr int LB = (ilog2<N1>::value + 1) / 2;
+    constexpr int Q = N1 / 8;
+    constexpr int ITEMS = C * Q;
+    constexpr int TN = pass_threads(ITEMS);
+    constexpr int PER = (ITEMS + TN - 1) / TN;
+    using TW = PassTw<R, N1>;
     __shared__ cx<R> buf[padded(N1 * SEQ)];
-    __shared__ cx<R> tw_lo[1 << LB];
-    __shared__ cx<R> tw_hi[N1 >> LB];
-    __shared__ double redbuf[2 * (NT / 64)];
+    __shared__ cx<R> tw_a[TW::A];
+    __shared__ cx<R> tw_b[TW::B];
+    __shared__ double redbuf[2 * (TN / 64)];
     const PassArgs& a = g.p;
     const int k = blockIdx.x / g.tiles, t = blockIdx.x % g.tiles;
     const int n2_0 = t * C;
@@ -142,77 +190,142 @@ __global__ __launch_bounds__(NT) void k_blu_cols(BluArgs<R> g, Tables<R> tb) {
     const int n = gw.n;
     cx<R>* rows = g.work + (long long)k * MODES * M;
     cx<R>* x = reinterpret_cast<cx<R>*>(a.field) + w.offset;
-    load_tw_split<R, N1, LB, NT>(tw_lo, tw_hi, tb.bank);
-    const TwSplit<R, LB> tw{tw_lo, tw_hi};
+    const auto tw = TW::template load<TN>(tw_a, tw_b, tb.bank);
+    cx<R> v[PER][MODES][8];
     if (KIND == BLU_ENTRY) {
-        for (int i = threadIdx.x; i < N1 * SEQ; i += NT) {
-            const int n1 = i / SEQ, q = i % SEQ, mm = q / C, e = n1 * N2 + n2_0 + q % C;
-            buf[pad(i)] = e < n ? x[(long long)mm * n + e] : mk<R>(0, 0);
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+#pragma unroll
+                for (int mm = 0; mm < MODES; ++mm)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        const int e = (j + r * Q) * N2 + n2_0 + c;
+                        v[b][mm][r] = e < n ? x[(long long)mm * n + e] : mk<R>(0, 0);
+                    }
+            }
         }
     } else {
-        for (int i = threadIdx.x; i < N1 * SEQ; i += NT) {
-            const int k1 = i / SEQ, q = i % SEQ, mm = q / C;
-            buf[pad(i)] = rows[(long long)mm * M + (long long)k1 * N2 + n2_0 + q % C];
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+#pragma unroll
+                for (int mm = 0; mm < MODES; ++mm) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) v[b][mm][r] = rows[(long long)mm * M + (j + r * Q) * N2 + n2_0 + c];
+                    first_stage_store<R, N1, SEQ, +1>(buf, v[b][mm], mm * C + c, j);
+                }
+            }
         }
         __syncthreads();
-        fft_lds<R, N1, SEQ, NT, +1>(buf, tw);  // conv at n1 N2 + n2
+        middle_stages<R, N1, SEQ, TN, +1, false>(buf, tw);
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+#pragma unroll
+                for (int mm = 0; mm < MODES; ++mm) last_stage_load<R, N1, SEQ, +1>(buf, tw, v[b][mm], mm * C + c, j);
+            }
+        }
     }
-    __syncthreads();
     bool last = false;
     if (KIND == BLU_MID) {
         const double arg = 0.5 * w.beta2 * op.s;
         const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)n;
-        for (int i = threadIdx.x; i < N1 * SEQ; i += NT) {
-            const int e = (i / SEQ) * N2 + n2_0 + (i % SEQ) % C;
-            cx<R> v = mk<R>(0, 0);
-            if (e < n) {
-                const cx<R> c = gw.chirp[e];
-                v = conj((c * buf[pad(i)]) * spectral_factor<R>(w, op, e, arg, mag)) * c;
+#pragma unroll
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const int e = (j + r * Q) * N2 + n2_0 + c;
+                    if (e < n) {
+                        const cx<R> ce = gw.chirp[e];
+                        const cx<R> f = spectral_factor<R>(w, op, e, arg, mag);
+#pragma unroll
+                        for (int mm = 0; mm < MODES; ++mm) v[b][mm][r] = conj((ce * v[b][mm][r]) * f) * ce;
+                    } else {
+#pragma unroll
+                        for (int mm = 0; mm < MODES; ++mm) v[b][mm][r] = mk<R>(0, 0);
+                    }
+                }
             }
-            buf[pad(i)] = v;
         }
     } else {
-        // time-domain values of all modes of one sample per thread
+        // the time-domain op on all modes of one sample at a time
         last = st.phase == PH_DONE;  // this op is the program's last
         double pk = 0, sm = 0;
-        for (int i = threadIdx.x; i < N1 * C; i += NT) {
-            const int n1 = i / C, cc = i % C, e = n1 * N2 + n2_0 + cc;
-            cx<R> u[MODES];
-            cx<R> c = mk<R>(0, 0);
-            if (e < n) {
-                c = gw.chirp[e];
 #pragma unroll
-                for (int mm = 0; mm < MODES; ++mm) {
-                    const cx<R> v = buf[pad(n1 * SEQ + mm * C + cc)];
-                    u[mm] = KIND == BLU_ENTRY ? v : conj(c * v);
-                }
-                time_op<R, MODES>(u, w, st, e, pk, sm);
-            }
+        for (int b = 0; b < PER; ++b) {
+            const int id = threadIdx.x + b * TN;
+            if (ITEMS % TN == 0 || id < ITEMS) {
+                const int c = id % C, j = id / C;
 #pragma unroll
-            for (int mm = 0; mm < MODES; ++mm) {
-                if (last) {
-                    if (e < n) x[(long long)mm * n + e] = u[mm];
-                } else {
-                    buf[pad(n1 * SEQ + mm * C + cc)] = e < n ? u[mm] * c : mk<R>(0, 0);
+                for (int r = 0; r < 8; ++r) {
+                    const int e = (j + r * Q) * N2 + n2_0 + c;
+                    if (e < n) {
+                        const cx<R> ce = gw.chirp[e];
+                        cx<R> u[MODES];
+#pragma unroll
+                        for (int mm = 0; mm < MODES; ++mm)
+                            u[mm] = KIND == BLU_ENTRY ? v[b][mm][r] : conj(ce * v[b][mm][r]);
+                        time_op<R, MODES>(u, w, st, e, pk, sm);
+#pragma unroll
+                        for (int mm = 0; mm < MODES; ++mm) {
+                            if (last) x[(long long)mm * n + e] = u[mm];
+                            v[b][mm][r] = u[mm] * ce;
+                        }
+                    } else {
+#pragma unroll
+                        for (int mm = 0; mm < MODES; ++mm) v[b][mm][r] = mk<R>(0, 0);
+                    }
                 }
             }
         }
-        block_max_sum<NT>(pk, sm, redbuf);
+        // the reduction's barriers also order the last-stage LDS reads above
+        // before the first-stage writes below
+        block_max_sum<TN>(pk, sm, redbuf);
         if (threadIdx.x == 0) {
-            Reduction* r = a.red + (a.counter % 3) * a.n_waves + w_id;
-            atomicMax(&r->peak_bits, double_to_bits(pk));
-            atomicAdd(&r->sum, sm);
+            Reduction* rd = a.red + (a.counter % 3) * a.n_waves + w_id;
+            atomicMax(&rd->peak_bits, double_to_bits(pk));
+            atomicAdd(&rd->sum, sm);
             if (w.grid == GRID_LOGARITHMIC)
                 a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + t] = sm;
         }
         if (last) return;
     }
+    if (KIND == BLU_MID) __syncthreads();  // the last-stage LDS reads before the first-stage writes
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int c = id % C, j = id / C;
+#pragma unroll
+            for (int mm = 0; mm < MODES; ++mm) first_stage_store<R, N1, SEQ, -1>(buf, v[b][mm], mm * C + c, j);
+        }
+    }
     __syncthreads();
-    fft_lds<R, N1, SEQ, NT, -1>(buf, tw);
-    for (int i = threadIdx.x; i < N1 * SEQ; i += NT) {
-        const int k1 = i / SEQ, q = i % SEQ, mm = q / C, n2 = n2_0 + q % C;
-        const cx<R> wv = tw_big<R>(tb, (unsigned)(n2 * k1) & (unsigned)(M - 1), LOGM);
-        rows[(long long)mm * M + (long long)k1 * N2 + n2] = buf[pad(i)] * wv;
+    middle_stages<R, N1, SEQ, TN, -1, true>(buf, tw);
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int id = threadIdx.x + b * TN;
+        if (ITEMS % TN == 0 || id < ITEMS) {
+            const int c = id % C, j = id / C;
+            cx<R> itw[8];  // W_M^{n2 (j + r Q)}
+            inter_twiddles<R>(tb, (unsigned)(n2_0 + c), (unsigned)j, (unsigned)Q, LOGM, itw);
+#pragma unroll
+            for (int mm = 0; mm < MODES; ++mm) {
+                last_stage_load<R, N1, SEQ, -1>(buf, tw, v[b][mm], mm * C + c, j);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) rows[(long long)mm * M + (j + r * Q) * N2 + n2_0 + c] = v[b][mm][r] * itw[r];
+            }
+        }
     }
 }
 

@@ -603,13 +603,13 @@ void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const 
         const BluRegistry<R>& r = blu_reg<R>();
         BluArgs<R> g = blu_args<R>(c, gc, a);
         g.tiles = gc.modes * r.row_tiles[gc.log_m];
-        const dim3 rows_grid(gc.count * g.tiles);
-        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, dim3(NT), 0, c->stream, g, 0, tb);
+        const dim3 rows_grid(gc.count * g.tiles), rows_block(r.row_threads[gc.log_m]);
+        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 0, tb);
         BluArgs<R> gm = g;
         gm.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
-        hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][BLU_MID], dim3(gc.count * gm.tiles), dim3(NT), 0, c->stream,
-                           gm, tb);
-        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, dim3(NT), 0, c->stream, g, 1, tb);
+        hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][BLU_MID], dim3(gc.count * gm.tiles),
+                           dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, c->stream, gm, tb);
+        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 1, tb);
         return;
     }
     const GenArgs<R> g = gen_args<R>(c, gc, a);
@@ -634,7 +634,7 @@ void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a, const 
         BluArgs<R> g = blu_args<R>(c, gc, a);
         g.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
         hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][entry ? BLU_ENTRY : BLU_OUT], dim3(gc.count * g.tiles),
-                           dim3(NT), 0, c->stream, g, tb);
+                           dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, c->stream, g, tb);
         return;
     }
     GenArgs<R> g = gen_args<R>(c, gc, a);

@@ -136,6 +136,8 @@ struct BluRegistry {
     void (*cols[21][2][3])(BluArgs<R>, Tables<R>) = {};
     int row_tiles[21] = {};     // row-kernel workgroups per mode (N1 / S)
     int col_tiles[21][2] = {};  // column-kernel workgroups per waveform (N2 / C)
+    int row_threads[21] = {};   // workgroup sizes the kernels are built for
+    int col_threads[21][2] = {};
 };
 
 template <typename R, int LOGM, int MODES>
@@ -145,6 +147,7 @@ void reg_blu_modes(BluRegistry<R>& r) {
     r.cols[LOGM][MODES - 1][BLU_MID] = &k_blu_cols<R, LOGM, MODES, BLU_MID>;
     r.cols[LOGM][MODES - 1][BLU_OUT] = &k_blu_cols<R, LOGM, MODES, BLU_OUT>;
     r.col_tiles[LOGM][MODES - 1] = Sh::N2 / Sh::template C<MODES>;
+    r.col_threads[LOGM][MODES - 1] = pass_threads(Sh::template C<MODES> * Sh::N1 / 8);
 }
 
 template <typename R, int LOGM>
@@ -154,6 +157,7 @@ void reg_blu(BluRegistry<R>& r) {
                   "Bluestein tile shape");
     r.rows[LOGM] = &k_blu_rows<R, LOGM>;
     r.row_tiles[LOGM] = Sh::N1 / Sh::S;
+    r.row_threads[LOGM] = pass_threads(Sh::S * Sh::N2 / 8);
     reg_blu_modes<R, LOGM, 1>(r);
     reg_blu_modes<R, LOGM, 2>(r);
 }
