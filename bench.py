@@ -196,29 +196,41 @@ def run_c5(F, ctx, batch, log):
     return out
 
 
-def f32_error_per_span(F, ctx, field, dt, grid):
+def f32_error_per_span(F, ctx, field, dt, grid, replay=False):
     """Relative L2 error of the FP32 path against the FP64 path after each
     span, one realisation carried span by span through both precisions with
     identical ASE (seed per span).  The FP64 device path is the reference:
     tests/test_ssfm_twopass_gpu.py holds it to the CPU oracle within 1e-10
-    on the first span of this field at this size."""
+    on the first span of this field at this size.  With replay, the FP32 run
+    takes each span's ADAPTIVE steps from the FP64 run's recorded schedule,
+    so the difference is the arithmetic alone."""
     gain = 10 ** (ALPHA_DB_KM * SPAN_LEN * 1e-3 / 10)
     sigma = np.sqrt(PLANCK * F_C * 10 ** (NF_DB / 10) * (gain - 1) / 2 / dt / 2)
     b64, b32 = ctx.alloc(field.size * 16), ctx.alloc(field.size * 8)
     b64.upload(field.astype(np.complex128))
     b32.upload(field.astype(np.complex64))
     off = np.zeros(1, dtype=np.int64)
+    cap = 1 << 16
+    sched = ctx.alloc(cap * 8) if replay else None
     errs = []
     for s in range(SPANS):
         link = F.make_link(F.Fiber(**fiber_dict()), grid, F.FORWARD, 1, N_SAMPLES, dt, center_frequency=F_C,
                            post_scale=np.sqrt(gain), noise_sigma=sigma, noise_seed=SEED + 1000 + s)
-        ctx.link_run([link], b64, off, precision=F.F64)
-        ctx.link_run([link], b32, off, precision=F.F32)
+        if replay:
+            s64 = ctx.link_run_schedule([link], b64, off, F.SCHEDULE_RECORD, sched, cap, precision=F.F64)
+            s32 = ctx.link_run_schedule([link], b32, off, F.SCHEDULE_REPLAY, sched, cap, precision=F.F32)
+            if s64[0] != s32[0]:
+                raise RuntimeError(f"replayed schedule: {s32[0]} against {s64[0]} split steps")
+        else:
+            ctx.link_run([link], b64, off, precision=F.F64)
+            ctx.link_run([link], b32, off, precision=F.F32)
         y64 = b64.download(np.empty(field.size, dtype=np.complex128))
         y32 = b32.download(np.empty(field.size, dtype=np.complex64)).astype(np.complex128)
         errs.append(float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64)))
     b64.free()
     b32.free()
+    if sched is not None:
+        sched.free()
     return errs
 
 
@@ -353,6 +365,9 @@ def main():
             c5 = run_c5(F, ctx, args.c5_batch, log)
         if args.precision == "both":
             span_err = {"adaptive": f32_error_per_span(F, ctx, field, dt, F.Grid(F.ADAPTIVE, 0, PHI_MAX, H_MAX)),
+                        "adaptive_fp64_schedule": f32_error_per_span(F, ctx, field, dt,
+                                                                     F.Grid(F.ADAPTIVE, 0, PHI_MAX, H_MAX),
+                                                                     replay=True),
                         "uniform_100_steps_per_span": f32_error_per_span(F, ctx, field, dt,
                                                                          F.Grid(F.UNIFORM, 100, 0.0, 0.0))}
     head = res.get("f64") or res.get("f32")
@@ -404,8 +419,8 @@ def main():
                        "at_other_batch": big.get("f32")}
         if span_err:
             # the ADAPTIVE schedule follows each precision's own peak power, so
-            # the two runs take different steps; the UNIFORM run isolates the
-            # arithmetic
+            # the two runs take different steps; the replayed FP64 schedule and
+            # the UNIFORM run isolate the arithmetic
             line["f32"]["rel_l2_vs_f64_per_span"] = {k: [float(f"{e:.3g}") for e in v]
                                                      for k, v in span_err.items()}
     if c5:

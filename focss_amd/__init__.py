@@ -58,6 +58,7 @@ _sig = {
     "focss_synchronize": (C.c_int, [_P]),
     "focss_link_run": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, _P]),
     "focss_link_run_host": (C.c_int, [_P, C.c_int, C.POINTER(Link), _P, _P]),
+    "focss_link_run_schedule": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Link), _P, _P, _P, C.c_int, _P, C.c_int]),
     "focss_fft": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "focss_fft_noalloc": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "focss_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
@@ -209,6 +210,19 @@ class Context:
         steps = np.zeros(n, dtype=np.int32)
         _check(_lib.focss_link_run(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
                                    steps.ctypes.data))
+        return steps
+
+    def link_run_schedule(self, links, batch, offsets, mode, schedule, capacity, precision=F64):
+        """link_run recording (SCHEDULE_RECORD) or replaying (SCHEDULE_REPLAY)
+        the ADAPTIVE step schedule in `schedule`, a device buffer of
+        len(links) x capacity doubles."""
+        n = len(links)
+        _need(schedule, n * capacity * 8, "link_run_schedule")
+        arr = (Link * n)(*links)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        steps = np.zeros(n, dtype=np.int32)
+        _check(_lib.focss_link_run_schedule(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
+                                            steps.ctypes.data, mode, schedule.ptr, capacity))
         return steps
 
     def link_profile(self, links, batch, offsets, n_pairs, precision=F64):
@@ -595,6 +609,7 @@ def validate(link):
 
 
 FAULT_ALLOC = 1
+SCHEDULE_RECORD, SCHEDULE_REPLAY = 1, 2
 
 
 def inject_fault(kind, count):

@@ -174,6 +174,7 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
         }
         op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
         if (t == 0 && threadIdx.x == 0) {
+            record_step(w, st);
             a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
             Reduction z;
             z.peak_bits = 0;

@@ -109,6 +109,7 @@ __global__ __launch_bounds__(256) void k_gen_mid(GenArgs<R> g) {
     }
     SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
     if (blockIdx.x % (g.chunks_per_row * g.modes) == 0 && threadIdx.x == 0) {
+        record_step(w, st);
         a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
         Reduction z;
         z.peak_bits = 0;

@@ -360,6 +360,9 @@ int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
     w.modes = L.modes;
     w.samples = L.samples;
     w.linear_only = (g == FOCSS_UNIFORM && w.kgamma == 0) ? 1 : 0;
+    w.sched_mode = 0;
+    w.sched_cap = 0;
+    w.sched = nullptr;
     return 0;
 }
 
@@ -671,14 +674,26 @@ int prepare_generic(focss_ctx* c, int n, const focss_link* links, const std::vec
     return 0;
 }
 
+// an ADAPTIVE step schedule to record or replay: sched[i * cap ..] for waveform i
+struct Schedule {
+    int mode = 0;  // 0 none, FOCSS_SCHEDULE_RECORD, FOCSS_SCHEDULE_REPLAY
+    int cap = 0;
+    double* sched = nullptr;
+};
+
 template <typename R>
 int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void* batch,
               const int64_t* offsets, int* total_steps, int profile_pairs = 0, double* prof_ms = nullptr,
-              int graph_replays = 0) {
+              int graph_replays = 0, Schedule schedule = Schedule()) {
     std::vector<WaveStatic> st(n);
     for (int i = 0; i < n; ++i) {
         int rc = fill_static(links[i], offsets[i], st[i]);
         if (rc) return rc;
+        if (schedule.mode) {
+            st[i].sched_mode = schedule.mode;
+            st[i].sched_cap = schedule.cap;
+            st[i].sched = schedule.sched + (size_t)i * schedule.cap;
+        }
     }
     if (int rc = ensure_capacity(c, n)) return rc;
     // group by length class; lengths without a kernel of their own (not a
@@ -1550,6 +1565,23 @@ int focss_link_run(focss_ctx* c, int precision, int n_waves, const focss_link* l
     HIP_TRY(hipSetDevice(c->device));
     if (precision == FOCSS_F64) return run_links<double>(c, precision, n_waves, links, batch, offsets, total_steps);
     return run_links<float>(c, precision, n_waves, links, batch, offsets, total_steps);
+}
+
+int focss_link_run_schedule(focss_ctx* c, int precision, int n_waves, const focss_link* links, void* batch,
+                            const int64_t* offsets, int* total_steps, int mode, double* schedule, int capacity) {
+    if (!c || !links || !batch || !offsets || n_waves < 1 || !schedule) return fail(FOCSS_E_ARG, "null argument");
+    if (precision != FOCSS_F64 && precision != FOCSS_F32) return fail(FOCSS_E_ARG, "bad precision");
+    if (mode != FOCSS_SCHEDULE_RECORD && mode != FOCSS_SCHEDULE_REPLAY)
+        return fail(FOCSS_E_ARG, "schedule mode must be FOCSS_SCHEDULE_RECORD or FOCSS_SCHEDULE_REPLAY");
+    if (capacity < 1) return fail(FOCSS_E_ARG, "schedule capacity must be >= 1");
+    HIP_TRY(hipSetDevice(c->device));
+    Schedule sc;
+    sc.mode = mode;
+    sc.cap = capacity;
+    sc.sched = schedule;
+    if (precision == FOCSS_F64)
+        return run_links<double>(c, precision, n_waves, links, batch, offsets, total_steps, 0, nullptr, 0, sc);
+    return run_links<float>(c, precision, n_waves, links, batch, offsets, total_steps, 0, nullptr, 0, sc);
 }
 
 int focss_link_run_host(focss_ctx* c, int precision, const focss_link* link, double* field, int* total_steps) {

@@ -50,6 +50,11 @@ struct WaveStatic {
     int grid, steps;     // steps = UNIFORM total_steps
     int n_spans, modes, samples;
     int linear_only;     // UNIFORM with kappa*gamma == 0: one multiply per span
+    // ADAPTIVE step schedule: 1 = record each step taken into sched (its
+    // length, negated for a span's last step), 2 = take the steps from sched
+    // instead of estimating them; sched_cap entries
+    int sched_mode, sched_cap;
+    double* sched;
 };
 
 struct WaveState {
@@ -132,6 +137,36 @@ FOCSS_HD inline void take_step(const WaveStatic& w, WaveState& st, SpectralOp& o
     st.phase = is_final ? PH_FINAL_DONE : PH_IN_SPAN;
 }
 
+// One ADAPTIVE step (ssfm.cpp:92-118): h from the peak power, or the span's
+// remainder when h would reach its end; with a replayed schedule, the
+// recorded step instead (its sign marks the span's last step)
+FOCSS_HD inline void adaptive_step(const WaveStatic& w, WaveState& st, SpectralOp& op, double peak) {
+    if (w.sched_mode == 2 && st.total_steps < w.sched_cap) {
+        const double v = w.sched[st.total_steps];
+        const bool last = v < 0;
+        const double h = last ? -v : v;
+        take_step(w, st, op, h, w.dir * h, last);
+        if (!last) st.d += h;
+        return;
+    }
+    double h = adaptive_h(w, peak);
+    if (st.d + h < w.length) {
+        take_step(w, st, op, h, w.dir * h, false);
+        st.d += h;
+    } else {
+        double hf = w.length - st.d;
+        take_step(w, st, op, hf, w.dir * hf, true);
+    }
+}
+
+// After a plan: the step just taken into the recorded schedule (one writer
+// per waveform calls this)
+FOCSS_HD inline void record_step(const WaveStatic& w, const WaveState& st) {
+    if (w.sched_mode != 1 || w.grid != GRID_ADAPTIVE || st.top != TOP_NL) return;
+    const int i = st.total_steps - 1;
+    if (i >= 0 && i < w.sched_cap) w.sched[i] = st.phase == PH_FINAL_DONE ? -st.h : st.h;
+}
+
 // Plan the next spectral multiply and the time-domain op after it.
 // red = reductions of the previous time-domain op.  Pure function of its
 // inputs, so every workgroup of a pass computes the same plan.
@@ -179,14 +214,7 @@ FOCSS_HD inline SpectralOp plan_spectral(const WaveStatic& w, WaveState& st, dou
                 double step = inv_alpha * FOCSS_LOG(1 + inv_gain / (1 + inv_gain * 0));
                 take_step(w, st, op, step * w.dir, step, st.n_log == 1);
             } else {
-                double h = adaptive_h(w, peak);
-                if (st.d + h < w.length) {
-                    take_step(w, st, op, h, w.dir * h, false);
-                    st.d += h;
-                } else {
-                    double hf = w.length - st.d;
-                    take_step(w, st, op, hf, w.dir * hf, true);
-                }
+                adaptive_step(w, st, op, peak);
             }
             return op;
         }
@@ -201,14 +229,7 @@ FOCSS_HD inline SpectralOp plan_spectral(const WaveStatic& w, WaveState& st, dou
                 take_step(w, st, op, step * w.dir, step, st.step + 1 == st.n_log);
             } else {
                 // next_step was estimated after the previous Kerr op (ssfm.cpp:102-108)
-                double h = adaptive_h(w, peak);
-                if (st.d + h < w.length) {
-                    take_step(w, st, op, h, w.dir * h, false);
-                    st.d += h;
-                } else {
-                    double hf = w.length - st.d;
-                    take_step(w, st, op, hf, w.dir * hf, true);
-                }
+                adaptive_step(w, st, op, peak);
             }
             return op;
         }

@@ -178,6 +178,7 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
     fft_lds<R, N, M, NT, -1>(buf, tw);
     for (;;) {
         SpectralOp op = plan_spectral(w, st, pk, sm);
+        if (threadIdx.x == 0) record_step(w, st);
         if (!op.active) break;
         const double arg = 0.5 * w.beta2 * op.s;
         const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
@@ -508,6 +509,7 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
     }
     SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
     if (tile0 == 0 && threadIdx.x == 0) {
+        record_step(w, st);
         a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
         Reduction z;
         z.peak_bits = 0;

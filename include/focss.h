@@ -74,7 +74,7 @@ typedef struct {
     double direction;        /* +1 FORWARD, -1 BACKWARD (ssfm.cpp:13-14) */
     int n_spans;
     int modes;               /* 1 or 2 (Manakov) */
-    int samples;             /* power of two */
+    int samples;             /* any length in [2, 2^19], or a power of two up to 2^20 */
     int pad;
     double dt;               /* sample spacing [s] (Domain::time_step) */
     double center_frequency; /* [Hz] (Domain::center_frequency) */
@@ -119,6 +119,20 @@ int focss_synchronize(focss_ctx* ctx);
  * (src/tasks/systems.cpp:26-91, src/tasks/wdm_run.cpp:26-47). */
 int focss_link_run(focss_ctx* ctx, int precision, int n_waves, const focss_link* links,
                    void* batch, const int64_t* offsets, int* total_steps);
+
+/* focss_link_run with the ADAPTIVE grid's step schedule recorded or
+ * replayed, for precision studies (no counterpart in the reference):
+ * schedule is a device array of n_waves x capacity doubles, waveform i's at
+ * schedule + i * capacity.  FOCSS_SCHEDULE_RECORD writes each ADAPTIVE step
+ * taken (its length, negated for a span's last step; steps beyond capacity
+ * are not recorded); FOCSS_SCHEDULE_REPLAY takes those steps instead of
+ * estimating them from the field's peak power (ssfm.cpp:92-118), so an FP32
+ * run can follow the FP64 run's schedule exactly, and continues with its
+ * own estimates past capacity.  Other grids are unaffected. */
+#define FOCSS_SCHEDULE_RECORD 1
+#define FOCSS_SCHEDULE_REPLAY 2
+int focss_link_run_schedule(focss_ctx* ctx, int precision, int n_waves, const focss_link* links, void* batch,
+                            const int64_t* offsets, int* total_steps, int mode, double* schedule, int capacity);
 
 /* Measurement: runs only the first n_pairs split steps of the programs, one
  * kernel launch at a time, each bracketed by HIP events on the context's
