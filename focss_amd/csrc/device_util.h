@@ -104,13 +104,16 @@ __device__ __forceinline__ cx<float> phase_factor<float>(double phase, double ma
 }
 
 // Kerr rotation exp(i coef P), computed in the field's precision.  When every
-// lane of the wave has |phase| <= 1/16 (always so on an adaptive grid, whose
-// steps bound the phase by phi_max) the short series replaces the reduced
-// one; the choice is uniform over the wave.
+// lane of the wave has |phase| <= 2^-8 (always so on an adaptive grid, whose
+// steps bound the phase by phi_max) the three-term series replaces the
+// reduced sin/cos, within 1/16 the short series; the choice is uniform over
+// the wave.
 template <typename R>
 __device__ __forceinline__ cx<R> kerr_factor(R ph) {
     R s, c;
-    if (__all(fabs(ph) <= (R)0.0625))
+    if (__all(fabs(ph) <= (R)0.00390625))
+        sincos_tiny(ph, &s, &c);
+    else if (__all(fabs(ph) <= (R)0.0625))
         sincos_small(ph, &s, &c);
     else
         sincos_fast(ph, &s, &c);

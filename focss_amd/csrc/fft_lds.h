@@ -233,6 +233,51 @@ struct PassTw {
             return type{a};
         }
     }
+    // The same in two halves: fetch() issues the global loads into
+    // registers, store() writes them to LDS.  A kernel issues its field
+    // loads between the two, so that the table's latency and the field's
+    // overlap instead of adding up.
+    template <int NT>
+    struct Staged {
+        static constexpr int PA = (A + NT - 1) / NT, PB = (B + NT - 1) / NT;
+        cx<R> a[PA], b[PB];
+    };
+    template <int NT>
+    static __device__ __forceinline__ Staged<NT> fetch(const cx<R>* __restrict__ bank) {
+        const cx<R>* src = bank + tw_offset(ilog2<L>::value);
+        Staged<NT> s;
+#pragma unroll
+        for (int k = 0; k < Staged<NT>::PA; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (A % NT == 0 || i < A) s.a[k] = src[i];
+        }
+        if constexpr (kSplit) {
+#pragma unroll
+            for (int k = 0; k < Staged<NT>::PB; ++k) {
+                const int i = threadIdx.x + k * NT;
+                if (B % NT == 0 || i < B) s.b[k] = src[i << LB];
+            }
+        }
+        return s;
+    }
+    template <int NT>
+    static __device__ __forceinline__ type store(cx<R>* a, cx<R>* b, const Staged<NT>& s) {
+#pragma unroll
+        for (int k = 0; k < Staged<NT>::PA; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (A % NT == 0 || i < A) a[i] = s.a[k];
+        }
+        if constexpr (kSplit) {
+#pragma unroll
+            for (int k = 0; k < Staged<NT>::PB; ++k) {
+                const int i = threadIdx.x + k * NT;
+                if (B % NT == 0 || i < B) b[i] = s.b[k];
+            }
+            return type{{a, b}};
+        } else {
+            return type{a};
+        }
+    }
 };
 
 // ------------------------------------------------------------------

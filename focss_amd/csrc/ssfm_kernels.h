@@ -439,7 +439,9 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
 // bins above N/2 are k - N; the Nyquist bin's sign does not matter in k^2.
 // FP64: exp(i theta k_s^2) by the recurrence f_{t+1} = f_t g_t,
 // g_{t+1} = g_t exp(2 i theta D^2) (3 sincos per 8 bins, error ~ 30 ulp).
-// FP32: each phase reduced mod 2 pi in FP64, then FP32 sin/cos.
+// FP32: each phase reduced mod 2 pi in FP64, then FP32 sin/cos (the FP64
+// recurrence rounded per bin needs 133 instead of 127 registers in the
+// two-tile FP32 spectral pass, one wave per SIMD fewer, and measured slower).
 template <typename R>
 __device__ __forceinline__ void spectral_multiply(cx<R>* v, const WaveStatic& w, int k1, int j, int N1, int Q,
                                                   double arg, double mag) {
@@ -507,21 +509,7 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
         sum = 0;
         for (int t = 0; t < a.time_tiles; ++t) sum += p[t];
     }
-    SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
-    if (tile0 == 0 && threadIdx.x == 0) {
-        record_step(w, st);
-        a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
-        Reduction z;
-        z.peak_bits = 0;
-        z.sum = 0;
-        a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
-        if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
-    }
-    if (!op.active) return;
-    const int N = w.samples;
-    const int N1 = N / N2;
     cx<R>* g0 = reinterpret_cast<cx<R>*>(a.field) + w.offset;
-    const auto tw_t = TW::template load<TN>(tw_a, tw_b, tb.bank);
     // offset of element j + r Q of row s of the tile: s N2 + j + r Q in
     // natural order, or Interleave(s, j) + 2 r Q (Q a multiple of W)
     auto row_at = [&](bool il, int s, int j) -> int {
@@ -541,13 +529,31 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
     };
     static_assert(TPB == 1 || TPB == 2, "one or two tiles per workgroup");
     cx<R> va[PER][8], vb[PER][8];
-    // one tile; v holds its values (tile 0 loads them here), vn receives the
-    // next tile's
+    // the stage-twiddle table's loads and the first tile's are issued before
+    // the step is planned, so that their latency overlaps the plan (a
+    // finished program's workgroups load nothing)
+    const auto tw_staged = TW::template fetch<TN>(tb.bank);
+    if (!was_done) load(tile0, va);
+    SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
+    if (tile0 == 0 && threadIdx.x == 0) {
+        record_step(w, st);
+        a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
+        Reduction z;
+        z.peak_bits = 0;
+        z.sum = 0;
+        a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
+        if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
+    }
+    if (!op.active) return;
+    const int N = w.samples;
+    const int N1 = N / N2;
+    const auto tw_t = TW::template store<TN>(tw_a, tw_b, tw_staged);
+    // one tile; v holds its values (loaded before the plan for tile 0), vn
+    // receives the next tile's
     auto run_tile = [&](auto tc, cx<R> (&v)[PER][8], cx<R> (&vn)[PER][8]) {
         constexpr int t = decltype(tc)::value;
         const int tile = tile0 + t;
         cx<R>* g = g0 + (long long)tile * S * N2;
-        if constexpr (t == 0) load(tile, v);
 #pragma unroll
         for (int b = 0; b < PER; ++b) {
             const int id = threadIdx.x + b * TN;

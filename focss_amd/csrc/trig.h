@@ -112,6 +112,21 @@ FOCSS_TRIG_HD inline void sincos_small(float x, float* s, float* c) {
     *c = fmaf(x2 * x2, fmaf(x2, -0.001388888888888889f, 0.041666666666666664f), fmaf(-0.5f, x2, 1.0f));
 }
 
+// |x| <= 2^-8 (every step of an adaptive grid: its phase is at most phi_max):
+// sin to x^5, cos to x^4; the first omitted terms are below 3e-21 and 5e-18
+FOCSS_TRIG_HD inline void sincos_tiny(double x, double* s, double* c) {
+    const double x2 = x * x;
+    *s = fma(x * x2, fma(x2, 0.008333333333333333, -0.16666666666666666), x);
+    *c = fma(x2, fma(x2, 0.041666666666666664, -0.5), 1.0);
+}
+
+// |x| <= 2^-8: sin to x^3, cos to x^2; omitted terms below 8e-15 and 1e-11
+FOCSS_TRIG_HD inline void sincos_tiny(float x, float* s, float* c) {
+    const float x2 = x * x;
+    *s = fmaf(x * x2, -0.16666666666666666f, x);
+    *c = fmaf(-0.5f, x2, 1.0f);
+}
+
 FOCSS_TRIG_HD inline void sincos_fast(double x, double* s, double* c) {
     if (fabs(x) <= 1048576.0)
         sincos_reduced(x, s, c);

@@ -16,7 +16,7 @@ def test_sincos_fast_accuracy(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "focss_amd", "csrc"),
                     os.path.join(ROOT, "tests", "trig_check.cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    e64, e64_big, e32, e_fb, e_small, e_small32 = map(float, out)
+    e64, e64_big, e32, e_fb, e_small, e_small32, e_tiny, e_tiny32 = map(float, out)
     print("max abs error: f64 %.2e (|x|<=1e2), %.2e (|x|<=1e6), f32 %.2e, fallback %.2e" % (e64, e64_big, e32, e_fb))
     assert e64 <= 4.5e-16
     assert e64_big <= 4.5e-16
@@ -25,3 +25,6 @@ def test_sincos_fast_accuracy(tmp_path):
     print("short series on |x| <= 1/16: f64 %.2e, f32 %.2e" % (e_small, e_small32))
     assert e_small <= 2.3e-16
     assert e_small32 <= 1.2e-7
+    print("three-term series on |x| <= 2^-8: f64 %.2e, f32 %.2e" % (e_tiny, e_tiny32))
+    assert e_tiny <= 2.3e-16
+    assert e_tiny32 <= 1.2e-7

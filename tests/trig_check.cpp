@@ -46,9 +46,20 @@ int main() {
         focss_hip::sincos_small(xf, &sf, &cf);
         e_small32 = fmax(e_small32, fmax(fabs(sf - (double)sinl(xf)), fabs(cf - (double)cosl(xf))));
     }
+    double e_tiny = 0, e_tiny32 = 0;
+    for (int i = -200000; i <= 200000; ++i) {  // the three-term series on |x| <= 2^-8
+        double x = 0.00390625 * i / 200000.0;
+        double ss, cc;
+        focss_hip::sincos_tiny(x, &ss, &cc);
+        e_tiny = fmax(e_tiny, fmax(fabs(ss - (double)sinl(x)), fabs(cc - (double)cosl(x))));
+        float sf, cf, xf = (float)x;
+        focss_hip::sincos_tiny(xf, &sf, &cf);
+        e_tiny32 = fmax(e_tiny32, fmax(fabs(sf - (double)sinl(xf)), fabs(cf - (double)cosl(xf))));
+    }
     double s, c;
     focss_hip::sincos_fast(3.0e7, &s, &c);  // library fallback
     double e_fb = fmax(fabs(s - sin(3.0e7)), fabs(c - cos(3.0e7)));
-    printf("%.3e %.3e %.3e %.3e %.3e %.3e\n", e64, e64_big, e32, e_fb, e_small, e_small32);
+    printf("%.3e %.3e %.3e %.3e %.3e %.3e %.3e %.3e\n", e64, e64_big, e32, e_fb, e_small, e_small32, e_tiny,
+           e_tiny32);
     return 0;
 }
