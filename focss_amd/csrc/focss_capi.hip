@@ -1326,39 +1326,54 @@ int records_check(focss_ctx* c, int envelopes, int samples, const void* in, void
 namespace {
 
 using LdbpKernel = void (*)(LdbpArgs);
+using LdbpFwdStepKernel = void (*)(LdbpArgs, int, double*);
+using LdbpStepKernel = void (*)(LdbpArgs, int, const cx<float>*, cx<float>*);
+
+// the kernels of one (taps, kerr taps, modes) model: the whole-waveform
+// forward (inference), and the per-step forward and backward of training
+struct LdbpSet {
+    LdbpKernel fwd;
+    LdbpFwdStepKernel fstep;
+    LdbpStepKernel bwd;
+};
+
+template <int K, int KP, int M>
+LdbpSet ldbp_set() {
+    return {&k_ldbp_fwd<K, KP, M>, &k_ldbp_fwd_step<K, KP, M>, &k_ldbp_bwd_step<K, KP, M>};
+}
 
 template <int K, int M>
-bool ldbp_kernels_km(int ktaps, LdbpKernel* fwd, LdbpKernel* bwd) {
+bool ldbp_kernels_km(int ktaps, LdbpSet* set) {
     switch (ktaps) {
-        case 1: *fwd = &k_ldbp_fwd<K, 1, M>; *bwd = &k_ldbp_bwd<K, 1, M>; return true;
-        case 5: *fwd = &k_ldbp_fwd<K, 5, M>; *bwd = &k_ldbp_bwd<K, 5, M>; return true;
-        case 9: *fwd = &k_ldbp_fwd<K, 9, M>; *bwd = &k_ldbp_bwd<K, 9, M>; return true;
-        case 15: *fwd = &k_ldbp_fwd<K, 15, M>; *bwd = &k_ldbp_bwd<K, 15, M>; return true;
+        case 1: *set = ldbp_set<K, 1, M>(); return true;
+        case 5: *set = ldbp_set<K, 5, M>(); return true;
+        case 9: *set = ldbp_set<K, 9, M>(); return true;
+        case 15: *set = ldbp_set<K, 15, M>(); return true;
         default: return false;
     }
 }
 
 template <int K>
-bool ldbp_kernels_k(int ktaps, int modes, LdbpKernel* fwd, LdbpKernel* bwd) {
-    if (modes == 1) return ldbp_kernels_km<K, 1>(ktaps, fwd, bwd);
-    if (modes == 2) return ldbp_kernels_km<K, 2>(ktaps, fwd, bwd);
+bool ldbp_kernels_k(int ktaps, int modes, LdbpSet* set) {
+    if (modes == 1) return ldbp_kernels_km<K, 1>(ktaps, set);
+    if (modes == 2) return ldbp_kernels_km<K, 2>(ktaps, set);
     return false;
 }
 
-bool ldbp_kernels(int taps, int ktaps, int modes, LdbpKernel* fwd, LdbpKernel* bwd) {
-    if (taps == 15) return ldbp_kernels_k<15>(ktaps, modes, fwd, bwd);
-    if (taps == 23) return ldbp_kernels_k<23>(ktaps, modes, fwd, bwd);
-    if (taps == 31) return ldbp_kernels_k<31>(ktaps, modes, fwd, bwd);
-    if (taps == 47) return ldbp_kernels_k<47>(ktaps, modes, fwd, bwd);
-    if (taps == 63) return ldbp_kernels_k<63>(ktaps, modes, fwd, bwd);
-    if (taps == 5) return ldbp_kernels_k<5>(ktaps, modes, fwd, bwd);
+bool ldbp_kernels(int taps, int ktaps, int modes, LdbpSet* set) {
+    if (taps == 15) return ldbp_kernels_k<15>(ktaps, modes, set);
+    if (taps == 23) return ldbp_kernels_k<23>(ktaps, modes, set);
+    if (taps == 31) return ldbp_kernels_k<31>(ktaps, modes, set);
+    if (taps == 47) return ldbp_kernels_k<47>(ktaps, modes, set);
+    if (taps == 63) return ldbp_kernels_k<63>(ktaps, modes, set);
+    if (taps == 5) return ldbp_kernels_k<5>(ktaps, modes, set);
     return false;
 }
 
 int ldbp_check(int steps, int taps, int ktaps, int modes, int samples, int batch) {
     if (modes != 1 && modes != 2) return fail(FOCSS_E_ARG, "learned DBP: modes must be 1 or 2");
-    LdbpKernel f, b;
-    if (!ldbp_kernels(taps, ktaps, modes, &f, &b))
+    LdbpSet set;
+    if (!ldbp_kernels(taps, ktaps, modes, &set))
         return fail(FOCSS_E_UNSUPPORTED, "learned DBP: taps must be 5, 15, 23, 31, 47 or 63 and kerr_taps 1, 5, 9 or 15");
     if (steps < 1 || batch < 1) return fail(FOCSS_E_ARG, "learned DBP: steps and batch must be >= 1");
     int l = ilog2_int(samples);
@@ -1367,8 +1382,8 @@ int ldbp_check(int steps, int taps, int ktaps, int modes, int samples, int batch
     return 0;
 }
 
-// forward: N / 16 threads; backward: 1024 threads; both hold every mode in
-// LDS in the p16 layout, M (N + N/16) complex64
+// forward: one workgroup of N / 16 threads per waveform, every mode in LDS
+// in the p16 layout, M (N + N/16) complex64
 int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
     hipLaunchKernelGGL(k, dim3(a.B), dim3(a.N / LDBP_R), (size_t)modes * (a.N + a.N / 16) * sizeof(cx<float>),
                        c->stream, a);
@@ -1376,9 +1391,33 @@ int ldbp_launch_fwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
     return 0;
 }
 
-int ldbp_launch_bwd(focss_ctx* c, LdbpKernel k, const LdbpArgs& a, int modes) {
-    hipLaunchKernelGGL(k, dim3(a.B), dim3(LDBP_NT), (size_t)modes * (a.N + a.N / 16) * sizeof(cx<float>),
-                       c->stream, a);
+// training forward: one launch per step, one workgroup per LDBP_CH-sample
+// chunk of each waveform; the loss shares of the chunks added in order
+int ldbp_launch_fwd_steps(focss_ctx* c, LdbpFwdStepKernel k, const LdbpArgs& a, int modes, double* lslot) {
+    const int groups = a.B * (a.N / LDBP_CH);
+    for (int j = 0; j < a.J; ++j) {
+        hipLaunchKernelGGL(k, dim3(groups), dim3(ldbp_bwd_threads(modes)), 0, c->stream, a, j, lslot);
+        HIP_TRY(hipGetLastError());
+    }
+    if (a.loss && a.target) {
+        hipLaunchKernelGGL(k_ldbp_loss_reduce, dim3(1), dim3(256), 0, c->stream, (const double*)lslot, groups, a.loss);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// backward: one launch per step, last step first, one workgroup per
+// LDBP_CH-sample chunk of each waveform; G_y alternates between gy0 and gy1
+int ldbp_launch_bwd(focss_ctx* c, LdbpStepKernel k, const LdbpArgs& a, int modes, cx<float>* gy0, cx<float>* gy1) {
+    const int groups = a.B * (a.N / LDBP_CH);
+    for (int j = a.J - 1; j >= 0; --j) {
+        cx<float>* out = (j & 1) ? gy1 : gy0;
+        const cx<float>* in = (j & 1) ? gy0 : gy1;
+        hipLaunchKernelGGL(k, dim3(groups), dim3(ldbp_bwd_threads(modes)), 0, c->stream, a, j, in, out);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_ldbp_grad_reduce, dim3(a.J * (2 * a.K + a.KP)), dim3(256), 0, c->stream,
+                       (const double*)a.part, groups, a.J, a.K, a.KP, a.grad);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1852,10 +1891,17 @@ int focss_wdm_ppe(focss_ctx* c, int precision, int c_radius, const double* C, in
     return 0;
 }
 
+// activations, final output, two G_y buffers, then the per-workgroup
+// gradient sums of every backward step (taps up to 63, kerr taps up to 15)
+// and the per-workgroup loss shares
+static size_t ldbp_part_doubles(int steps, int samples, int batch) {
+    return (size_t)(steps * (2 * 63 + 15) + 1) * batch * (samples / LDBP_CH);
+}
+
 size_t focss_ldbp_workspace_bytes(int steps, int modes, int samples, int batch) {
     size_t acts = 2ull * (size_t)batch * steps * modes * samples * sizeof(cx<float>);
     size_t out = (size_t)batch * modes * samples * sizeof(cx<float>);
-    return acts + 2 * out + 256;  // activations, final output, step scratch
+    return acts + 3 * out + ldbp_part_doubles(steps, samples, batch) * sizeof(double) + 256;
 }
 
 int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
@@ -1864,8 +1910,8 @@ int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int mod
         return fail(FOCSS_E_ARG, "learned DBP: null argument");
     if (int rc = ldbp_check(steps, taps, kerr_taps, modes, samples, batch)) return rc;
     HIP_TRY(hipSetDevice(c->device));
-    LdbpKernel fwd, bwd;
-    ldbp_kernels(taps, kerr_taps, modes, &fwd, &bwd);
+    LdbpSet set;
+    ldbp_kernels(taps, kerr_taps, modes, &set);
     double* dloss = nullptr;
     if (target) {
         HIP_TRY(dev_malloc(&dloss, sizeof(double)));
@@ -1883,7 +1929,7 @@ int focss_ldbp_forward(focss_ctx* c, int steps, int taps, int kerr_taps, int mod
     a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    int rc = ldbp_launch_fwd(c, fwd, a, modes);
+    int rc = ldbp_launch_fwd(c, set.fwd, a, modes);
     if (dloss) {
         hipError_t e = hipSuccess;
         if (rc == 0) e = hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream);
@@ -1951,15 +1997,16 @@ int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int m
     if (workspace_bytes < focss_ldbp_workspace_bytes(steps, modes, samples, batch))
         return fail(FOCSS_E_ARG, "learned DBP: workspace too small");
     HIP_TRY(hipSetDevice(c->device));
-    LdbpKernel fwd, bwd;
-    ldbp_kernels(taps, kerr_taps, modes, &fwd, &bwd);
+    LdbpSet set;
+    ldbp_kernels(taps, kerr_taps, modes, &set);
     size_t act = (size_t)batch * steps * modes * samples;
     cx<float>* xs = (cx<float>*)workspace;
     cx<float>* ys = xs + act;
     cx<float>* out = ys + act;
-    cx<float>* zs = out + (size_t)batch * modes * samples;
+    cx<float>* gy0 = out + (size_t)batch * modes * samples;
+    cx<float>* gy1 = gy0 + (size_t)batch * modes * samples;
     LdbpArgs a = {};
-    a.zs = zs;
+    a.part = reinterpret_cast<double*>(gy1 + (size_t)batch * modes * samples);
     a.in = (const cx<float>*)in;
     a.out = out;
     a.target = (const cx<float>*)target;
@@ -1974,8 +2021,9 @@ int focss_ldbp_gradients(focss_ctx* c, int steps, int taps, int kerr_taps, int m
     a.KP = kerr_taps;
     a.N = samples;
     a.B = batch;
-    if (int rc = ldbp_launch_fwd(c, fwd, a, modes)) return rc;
-    if (int rc = ldbp_launch_bwd(c, bwd, a, modes)) return rc;
+    double* lslot = a.part + (size_t)steps * (2 * taps + kerr_taps) * batch * (samples / LDBP_CH);
+    if (int rc = ldbp_launch_fwd_steps(c, set.fstep, a, modes, lslot)) return rc;
+    if (int rc = ldbp_launch_bwd(c, set.bwd, a, modes, gy0, gy1)) return rc;
     return 0;
 }
 

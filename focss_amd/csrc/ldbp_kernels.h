@@ -17,15 +17,15 @@
 // the intensity summed over both, so the phase is common to them; the loss
 // averages over every mode and sample, and
 //     q = -sum_modes Im(conj(G_z) z),   G_h_k sums over the modes.
-// One workgroup owns one waveform (M N <= 16384 complex64 in LDS) and runs
-// every step on its CU.
+// The forward pass runs one workgroup per waveform (M N <= 16384 complex64
+// in LDS) through every step; the backward pass runs one launch per step
+// over chunks of every waveform (k_ldbp_bwd_step).
 #pragma once
 #include "cx.h"
 #include "trig.h"
 
 namespace focss_hip {
 
-constexpr int LDBP_NT = 1024;       // backward pass threads
 constexpr int LDBP_MAX_N = 16384;
 constexpr int LDBP_MIN_N = 1024;
 constexpr int LDBP_R = 16;          // forward pass: consecutive outputs per thread (N / 16 threads)
@@ -41,13 +41,13 @@ struct LdbpArgs {
     cx<float>* ys;            // [B][J][M][N] FIR outputs (training) or null
     double* loss;             // [1] accumulated, may be null
     double* grad;             // [J*K*2 + J*KP] accumulated: Re/Im of G_h, then dL/dg
-    cx<float>* zs;            // [B][M][N] backward scratch (per-sample q and phi, input gradient)
+    double* part;             // [J][backward groups][2K + KP] per-workgroup gradient sums
     int J, K, KP, N, B;
 };
 
 // forward-pass LDS index: one padding element every 16, so the 16-element
 // blocks that neighbouring lanes own start in different banks
-__device__ __forceinline__ int p16(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int p16(int i) { return i + (i >> 4); }
 
 // wave-uniform read through the scalar data cache
 __device__ __forceinline__ float sload(const float* p) {
@@ -204,151 +204,152 @@ __global__ __launch_bounds__(LDBP_MAX_N / LDBP_R) void k_ldbp_fwd(LdbpArgs a) {
     }
 }
 
-// needs the forward pass run with xs / ys stored; out holds z_J.  LDS holds
-// the gradient of every mode, p16 layout with mode stride N + N / 16; zs
-// ([B][M][N]) stages the per-sample q and phi (first N).  The two FIR-shaped
-// contractions -- the tap gradients and the input gradient -- run on work
-// units of 16 consecutive samples of one mode (M N / 16 <= 1024 units, at
-// most one per thread) with register windows, as the forward FIR does.
+// ------------------------------------------------------------------
+// Backward pass split over the whole device: one launch per step j (from
+// J - 1 down to 0), one workgroup per LDBP_CH-sample chunk of one waveform
+// (all modes).  A launch forms the step's output gradient G_z on the chunk
+// and a LDBP_HL-sample halo either side -- from (out - target) for the last
+// step, else as the input gradient of step j + 1 from the G_y that the
+// previous launch left in global memory -- then runs the nonlinear-phase
+// backward (q, phi over chunk and halo, G_y over the chunk), writes G_y for
+// the next launch and adds the step's tap gradients.  The kernel boundary is
+// the only synchronisation between chunks; the halo work (2 LDBP_HL of
+// LDBP_CH samples) is done twice.  Every operand window (G_y of step j + 1,
+// y and x of step j) is staged into LDS by coalesced loads first; the two
+// FIR-shaped contractions run on units of LDBP_RB consecutive samples with
+// register windows of LDBP_KB taps.
+// ------------------------------------------------------------------
+constexpr int LDBP_CH = 512;   // samples per workgroup
+constexpr int LDBP_HL = 16;    // halo either side (>= KP / 2)
+constexpr int LDBP_EXT = LDBP_CH + 2 * LDBP_HL;
+constexpr int LDBP_RB = 4;     // consecutive samples per work unit
+constexpr int LDBP_KB = 8;     // taps per register window
+
+__host__ __device__ constexpr int ldbp_bwd_threads(int modes) {
+    return ((modes * LDBP_EXT / LDBP_RB + 63) / 64) * 64;
+}
+
 template <int K, int KP, int M>
-__global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
-    extern __shared__ unsigned char smem_raw[];
-    cx<float>* g = reinterpret_cast<cx<float>*>(smem_raw);  // [M][p16(N)]
-    // per-wave sums of the step's parameter gradients: Re/Im of the FIR taps,
-    // then the intensity taps; one barrier per step combines them
-    __shared__ double gsum[LDBP_NT / 64][2 * K + KP];
-    constexpr int R = LDBP_R, KC = 4;  // taps per register window (register budget at 1024 threads)
-    constexpr int P = KP / 2;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int N = a.N, mask = N - 1, c = K / 2, S = N + N / 16, lg = __builtin_ctz(N);
-    const int b = blockIdx.x;
-    const float scale = 1.0f / ((float)N * (float)M * (float)a.B);
-    const long long wb = (long long)b * M * N;  // this waveform's element offset
-    cx<float>* zs = a.zs + wb;
-    // this thread's work unit: mode um, samples u0 .. u0 + 15
-    const int units = M * N / R;
-    const bool has_unit = threadIdx.x < units;
-    const int um = (threadIdx.x * R) >> lg, u0 = (threadIdx.x * R) & mask;
-    for (int i = threadIdx.x; i < M * N; i += LDBP_NT)
-        g[(i >> lg) * S + p16(i & mask)] = (a.out[wb + i] - a.target[wb + i]) * scale;
+__global__ __launch_bounds__(ldbp_bwd_threads(M)) void k_ldbp_bwd_step(LdbpArgs a, int j, const cx<float>* gy_in,
+                                                                        cx<float>* gy_out) {
+    constexpr int NT = ldbp_bwd_threads(M);
+    constexpr int R = LDBP_RB, KC = LDBP_KB, P = KP / 2, c = K / 2;
+    constexpr int CH = LDBP_CH, HL = LDBP_HL, EXT = LDBP_EXT;
+    constexpr int KR = (K + KC - 1) / KC * KC;  // taps rounded up to whole windows
+    constexpr int WIN = EXT + KR + R;           // G_y window of the halo-wide input gradient
+    constexpr int YW = EXT + 2 * P;             // y window of the intensities on chunk and halo
+    constexpr int XW = CH + KR + R;             // x window of the tap gradients
+    static_assert(P <= HL, "the halo covers the intensity filter");
+    static_assert(EXT % R == 0 && CH % R == 0, "whole units");
+    __shared__ cx<float> gyw[M][WIN];  // G_y of step j + 1 at lo - HL - c + w
+    __shared__ cx<float> gz[M][EXT];   // G_z of step j at lo - HL + e
+    __shared__ cx<float> yw[M][YW];    // y of step j at lo - HL - P + v
+    __shared__ cx<float> xw[M][XW];    // x of step j at lo + c - (KR - 1) - ... (see below)
+    __shared__ float2 qphi[EXT];       // q and phi at lo - HL + e
+    __shared__ float pw[YW];           // intensity summed over the modes, as yw
+    constexpr int TP = NT / KR;        // slices of the chunk per FIR tap
+    constexpr int MP = NT / KP;        // slices per intensity tap
+    __shared__ double hsum[TP][KR][2];  // per-slice FIR tap gradients
+    __shared__ double psum[MP][KP];     // per-slice intensity-tap gradients
+    const int N = a.N, mask = N - 1;
+    const int nch = N / CH;
+    const int b = blockIdx.x / nch, lo = (blockIdx.x % nch) * CH;
+    const long long wb = (long long)b * M * N;
+    const cx<float>* x = a.xs + ((long long)b * a.J + j) * M * N;  // [M][N] FIR input of step j
+    const cx<float>* y = a.ys + ((long long)b * a.J + j) * M * N;  // [M][N] FIR output of step j
+    const float* gk = a.kerr + j * KP;
+    const float* hj = reinterpret_cast<const float*>(a.fir + (long long)j * K);
+    // x window: xw[v] = x(lo + c - KR + 1 - ... ) laid out so that unit u0's
+    // tap k reads xw[u0 + r - k + KR - 1]: xw[v] = x(lo + c - (KR - 1) + v)
+    const int x0 = lo + c - (KR - 1);
+
+    // stage the windows: every load issued before the first LDS write, so
+    // that their latencies overlap
+    constexpr int IY = (M * YW + NT - 1) / NT, IX = (M * XW + NT - 1) / NT;
+    constexpr int IG = (M * WIN + NT - 1) / NT, IZ = (M * EXT + NT - 1) / NT;
+    const bool last = j == a.J - 1;
+    cx<float> sy[IY], sx[IX], sg[IG > 2 * IZ ? IG : 2 * IZ];
+#pragma unroll
+    for (int k = 0; k < IY; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < M * YW) sy[k] = y[(i / YW) * N + ((lo - HL - P + i % YW) & mask)];
+    }
+#pragma unroll
+    for (int k = 0; k < IX; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < M * XW) sx[k] = x[(i / XW) * N + ((x0 + i % XW) & mask)];
+    }
+    if (last) {
+#pragma unroll
+        for (int k = 0; k < IZ; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < M * EXT) {
+                const long long t = wb + (long long)(i / EXT) * N + ((lo - HL + i % EXT) & mask);
+                sg[2 * k] = a.out[t];
+                sg[2 * k + 1] = a.target[t];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < IG; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < M * WIN) sg[k] = gy_in[wb + (long long)(i / WIN) * N + ((lo - HL - c + i % WIN) & mask)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IY; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < M * YW) yw[i / YW][i % YW] = sy[k];
+    }
+#pragma unroll
+    for (int k = 0; k < IX; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < M * XW) xw[i / XW][i % XW] = sx[k];
+    }
+    if (last) {
+        const float scale = 1.0f / ((float)N * (float)M * (float)a.B);
+#pragma unroll
+        for (int k = 0; k < IZ; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < M * EXT) gz[i / EXT][i % EXT] = (sg[2 * k] - sg[2 * k + 1]) * scale;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < IG; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < M * WIN) gyw[i / WIN][i % WIN] = sg[k];
+        }
+    }
     __syncthreads();
-    for (int j = a.J - 1; j >= 0; --j) {
-        const cx<float>* x = a.xs + ((long long)b * a.J + j) * M * N;  // [M][N]
-        const cx<float>* y = a.ys + ((long long)b * a.J + j) * M * N;
-        const float* gk = a.kerr + j * KP;
-        const float* hj = reinterpret_cast<const float*>(a.fir + (long long)j * K);
-        // nonlinear phase backward, pass 1: q and phi per sample (staged in
-        // zs), intensity-tap gradients
-        double dg[KP];
+
+    for (int v = threadIdx.x; v < YW; v += NT) {
+        float pv = norm2(yw[0][v]);
 #pragma unroll
-        for (int m = 0; m < KP; ++m) dg[m] = 0;
-        for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-            float pn[KP];  // p(t - m + P), summed over the modes
+        for (int md = 1; md < M; ++md) pv += norm2(yw[md][v]);
+        pw[v] = pv;
+    }
+
+    // 1. G_z of step j as the input gradient of step j + 1 on chunk and halo:
+    // G_x(s) = sum_k conj(h_k) G_y(s + k - c), s = lo - HL + e, from
+    // gyw[e + k]
+    if (j < a.J - 1) {
+        const float* hn = reinterpret_cast<const float*>(a.fir + (long long)(j + 1) * K);
+        const int u = threadIdx.x;
+        if (u < M * (EXT / R)) {
+            const int md = u / (EXT / R), e0 = (u % (EXT / R)) * R;
+            cx<float> gx[R];
 #pragma unroll
-            for (int m = 0; m < KP; ++m) {
-                pn[m] = norm2(y[(t - m + P) & mask]);
-#pragma unroll
-                for (int md = 1; md < M; ++md) pn[m] += norm2(y[md * N + ((t - m + P) & mask)]);
-            }
-            float phi = 0;
-#pragma unroll
-            for (int m = 0; m < KP; ++m) phi = fmaf(gk[m], pn[m], phi);
-            float s, co;
-            sincos_fast(phi, &s, &co);
-            float q = 0;  // -sum_modes Im(conj(gz) z)
-#pragma unroll
-            for (int md = 0; md < M; ++md) {
-                const cx<float> zv = y[md * N + t] * mk<float>(co, s);
-                const cx<float> gz = g[md * S + p16(t)];
-                q -= gz.x * zv.y - gz.y * zv.x;
-            }
-#pragma unroll
-            for (int m = 0; m < KP; ++m) dg[m] += 2.0 * (double)q * (double)pn[m];
-            zs[t] = mk<float>(q, phi);
-        }
-#pragma unroll
-        for (int m = 0; m < KP; ++m) {
-            const double st = ldbp_wave_sum(dg[m]);
-            if (lane == 0) gsum[wid][2 * K + m] = st;
-        }
-        __syncthreads();
-        // pass 2: G_y = G_z e^{-i phi} + (dL/dp) y, in place
-        for (int t = threadIdx.x; t < N; t += LDBP_NT) {
-            float dp = 0;
-#pragma unroll
-            for (int m = 0; m < KP; ++m) dp = fmaf(gk[m], 2.0f * zs[(t + m - P) & mask].x, dp);
-            const float phi = zs[t].y;
-            float s, co;
-            sincos_fast(phi, &s, &co);
-#pragma unroll
-            for (int md = 0; md < M; ++md) {
-                cx<float>& gv = g[md * S + p16(t)];
-                gv = gv * mk<float>(co, -s) + y[md * N + t] * dp;
-            }
-        }
-        __syncthreads();
-        // FIR tap gradients G_h_k = sum_modes sum_t G_y(t) conj(x(t - k + c)):
-        // KC taps per sweep, the unit's 16 G_y values and a window of
-        // 16 + KC - 1 inputs in registers; partial sums in float, wave sums in
-        // double
-        cx<float> gy[R];
-        if (has_unit) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) gy[r] = g[um * S + p16(u0 + r)];
-        }
-#pragma unroll 1
-        for (int k0 = 0; k0 < K; k0 += KC) {
-            float dre[KC], dim[KC];
-#pragma unroll
-            for (int q = 0; q < KC; ++q) {
-                dre[q] = 0;
-                dim[q] = 0;
-            }
-            if (has_unit) {
-                cx<float> w[R + KC - 1];  // x(u0 + c - k0 - (KC - 1) + v)
-                const int base = u0 + c - k0 - (KC - 1);
-#pragma unroll
-                for (int v = 0; v < R + KC - 1; ++v) w[v] = x[um * N + ((base + v) & mask)];
-#pragma unroll
-                for (int q = 0; q < KC; ++q) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const cx<float> xv = w[r - q + KC - 1];  // x(u0 + r - (k0 + q) + c)
-                        dre[q] = fmaf(gy[r].x, xv.x, fmaf(gy[r].y, xv.y, dre[q]));
-                        dim[q] = fmaf(gy[r].y, xv.x, fmaf(-gy[r].x, xv.y, dim[q]));
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < KC; ++q) {
-                if (k0 + q < K) {
-                    const double sr = ldbp_wave_sum((double)dre[q]);
-                    const double si = ldbp_wave_sum((double)dim[q]);
-                    if (lane == 0) {
-                        gsum[wid][2 * (k0 + q)] = sr;
-                        gsum[wid][2 * (k0 + q) + 1] = si;
-                    }
-                }
-            }
-        }
-        // input gradient G_x(s) = sum_k conj(h_k) G_y(s + k - c) of the unit's
-        // 16 samples, KC taps per window of 16 + KC - 1 values of G_y
-        cx<float> gx[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) gx[r] = mk<float>(0, 0);
-        if (has_unit) {
-            const cx<float>* gm = g + um * S;
+            for (int r = 0; r < R; ++r) gx[r] = mk<float>(0, 0);
 #pragma unroll 1
             for (int k0 = 0; k0 < K; k0 += KC) {
-                cx<float> w[R + KC - 1];  // G_y(u0 + k0 - c + v)
-                const int base = u0 + k0 - c;
+                cx<float> w[R + KC - 1];  // G_y at window positions e0 + k0 + v
 #pragma unroll
-                for (int v = 0; v < R + KC - 1; ++v) w[v] = gm[p16((base + v) & mask)];
+                for (int v = 0; v < R + KC - 1; ++v) w[v] = gyw[md][e0 + k0 + v];
 #pragma unroll
                 for (int q = 0; q < KC; ++q) {
                     const bool live = k0 + q < K;
-                    const float hr = live ? sload(hj + 2 * (k0 + q)) : 0.0f;
-                    const float hi = live ? sload(hj + 2 * (k0 + q) + 1) : 0.0f;
+                    const float hr = live ? sload(hn + 2 * (k0 + q)) : 0.0f;
+                    const float hi = live ? sload(hn + 2 * (k0 + q) + 1) : 0.0f;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const cx<float> gv = w[r + q];  // conj(h) * G_y
@@ -357,22 +358,243 @@ __global__ __launch_bounds__(LDBP_NT) void k_ldbp_bwd(LdbpArgs a) {
                     }
                 }
             }
-        }
-        // every read of G_y (and of the waves' sums) is done; one atomic per
-        // parameter of the step, and G_x replaces G_y
-        __syncthreads();
-        for (int i = threadIdx.x; i < 2 * K + KP; i += LDBP_NT) {
-            double st = 0;
-            for (int w = 0; w < LDBP_NT / 64; ++w) st += gsum[w][i];
-            double* dst = i < 2 * K ? a.grad + (long long)j * K * 2 + i
-                                    : a.grad + (long long)a.J * K * 2 + (long long)j * KP + (i - 2 * K);
-            atomicAdd(dst, st);
-        }
-        if (has_unit) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) g[um * S + p16(u0 + r)] = gx[r];
+            for (int r = 0; r < R; ++r) gz[md][e0 + r] = gx[r];
         }
+    }
+    __syncthreads();
+
+    // 2. q and phi on chunk and halo
+    for (int e = threadIdx.x; e < EXT; e += NT) {
+        float phi = 0;  // sum_m g_m p(t - m + P), p(t - m + P) = pw[e + 2 P - m]
+#pragma unroll
+        for (int m = 0; m < KP; ++m) phi = fmaf(gk[m], pw[e + 2 * P - m], phi);
+        float s, co;
+        sincos_fast(phi, &s, &co);
+        float q = 0;  // -sum_modes Im(conj(gz) z)
+#pragma unroll
+        for (int md = 0; md < M; ++md) {
+            const cx<float> zv = yw[md][e + P] * mk<float>(co, s);
+            const cx<float> gzv = gz[md][e];
+            q -= gzv.x * zv.y - gzv.y * zv.x;
+        }
+        qphi[e] = make_float2(q, phi);
+    }
+    __syncthreads();
+
+    // 3. G_y = G_z e^{-i phi} + (dL/dp) y on the chunk, kept in gz and
+    // written out for the next launch
+    for (int i = threadIdx.x; i < CH; i += NT) {
+        const int e = HL + i;
+        float dp = 0;
+#pragma unroll
+        for (int m = 0; m < KP; ++m) dp = fmaf(gk[m], 2.0f * qphi[e + m - P].x, dp);
+        float s, co;
+        sincos_fast(qphi[e].y, &s, &co);
+#pragma unroll
+        for (int md = 0; md < M; ++md) {
+            const cx<float> gv = gz[md][e] * mk<float>(co, -s) + yw[md][e + P] * dp;
+            gz[md][e] = gv;
+            gy_out[wb + (long long)md * N + lo + i] = gv;
+        }
+    }
+    __syncthreads();
+
+    // 4. parameter gradients over the chunk, each as whole sums of one
+    // parameter over one slice of the chunk (no cross-lane reductions):
+    // FIR taps G_h_k = sum_modes sum_t G_y(t) conj(x(t - k + c)), with
+    // x(t - k + c) = xw[t - lo - k + KR - 1], and intensity taps
+    // dL/dg_m = sum_t 2 q(t) p(t - m + P)
+    {
+        const int k = threadIdx.x % KR, sl = threadIdx.x / KR;
+        if (sl < TP) {
+            constexpr int SPAN = (CH + TP - 1) / TP;
+            const int i0 = sl * SPAN, i1 = i0 + SPAN < CH ? i0 + SPAN : CH;
+            float dre = 0, dim = 0;
+#pragma unroll
+            for (int md = 0; md < M; ++md) {
+                for (int i = i0; i < i1; ++i) {
+                    const cx<float> gv = gz[md][HL + i];
+                    const cx<float> xv = xw[md][i - k + KR - 1];
+                    dre = fmaf(gv.x, xv.x, fmaf(gv.y, xv.y, dre));
+                    dim = fmaf(gv.y, xv.x, fmaf(-gv.x, xv.y, dim));
+                }
+            }
+            hsum[sl][k][0] = dre;
+            hsum[sl][k][1] = dim;
+        }
+        const int m = threadIdx.x % KP, sp = threadIdx.x / KP;
+        if (sp < MP) {
+            constexpr int SPAN = (CH + MP - 1) / MP;
+            const int i0 = sp * SPAN, i1 = i0 + SPAN < CH ? i0 + SPAN : CH;
+            double d = 0;
+            for (int i = i0; i < i1; ++i) d += 2.0 * (double)qphi[HL + i].x * (double)pw[HL + i + 2 * P - m];
+            psum[sp][m] = d;
+        }
+    }
+    // this workgroup's sums of the step's parameter gradients, slices added
+    // in order, in its own slot (k_ldbp_grad_reduce adds the slots in a
+    // fixed order)
+    __syncthreads();
+    double* slot = a.part + ((long long)j * gridDim.x + blockIdx.x) * (2 * K + KP);
+    for (int i = threadIdx.x; i < 2 * K + KP; i += NT) {
+        double st = 0;
+        if (i < 2 * K) {
+            for (int sl = 0; sl < TP; ++sl) st += hsum[sl][i >> 1][i & 1];
+        } else {
+            for (int sp = 0; sp < MP; ++sp) st += psum[sp][i - 2 * K];
+        }
+        slot[i] = st;
+    }
+}
+
+// Training forward pass split over the whole device: one launch per step j,
+// one workgroup per LDBP_CH-sample chunk of one waveform (all modes).  The
+// step reads its input x_j (the caller's input for j = 0, else xs[j]), forms
+// the FIR output y on the chunk and a LDBP_HL-sample halo either side, the
+// phase from the filtered intensity and z = y e^{i phi} on the chunk, and
+// stores y (ys[j]) and z (xs[j + 1], or out after the last step, with the
+// chunk's share of the loss in its own slot of lslot).
+template <int K, int KP, int M>
+__global__ __launch_bounds__(ldbp_bwd_threads(M)) void k_ldbp_fwd_step(LdbpArgs a, int j, double* lslot) {
+    constexpr int NT = ldbp_bwd_threads(M);
+    constexpr int R = LDBP_RB, KC = LDBP_KB, P = KP / 2, c = K / 2;
+    constexpr int CH = LDBP_CH, HL = LDBP_HL, EXT = LDBP_EXT;
+    constexpr int KR = (K + KC - 1) / KC * KC;
+    constexpr int XW = EXT + KR + R;  // x window: xw[v] = x(lo - HL + c - (KR - 1) + v)
+    static_assert(P <= HL, "the halo covers the intensity filter");
+    __shared__ cx<float> xw[M][XW];
+    __shared__ cx<float> yv[M][EXT];  // y at lo - HL + e
+    __shared__ float pw[EXT];         // intensity summed over the modes
+    __shared__ double red[NT / 64];
+    const int N = a.N, mask = N - 1;
+    const int nch = N / CH;
+    const int b = blockIdx.x / nch, lo = (blockIdx.x % nch) * CH;
+    const long long wb = (long long)b * M * N;
+    const cx<float>* x = j == 0 ? a.in + wb : a.xs + ((long long)b * a.J + j) * M * N;
+    const int x0 = lo - HL + c - (KR - 1);
+    constexpr int IX = (M * XW + NT - 1) / NT;
+    cx<float> sx[IX];
+#pragma unroll
+    for (int k = 0; k < IX; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < M * XW) sx[k] = x[(i / XW) * N + ((x0 + i % XW) & mask)];
+    }
+#pragma unroll
+    for (int k = 0; k < IX; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < M * XW) xw[i / XW][i % XW] = sx[k];
+    }
+    if (j == 0 && a.xs) {  // the backward pass reads every step's input
+        cx<float>* xs0 = a.xs + (long long)b * a.J * M * N;
+        for (int i = threadIdx.x; i < M * CH; i += NT) {
+            const int md = i / CH, t = lo + i % CH;
+            xs0[md * N + t] = x[md * N + t];
+        }
+    }
+    __syncthreads();
+    // FIR on chunk and halo, units of R outputs: y(t) = sum_k h_k x(t - k + c),
+    // x(t - k + c) = xw[e - k + KR - 1] for t = lo - HL + e
+    const float* hj = reinterpret_cast<const float*>(a.fir + (long long)j * K);
+    const int u = threadIdx.x;
+    if (u < M * (EXT / R)) {
+        const int md = u / (EXT / R), e0 = (u % (EXT / R)) * R;
+        cx<float> yr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) yr[r] = mk<float>(0, 0);
+#pragma unroll 1
+        for (int k0 = 0; k0 < K; k0 += KC) {
+            cx<float> w[R + KC - 1];
+            const int base = e0 - k0 + KR - KC;
+#pragma unroll
+            for (int v = 0; v < R + KC - 1; ++v) w[v] = xw[md][base + v];
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                const bool live = k0 + q < K;
+                const float hr = live ? sload(hj + 2 * (k0 + q)) : 0.0f;
+                const float hi = live ? sload(hj + 2 * (k0 + q) + 1) : 0.0f;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const cx<float> xv = w[r - q + KC - 1];
+                    yr[r].x = fmaf(hr, xv.x, fmaf(-hi, xv.y, yr[r].x));
+                    yr[r].y = fmaf(hr, xv.y, fmaf(hi, xv.x, yr[r].y));
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) yv[md][e0 + r] = yr[r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < EXT; e += NT) {
+        float pv = norm2(yv[0][e]);
+#pragma unroll
+        for (int md = 1; md < M; ++md) pv += norm2(yv[md][e]);
+        pw[e] = pv;
+    }
+    __syncthreads();
+    // phase and output on the chunk
+    const float* gk = a.kerr + j * KP;
+    const bool last = j == a.J - 1;
+    cx<float>* ys = a.ys + ((long long)b * a.J + j) * M * N;
+    cx<float>* z = last ? a.out + wb : a.xs + ((long long)b * a.J + j + 1) * M * N;
+    double part = 0;
+    for (int i = threadIdx.x; i < CH; i += NT) {
+        const int e = HL + i, t = lo + i;
+        float phi = 0;
+#pragma unroll
+        for (int m = 0; m < KP; ++m) phi = fmaf(gk[m], pw[e - m + P], phi);
+        float s, co;
+        sincos_fast(phi, &s, &co);
+#pragma unroll
+        for (int md = 0; md < M; ++md) {
+            const cx<float> yy = yv[md][e];
+            const cx<float> zz = yy * mk<float>(co, s);
+            ys[md * N + t] = yy;
+            z[md * N + t] = zz;
+            if (last && a.target) part += (double)norm2(zz - a.target[wb + md * N + t]);
+        }
+    }
+    if (last && a.target) {
+        part = ldbp_wave_sum(part);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
         __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = 0;
+            for (int w = 0; w < NT / 64; ++w) sum += red[w];
+            lslot[blockIdx.x] = sum / ((double)N * M * a.B);
+        }
+    }
+}
+
+// *loss += the chunks' loss shares, added in a fixed order
+__global__ __launch_bounds__(256) void k_ldbp_loss_reduce(const double* lslot, int groups, double* loss) {
+    __shared__ double red[4];
+    double s = 0;
+    for (int g = threadIdx.x; g < groups; g += 256) s += lslot[g];
+    s = ldbp_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *loss += (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// grad += the per-workgroup sums of every step, one workgroup per (step,
+// parameter), the groups' slots added in a fixed order: the gradient does
+// not depend on the order in which workgroups finish
+__global__ __launch_bounds__(256) void k_ldbp_grad_reduce(const double* part, int groups, int J, int K, int KP,
+                                                          double* grad) {
+    __shared__ double red[4];
+    const int np = 2 * K + KP;
+    const int j = blockIdx.x / np, i = blockIdx.x % np;
+    const double* p = part + (long long)j * groups * np + i;
+    double s = 0;
+    for (int g = threadIdx.x; g < groups; g += 256) s += p[(long long)g * np];
+    s = ldbp_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double t = (red[0] + red[1]) + (red[2] + red[3]);
+        double* dst = i < 2 * K ? grad + (long long)j * K * 2 + i : grad + (long long)J * K * 2 + (long long)j * KP + (i - 2 * K);
+        *dst += t;
     }
 }
 

@@ -322,8 +322,10 @@ int focss_ldbp_infer(focss_ctx* ctx, int steps, int taps, int kerr_taps, int mod
                      const float* params, const void* in, void* out);
 /* Accumulates into grad (device, double [2 steps taps + steps kerr_taps],
  * zeroed by the caller) the Wirtinger gradient dL/d(conj h) (re, im) and
- * dL/dg, and the loss into *loss (device double).  Stream-ordered, no host
- * sync. */
+ * dL/dg, and the loss into *loss (device double).  The backward pass runs one
+ * launch per step over 512-sample chunks of every waveform and adds the
+ * chunks' sums in a fixed order, so the gradient does not depend on
+ * scheduling.  Stream-ordered, no host sync. */
 int focss_ldbp_gradients(focss_ctx* ctx, int steps, int taps, int kerr_taps, int modes, int samples, int batch,
                          const float* params, const void* in, const void* target, void* workspace,
                          size_t workspace_bytes, double* grad, double* loss);
