@@ -22,6 +22,10 @@ launches each --
   the save_field record reorder of 11 channels x 2 polarisations x 2^20
     samples, both directions, FP64;
   WdmPPE training, c = 30, d = 2, 2^14 samples (features, Gram, sums).
+--workload ldbp_train: learned-DBP Adam training steps (120 steps of 31-tap
+FIRs and 9-tap intensity filters) on synthetic fields, one polarisation
+(32 waveforms of 2^14 samples) and two (48 waveforms of 2 x 2^13), 3
+untimed then `reps` timed steps each, host-timed.
 --workload generic: plain DBP (UNIFORM, 16 steps per span, 2 spans) of
 `batch` waveforms of N = 2 n_sym samples, n_sym uniform in [2^11, 2^15]
 (seed 5: the Bluestein link kernels), a complex Gaussian field, `reps`
@@ -46,7 +50,8 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
     ap.add_argument("--copy-mib", type=int, default=128)
     ap.add_argument("--variant", type=int, default=-1, help="tile variant (default: the library's choice)")
-    ap.add_argument("--workload", choices=["ssfm", "side", "roofline", "c5", "generic"], default="ssfm")
+    ap.add_argument("--workload", choices=["ssfm", "side", "roofline", "c5", "generic", "ldbp_train"],
+                    default="ssfm")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--batches", default="64,128")
     ap.add_argument("--replays", type=int, default=4)
@@ -59,6 +64,8 @@ def main():
         return roofline(args)
     if args.workload == "generic":
         return generic(args)
+    if args.workload == "ldbp_train":
+        return ldbp_train(args)
     sys.path.insert(0, ROOT)
     import bench
     import focss_amd as F
@@ -251,6 +258,40 @@ def side(args):
             print(f"probe side: triplet MLP {what}, {batch} x {S} symbols: {t * 1e3:.3f} ms", flush=True)
             for b in (bi, bo, mlp):
                 b.free()
+    return 0
+
+
+def ldbp_train(args):
+    sys.path.insert(0, ROOT)
+    import time
+    import focss_amd as F
+    J, K, KP = 120, 31, 9
+    for modes, n, b in ((1, 1 << 14, 32), (2, 1 << 13, 48)):
+        rng = np.random.default_rng(1)
+        with F.Context(0) as ctx:
+            fir = np.zeros((J, K), np.complex128)
+            fir[:, K // 2] = 1.0
+            fir += 0.01 * (rng.standard_normal((J, K)) + 1j * rng.standard_normal((J, K)))
+            kerr = 0.001 * rng.standard_normal((J, KP))
+            m = F.LearnedDBP(ctx, J, K, n, b, fir, kerr, modes=modes)
+            size = b * modes * n
+            x = ((rng.standard_normal(size) + 1j * rng.standard_normal(size)) * 0.3).astype(np.complex64)
+            t = (0.9 * x).astype(np.complex64)
+            bx, bt = ctx.alloc(x.nbytes), ctx.alloc(t.nbytes)
+            bx.upload(x)
+            bt.upload(t)
+            for _ in range(3):
+                m.train_step(bx, bt, 1e-4)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.reps):
+                m.train_step(bx, bt, 1e-4)
+            ctx.synchronize()
+            dt = (time.perf_counter() - t0) / args.reps
+            print(f"probe ldbp_train: modes {modes}, {b} x {n} samples, {J} steps, {K} taps, {KP} intensity taps: "
+                  f"{dt * 1e3:.2f} ms per training step", flush=True)
+            for buf in (bx, bt, m):
+                buf.free()
     return 0
 
 
