@@ -70,6 +70,16 @@ __global__ void k_scale(cx<R>* __restrict__ f, long long total, double scale) {
     f[i] = f[i] * (R)scale;
 }
 
+// every element times the complex scalar (re, im) (Field::operator*=,
+// src/focss/field.cpp:381-392), grid-stride
+template <typename R>
+__global__ void k_mul_scalar(cx<R>* __restrict__ f, long long total, double re, double im) {
+    const cx<R> m = mk<R>((R)re, (R)im);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x)
+        f[i] = f[i] * m;
+}
+
 // complex128 host array converted on the device
 template <typename R>
 __global__ void k_from_c128(const cx<double>* __restrict__ in, cx<R>* __restrict__ out, long long n) {

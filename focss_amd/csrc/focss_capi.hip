@@ -1753,6 +1753,21 @@ int focss_add_noise(focss_ctx* c, int precision, int modes, int samples, double 
     return 0;
 }
 
+int focss_field_scale(focss_ctx* c, int precision, int modes, int samples, double re, double im, void* data) {
+    if (!c || !data || modes < 1 || samples < 1) return fail(FOCSS_E_ARG, "field_scale: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const long long n = (long long)modes * samples;
+    const unsigned blocks = std::min<unsigned>(grid_for(n), 8192u);
+    if (precision == FOCSS_F64)
+        hipLaunchKernelGGL(k_mul_scalar<double>, dim3(blocks), dim3(256), 0, c->stream, (cx<double>*)data, n, re, im);
+    else if (precision == FOCSS_F32)
+        hipLaunchKernelGGL(k_mul_scalar<float>, dim3(blocks), dim3(256), 0, c->stream, (cx<float>*)data, n, re, im);
+    else
+        return fail(FOCSS_E_ARG, "bad precision");
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 int focss_sino_cubic(focss_ctx* c, int precision, int radius, const double* weights, int modes, int samples,
                      const void* in, void* out) {
     if (!c || !weights || !in || !out || radius < 0 || modes < 1 || samples <= 2 * radius)

@@ -23,12 +23,93 @@ Field::Field(const Domain& d, const ComplexVector& v)
     assert(d.size() == v.size() && d.time_step > 0);
 }
 
+// A copy of a field whose current data is on the device is made on the
+// device; otherwise from the host copy.
+Field::Field(const Field& o)
+    : modes_(o.modes_), samples_(o.samples_), time_step_(o.time_step_), center_frequency_(o.center_frequency_),
+      data_(o.data_.size()) {
+    if (o.where_ == kDevice) {
+        const size_t bytes = data_.size() * sizeof(complex_t);
+        check(focss_device_alloc(runtime(), bytes, &dev_), "focss_device_alloc");
+        check(focss_device_copy(runtime(), dev_, o.dev_, bytes), "focss_device_copy");
+        where_ = kDevice;
+    } else {
+        data_ = o.data_;
+        where_ = kHost;
+    }
+}
+
+Field::Field(Field&& o) noexcept
+    : modes_(o.modes_), samples_(o.samples_), time_step_(o.time_step_), center_frequency_(o.center_frequency_),
+      data_(std::move(o.data_)), dev_(o.dev_), where_(o.where_) {
+    o.dev_ = nullptr;
+    o.where_ = kHost;
+    o.data_.clear();
+}
+
+Field& Field::operator=(const Field& o) {
+    if (this != &o) {
+        Field t(o);
+        *this = std::move(t);
+    }
+    return *this;
+}
+
+Field& Field::operator=(Field&& o) noexcept {
+    if (this != &o) {
+        release();
+        modes_ = o.modes_;
+        samples_ = o.samples_;
+        time_step_ = o.time_step_;
+        center_frequency_ = o.center_frequency_;
+        data_ = std::move(o.data_);
+        dev_ = o.dev_;
+        where_ = o.where_;
+        o.dev_ = nullptr;
+        o.where_ = kHost;
+        o.data_.clear();
+    }
+    return *this;
+}
+
+Field::~Field() { release(); }
+
+void Field::release() {
+    if (dev_) focss_device_free(runtime(), dev_);
+    dev_ = nullptr;
+}
+
+const void* Field::device_data() const {
+    const size_t bytes = data_.size() * sizeof(complex_t);
+    if (!dev_) check(focss_device_alloc(runtime(), bytes, &dev_), "focss_device_alloc");
+    if (where_ == kHost) {
+        check(focss_memcpy_h2d(runtime(), dev_, data_.data(), bytes), "focss_memcpy_h2d");
+        count_transfer();
+        where_ = kBoth;
+    }
+    return dev_;
+}
+
+void* Field::device_data() {
+    static_cast<const Field*>(this)->device_data();
+    where_ = kDevice;
+    return dev_;
+}
+
+void Field::to_host() const {
+    if (where_ != kDevice) return;
+    check(focss_memcpy_d2h(runtime(), data_.data(), dev_, data_.size() * sizeof(complex_t)), "focss_memcpy_d2h");
+    count_transfer();
+    where_ = kBoth;
+}
+
 ComplexVector Field::operator()(const int& mode) { return ComplexVector::proxy(samples_, raw() + samples_ * mode); }
 
 // power / peak_power / average_power                  field.cpp:164-187
 real_t Field::power(const int& sample) const {
+    const complex_t* d = host_data();
     real_t p = 0;
-    for (int i = sample; i < size(); i += samples_) p += std::norm(data_[i]);
+    for (int i = sample; i < size(); i += samples_) p += std::norm(d[i]);
     return p;
 }
 
@@ -42,8 +123,9 @@ real_t Field::peak_power() const {
 }
 
 real_t Field::average_power() const {
+    const complex_t* d = host_data();
     real_t sum = 0;
-    for (const complex_t& v : data_) sum += std::norm(v);
+    for (int i = 0; i < size(); ++i) sum += std::norm(d[i]);
     return sum / samples_;
 }
 
@@ -85,8 +167,9 @@ Field Field::upsample(const int& factor) const {
     d.time_step /= factor;
     Field out(d);
     complex_t* o = out.raw();
-    for (const complex_t& v : data_) {
-        *o = v;
+    const complex_t* in = raw();
+    for (int i = 0; i < size(); ++i) {
+        *o = in[i];
         o += factor;
     }
     return out;
@@ -99,8 +182,9 @@ Field Field::downsample(const int& factor) const {
     d.time_step *= factor;
     Field out(d);
     const complex_t* src = raw();
-    for (complex_t& v : out.data_) {
-        v = *src;
+    complex_t* o = out.raw();
+    for (int i = 0; i < out.size(); ++i) {
+        o[i] = *src;
         src += factor;
     }
     return out;
@@ -115,10 +199,8 @@ Field Field::decimate(const int& factor) const {
     d.samples /= factor;
     d.time_step *= factor;
     Field out(d);
-    DeviceBuffer in(raw(), sizeof(complex_t) * (size_t)size());
-    DeviceBuffer res(sizeof(complex_t) * (size_t)out.size());
-    check(focss_decimate(runtime(), FOCSS_F64, modes_, samples_, factor, in.ptr, res.ptr), "focss_decimate");
-    res.download(out.raw());
+    check(focss_decimate(runtime(), FOCSS_F64, modes_, samples_, factor, device_data(), out.device_data()),
+          "focss_decimate");
     return out;
 }
 
@@ -143,14 +225,14 @@ void device_fft(int sign, int modes, int samples, complex_t* data) {
     check(focss_device_free(ctx, d), "focss_device_free");
 }
 
-// forward with x1/N, inverse unscaled                   field.cpp:343-352
+// forward with x1/N, inverse unscaled, on the device copy  field.cpp:343-352
 Field& Field::fft_inplace() {
-    device_fft(-1, modes_, samples_, raw());
+    check(focss_fft(runtime(), FOCSS_F64, -1, modes_, samples_, device_data()), "focss_fft");
     return *this;
 }
 
 Field& Field::ifft_inplace() {
-    device_fft(+1, modes_, samples_, raw());
+    check(focss_fft(runtime(), FOCSS_F64, +1, modes_, samples_, device_data()), "focss_fft");
     return *this;
 }
 
@@ -161,62 +243,75 @@ Field Field::ifft() const { return Field(*this).ifft_inplace(); }
 // (field.cpp:358-372; an odd-length response loses its last tap, as in the
 // reference); on the GPU (focss_filter)
 Field& Field::filter_inplace(const ComplexVector& h) {
-    DeviceBuffer dev(raw(), sizeof(complex_t) * (size_t)size());
     check(focss_filter(runtime(), FOCSS_F64, modes_, samples_, h.size(), reinterpret_cast<const double*>(h.raw()),
-                       dev.ptr),
+                       device_data()),
           "focss_filter");
-    dev.download(raw());
     return *this;
 }
 
 Field Field::filter(const ComplexVector& h) const { return Field(*this).filter_inplace(h); }
 
-Field& Field::operator*=(const real_t& v) {
-    for (auto& x : data_) x *= v;
-    return *this;
-}
+// by a scalar on the device while the device copy is the current one
+// (a span loop's amplifier gain), else on the host     field.cpp:381-392
+Field& Field::operator*=(const real_t& v) { return (*this) *= complex_t(v, 0); }
 Field& Field::operator*=(const complex_t& v) {
-    for (auto& x : data_) x *= v;
+    if (where_ == kDevice) {
+        check(focss_field_scale(runtime(), FOCSS_F64, modes_, samples_, v.real(), v.imag(), device_data()),
+              "focss_field_scale");
+        return *this;
+    }
+    complex_t* d = host_data_mut();
+    for (int i = 0; i < size(); ++i) d[i] *= v;
     return *this;
 }
 Field& Field::operator*=(const RealVector& v) {
     assert(v.size() == samples_);
+    complex_t* d = host_data_mut();
     for (int m = 0; m < modes_; ++m)
-        for (int s = 0; s < samples_; ++s) data_[s + m * samples_] *= v[s];
+        for (int s = 0; s < samples_; ++s) d[s + m * samples_] *= v[s];
     return *this;
 }
 Field& Field::operator*=(const ComplexVector& v) {
     assert(v.size() == samples_);
+    complex_t* d = host_data_mut();
     for (int m = 0; m < modes_; ++m)
-        for (int s = 0; s < samples_; ++s) data_[s + m * samples_] *= v[s];
+        for (int s = 0; s < samples_; ++s) d[s + m * samples_] *= v[s];
     return *this;
 }
 Field& Field::operator/=(const real_t& v) {
-    for (auto& x : data_) x /= v;
+    complex_t* d = host_data_mut();
+    for (int i = 0; i < size(); ++i) d[i] /= v;
     return *this;
 }
 Field& Field::operator/=(const complex_t& v) {
-    for (auto& x : data_) x /= v;
+    complex_t* d = host_data_mut();
+    for (int i = 0; i < size(); ++i) d[i] /= v;
     return *this;
 }
 Field& Field::operator/=(const RealVector& v) {
     assert(v.size() == samples_);
+    complex_t* d = host_data_mut();
     for (int m = 0; m < modes_; ++m)
-        for (int s = 0; s < samples_; ++s) data_[s + m * samples_] /= v[s];
+        for (int s = 0; s < samples_; ++s) d[s + m * samples_] /= v[s];
     return *this;
 }
 Field& Field::operator/=(const ComplexVector& v) {
     assert(v.size() == samples_);
+    complex_t* d = host_data_mut();
     for (int m = 0; m < modes_; ++m)
-        for (int s = 0; s < samples_; ++s) data_[s + m * samples_] /= v[s];
+        for (int s = 0; s < samples_; ++s) d[s + m * samples_] /= v[s];
     return *this;
 }
 Field& Field::operator+=(const Field& o) {
-    for (int i = 0; i < size(); ++i) data_[i] += o.data_[i];
+    const complex_t* b = o.host_data();
+    complex_t* d = host_data_mut();
+    for (int i = 0; i < size(); ++i) d[i] += b[i];
     return *this;
 }
 Field& Field::operator-=(const Field& o) {
-    for (int i = 0; i < size(); ++i) data_[i] -= o.data_[i];
+    const complex_t* b = o.host_data();
+    complex_t* d = host_data_mut();
+    for (int i = 0; i < size(); ++i) d[i] -= b[i];
     return *this;
 }
 

@@ -1,9 +1,18 @@
 // focss::Domain / focss::Field -- the signal container of the focss C++ API
 // with the reference's interface (src/focss/field.h:20-162).  Storage is the
 // reference's layout: mode-major complex<double>, (mode, sample) at
-// data[sample + mode * samples] (src/focss/field.cpp:140-155).  The data
-// stays in host memory as in the reference; transforms run on the GPU
-// through the focss C-ABI (include/focss.h).
+// data[sample + mode * samples] (src/focss/field.cpp:140-155).
+//
+// A Field holds a host copy and, once a device operation has touched it, a
+// device copy, each current or stale.  The operations that run on the GPU
+// (SSFM::run, fft_inplace / ifft_inplace, filter_inplace, decimate,
+// Amplifier::add_noise, and *= by a scalar while the device copy is the
+// current one) work on the device copy and leave the host copy stale; any
+// host access (operator[], operator(), raw(), the power and arithmetic
+// members) first brings the host copy up to date, and non-const host access
+// marks the device copy stale.  A span loop of device operations therefore
+// crosses PCIe once each way, however many spans it runs.  A pointer from
+// raw() stays valid for host writes until the next device operation.
 #ifndef FOCSS_FIELD_H_
 #define FOCSS_FIELD_H_
 
@@ -31,16 +40,16 @@ class Field {
     Field();
     explicit Field(const Domain& domain);
     explicit Field(const Domain& domain, const ComplexVector& data);
-    Field(const Field&) = default;
-    Field(Field&&) = default;
-    Field& operator=(const Field&) = default;
-    Field& operator=(Field&&) = default;
-    ~Field() = default;
+    Field(const Field&);
+    Field(Field&&) noexcept;
+    Field& operator=(const Field&);
+    Field& operator=(Field&&) noexcept;
+    ~Field();
 
-    complex_t operator[](const int& index) const { return data_[index]; }
-    complex_t operator()(const int& mode, const int& sample) const { return data_[sample + mode * samples_]; }
-    complex_t& operator[](const int& index) { return data_[index]; }
-    complex_t& operator()(const int& mode, const int& sample) { return data_[sample + mode * samples_]; }
+    complex_t operator[](const int& index) const { return host_data()[index]; }
+    complex_t operator()(const int& mode, const int& sample) const { return host_data()[sample + mode * samples_]; }
+    complex_t& operator[](const int& index) { return host_data_mut()[index]; }
+    complex_t& operator()(const int& mode, const int& sample) { return host_data_mut()[sample + mode * samples_]; }
     ComplexVector operator()(const int& mode);
 
     real_t power(const int& sample) const;
@@ -93,15 +102,39 @@ class Field {
     Field& operator+=(const Field&);
     Field& operator-=(const Field&);
 
-    complex_t* raw() { return data_.data(); }
-    const complex_t* raw() const { return data_.data(); }
+    complex_t* raw() { return host_data_mut(); }
+    const complex_t* raw() const { return host_data(); }
+
+    // Device residency (no counterpart in the reference's interface): the
+    // device copy (complex<double>, the same layout), current on return.
+    // The non-const form is for writing on the device: it marks the host
+    // copy stale.  to_host() brings the host copy up to date.
+    void* device_data();
+    const void* device_data() const;
+    void to_host() const;
+    bool host_current() const { return where_ != kDevice; }
+    bool device_current() const { return where_ != kHost; }
 
   private:
+    enum { kHost = 0, kDevice = 1, kBoth = 2 };  // which copies are current
+    const complex_t* host_data() const {
+        if (where_ == kDevice) to_host();
+        return data_.data();
+    }
+    complex_t* host_data_mut() {
+        if (where_ == kDevice) to_host();
+        where_ = kHost;
+        return data_.data();
+    }
+    void release();
+
     int modes_;
     int samples_;
     real_t time_step_;
     real_t center_frequency_;
-    std::vector<complex_t> data_;
+    mutable std::vector<complex_t> data_;
+    mutable void* dev_ = nullptr;
+    mutable int where_ = kHost;
 };
 
 Field operator*(const Field&, const real_t&);

@@ -22,19 +22,12 @@ double Amplifier::noise_sigma(const Field& field) const {
     return std::sqrt(variance / 2);
 }
 
-// AWGN realisation on the device: Philox stream = the runtime's next
-// stream number under the runtime's seed            amplifier.cpp:20-28
+// AWGN realisation on the field's device copy: Philox stream = the
+// runtime's next stream number under the runtime's seed  amplifier.cpp:20-28
 void Amplifier::add_noise(Field& field) const {
-    focss_ctx* ctx = runtime();
-    size_t bytes = (size_t)field.size() * sizeof(complex_t);
-    void* d = nullptr;
-    check(focss_device_alloc(ctx, bytes, &d), "focss_device_alloc");
-    check(focss_memcpy_h2d(ctx, d, field.raw(), bytes), "focss_memcpy_h2d");
-    check(focss_add_noise(ctx, FOCSS_F64, field.modes(), field.samples(), noise_sigma(field), noise_seed(),
-                          next_noise_stream(), d),
+    check(focss_add_noise(runtime(), FOCSS_F64, field.modes(), field.samples(), noise_sigma(field), noise_seed(),
+                          next_noise_stream(), field.device_data()),
           "focss_add_noise");
-    check(focss_memcpy_d2h(ctx, field.raw(), d, bytes), "focss_memcpy_d2h");
-    check(focss_device_free(ctx, d), "focss_device_free");
 }
 
 void Amplifier::give_power(Field& field) const {

@@ -15,7 +15,11 @@ thread_local Holder holder;
 int g_precision = FOCSS_F64;
 uint64_t g_seed = 0xF0C55000ull;
 uint32_t g_stream = 0;
+thread_local uint64_t g_transfers = 0;
 }  // namespace
+
+void count_transfer() { ++g_transfers; }
+uint64_t transfer_count() { return g_transfers; }
 
 void check(int status, const char* what) {
     if (status != 0) {

@@ -18,6 +18,11 @@ void check(int status, const char* what);
 void set_precision(int precision);
 int precision();
 
+// Host<->device copies of Field data made so far on this thread (each
+// upload or download of a field counts one)
+void count_transfer();
+uint64_t transfer_count();
+
 // ASE realisations: Philox key and the stream counter of Amplifier::add_noise
 void set_noise_seed(uint64_t seed);
 uint64_t noise_seed();

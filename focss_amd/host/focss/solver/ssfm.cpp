@@ -48,8 +48,19 @@ focss_link SSFM::link(const Field& field, const double& direction, const int& n_
 // ssfm.cpp:39-60 (grids 62-118) on the device
 void SSFM::run(Field& field, const Direction& direction) {
     focss_link l = link(field, direction, 1);
-    check(focss_link_run_host(runtime(), precision(), &l, reinterpret_cast<double*>(field.raw()), &last_steps_),
-          "focss_link_run_host");
+    run_link_program(field, l, &last_steps_);
     if (grid_ == ADAPTIVE) std::cout << "    * SSFM done " << last_steps_ << " steps" << std::endl;
+}
+
+// FP64 programs run on the field's device copy, which stays current; FP32
+// programs convert through the host copy (focss_link_run_host)
+void run_link_program(Field& field, const focss_link& l, int* steps) {
+    if (precision() == FOCSS_F64) {
+        const int64_t offset = 0;
+        check(focss_link_run(runtime(), FOCSS_F64, 1, &l, field.device_data(), &offset, steps), "focss_link_run");
+    } else {
+        check(focss_link_run_host(runtime(), precision(), &l, reinterpret_cast<double*>(field.raw()), steps),
+              "focss_link_run_host");
+    }
 }
 }  // namespace focss

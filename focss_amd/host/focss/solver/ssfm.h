@@ -45,6 +45,10 @@ class SSFM {
     double maximum_step_size_;
     int last_steps_;
 };
+
+// Runs one link program on `field` at the runtime's precision: FP64 on the
+// field's device copy (which stays current), FP32 through the host copy.
+void run_link_program(Field& field, const focss_link& link, int* total_steps);
 }  // namespace focss
 
 #endif  // FOCSS_SOLVER_SSFM_H_

@@ -29,6 +29,7 @@
 //   focss_tasks load-transmission <csv> <tx> <rx> <modes> <samples>
 // <in>/<out>/<tx>/<rx> are raw little-endian complex<double> files in the
 // field layout.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -43,9 +44,11 @@
 #include "focss/equalizer/wdm_sino.h"
 #include "focss/field.h"
 #include "focss/functions.h"
+#include "focss/module/amplifier.h"
 #include "focss/processing/modulation.h"
 #include "focss/runtime.h"
 #include "tasks/field_io.h"
+#include "focss/solver/ssfm.h"
 #include "tasks/soliton_check.h"
 #include "tasks/systems.h"
 
@@ -112,6 +115,28 @@ int main(int argc, char** argv) {
             int steps = focss::forward_propagation(f, p);
             std::cout << "split steps " << steps << std::endl;
         }
+        write_field(argv[3], f);
+        return 0;
+    }
+    // the reference's per-span calling pattern (src/tasks/wdm_run.cpp:26-47):
+    // for each span SSFM::run, then Amplifier::amplify with ASE; prints the
+    // host<->device copies of the field the loop made
+    if (cmd == "span-loop" && argc == 17) {
+        int samples = std::atoi(argv[4]), modes = std::atoi(argv[5]);
+        double dt = std::atof(argv[6]);
+        int spans = std::atoi(argv[7]), steps = std::atoi(argv[8]);
+        focss::Field f = read_field(argv[2], samples, modes, dt, std::atof(argv[16]));
+        focss::LinkParameters p = params(argv + 9);
+        focss::reseed_global_urng(std::strtoull(argv[15], nullptr, 0));
+        focss::SSFM ssfm(p.fiber, steps);
+        focss::Amplifier amp(std::exp(p.fiber.alpha * p.fiber.length), focss::db_to_linear(std::atof(argv[14])));
+        const uint64_t before = focss::transfer_count();
+        for (int s = 0; s < spans; ++s) {
+            ssfm.run(f, focss::SSFM::FORWARD);
+            amp.amplify(f);
+        }
+        f.to_host();
+        std::cout << "transfers " << (focss::transfer_count() - before) << std::endl;
         write_field(argv[3], f);
         return 0;
     }

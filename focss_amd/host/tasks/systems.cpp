@@ -28,8 +28,7 @@ double span_gain(const Fiber& f) { return std::exp(f.alpha * f.length); }  // db
 
 int run_link(Field& signal, focss_link& l) {
     int steps = 0;
-    check(focss_link_run_host(runtime(), precision(), &l, reinterpret_cast<double*>(signal.raw()), &steps),
-          "focss_link_run_host");
+    run_link_program(signal, l, &steps);
     return steps;
 }
 }  // namespace

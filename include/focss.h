@@ -190,6 +190,11 @@ int focss_fft_noalloc(focss_ctx* ctx, int precision, int sign, int modes, int sa
 int focss_add_noise(focss_ctx* ctx, int precision, int modes, int samples, double sigma,
                     uint64_t seed, uint32_t stream, void* data);
 
+/* Field::operator*=(real / complex) (src/focss/field.cpp:381-392) on a
+ * device field: every element of the modes x samples field times
+ * (re + i im).  Stream-ordered. */
+int focss_field_scale(focss_ctx* ctx, int precision, int modes, int samples, double re, double im, void* data);
+
 /* ---- perturbation-triplet equalizer ----
  * Weights W are (2R+1)^2 complex<double> (interleaved re, im), row-major:
  * W(m, n) at index (R + m) * (2R + 1) + (R + n).  Fields are device

@@ -91,6 +91,32 @@ def test_forward_propagation_with_ase(tmp_path):
     assert rel_l2(y, ref) <= 1e-10
 
 
+@pytest.mark.parametrize("modes", [1, 2])
+def test_span_loop_stays_on_device(tmp_path, modes):
+    """The reference's per-span loop (wdm_run.cpp:26-47): SSFM::run then
+    Amplifier::amplify with ASE, span after span, on one focss::Field.  The
+    field crosses PCIe once each way however many spans run, and the result
+    matches the oracle span by span (SSFM, x sqrt(G), ASE of stream s)."""
+    n, spans, steps = 1 << 12, 4, 3
+    f = ssmf(kappa=8 / 9 if modes == 2 else 1.0)
+    x, _, dt = qam_waveform(77 + modes, n, power_dbm=2.0, modes=modes)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(src)
+    seed, nf_db = 0x5EED, 5.0
+    out = run(["span-loop", src, dst, n, modes, repr(dt), spans, steps] + fib_args(f) + [nf_db, seed, repr(F_C)],
+              tmp_path)
+    y = np.fromfile(dst, dtype=np.complex128)
+    gain = np.exp(f["alpha"] * f["length"])
+    ref = x.astype(np.complex128)
+    for s in range(spans):
+        ref, _ = O.ssfm_run(O.fiber(**f), O.grid(O.UNIFORM, steps), O.FORWARD, ref, dt, modes=modes)
+        ref = O.amplify(ref, gain, O.db_to_linear(nf_db), F_C, dt, seed, s, modes=modes)
+    err = rel_l2(y, ref)
+    print(f"span loop, {modes} mode(s), {spans} spans: {out.strip()}, rel L2 vs oracle {err:.2e}")
+    assert "transfers 2" in out
+    assert err <= 1e-10
+
+
 def test_sino_class(tmp_path):
     """focss::SINO train + equalize against the oracle's train + cubic."""
     radius, n, modes = 5, 1200, 1
