@@ -702,11 +702,12 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             xs[md * NQ + q] = h;
         }
     __syncthreads();
+    // the matrix-core operands of step j, loaded during step j - 1's phase
+    ldbp_h8 A[NKS];
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) A[s] = afrag[s * 64 + lane];
     for (int j = 0; j < a.J; ++j) {
         // FIR on the matrix cores
-        ldbp_h8 A[NKS];
-#pragma unroll
-        for (int s = 0; s < NKS; ++s) A[s] = afrag[(j * NKS + s) * 64 + lane];
         ldbp_f16 acc[2];
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
@@ -749,6 +750,11 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             }
         }
         __syncthreads();
+        // the next step's operands, in flight during this step's phase
+        if (j + 1 < a.J) {
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) A[s] = afrag[((j + 1) * NKS + s) * 64 + lane];
+        }
         // nonlinear phase, one thread per block of 16 samples
         const bool last = j == a.J - 1;
         float g[KP];
@@ -804,40 +810,65 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
                     pe[P + 16 + i] = hi[i];
                 }
             }
+            // eight phases at a time, then one wave-uniform choice of the
+            // rotation: every phase within 1/8 rad (the learned models'
+            // per-step phases) takes a polynomial (sin to x^5, cos to x^4:
+            // below 1e-8 off), within 1000 rad the hardware sine and cosine
+            // (v_sin_f32 / v_cos_f32, ~1e-6), else the reduced series
 #pragma unroll
-            for (int rq = 0; rq < 4; ++rq) {
-                float sn[4], cs[4];
+            for (int half = 0; half < 2; ++half) {
+                float sn[8], cs[8], mx = 0.0f;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int r = 4 * rq + e;
+                for (int u = 0; u < 8; ++u) {
+                    const int r = 8 * half + u;
                     float phi = 0.0f;
 #pragma unroll
                     for (int mm = 0; mm < KP; ++mm) phi = fmaf(g[mm], pe[r - mm + 2 * P], phi);
-                    // the hardware sine and cosine (v_sin_f32 / v_cos_f32, ~1e-6
-                    // absolute): the field itself is held in FP16 here
-                    if (fabsf(phi) <= 1000.0f) {
-                        sn[e] = __sinf(phi);
-                        cs[e] = __cosf(phi);
-                    } else {
-                        sincos_fast(phi, &sn[e], &cs[e]);
+                    sn[u] = phi;
+                    mx = fmaxf(mx, fabsf(phi));
+                }
+                if (__all(mx <= 0.125f)) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float x = sn[u], x2 = x * x;
+                        sn[u] = fmaf(x * x2, fmaf(x2, 0.008333333333333333f, -0.16666666666666666f), x);
+                        cs[u] = fmaf(x2, fmaf(x2, 0.041666666666666664f, -0.5f), 1.0f);
+                    }
+                } else if (__all(mx <= 1000.0f)) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float x = sn[u];
+                        sn[u] = __sinf(x);
+                        cs[u] = __cosf(x);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float x = sn[u];
+                        sincos_fast(x, &sn[u], &cs[u]);
                     }
                 }
 #pragma unroll
-                for (int md = 0; md < M; ++md) {
-                    const int q = md * NQ + (rq << lnb) + blk;
-                    cx_h4 yh = xs[q];
-                    cx<float> z[4];
+                for (int rh = 0; rh < 2; ++rh) {
+                    const int rq = 2 * half + rh;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        z[e] = mk<float>((float)yh.s[e].x, (float)yh.s[e].y) * mk<float>(cs[e], sn[e]);
-                    if (last) {
-                        cx<float>* o = a.out + ((long long)b * M + md) * N + 16 * blk + 4 * rq;
+                    for (int md = 0; md < M; ++md) {
+                        const int q = md * NQ + (rq << lnb) + blk;
+                        cx_h4 yh = xs[q];
+                        cx<float> z[4];
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) o[e] = z[e];
-                    } else {
+                        for (int e = 0; e < 4; ++e)
+                            z[e] = mk<float>((float)yh.s[e].x, (float)yh.s[e].y) *
+                                   mk<float>(cs[4 * rh + e], sn[4 * rh + e]);
+                        if (last) {
+                            cx<float>* o = a.out + ((long long)b * M + md) * N + 16 * blk + 4 * rq;
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) yh.s[e] = cx_h{(_Float16)z[e].x, (_Float16)z[e].y};
-                        xs[q] = yh;
+                            for (int e = 0; e < 4; ++e) o[e] = z[e];
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) yh.s[e] = cx_h{(_Float16)z[e].x, (_Float16)z[e].y};
+                            xs[q] = yh;
+                        }
                     }
                 }
             }
