@@ -306,10 +306,11 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
     };
     const auto tw_t = TW::template load<TN>(tw_a, tw_b, tb.bank);
     static_assert(TPB == 1 || TPB == 2, "one or two tiles per workgroup");
-    // the inter-pass twiddles formed again at the store (single-mode columns,
-    // or two tiles) or held from the load (two modes: 88 against 111 us for
-    // 1024 x 2 x 2 complex128)
-    constexpr bool kRecompute = M == 1 || TPB > 1;
+    // the inter-pass twiddles formed again at the store (single-mode FP64
+    // columns, or two tiles) or held from the load (two modes: 88 against
+    // 111 us for 1024 x 2 x 2 complex128; FP32, where eight twiddles are 16
+    // registers: 78-79 against 83 us for 512 x 4 complex64)
+    constexpr bool kRecompute = (M == 1 && sizeof(R) == 8) || TPB > 1;
     // (a loop here allocates fewer registers than the compile-time tile
     // calls of the spectral pass)
     cx<R> v[PER][M][8];
