@@ -21,6 +21,20 @@
 // once, after the program's last time op.  Both kernels use the split-stage
 // DFTs of the SSFM passes (first and last radix-8 stage in registers, one
 // LDS round trip per middle stage).
+//
+// UNIFORM programs, whose spectral ops apply at most two distinct step
+// lengths s, skip the chirps: the spectral op is the length-N circular
+// convolution with d_s = IDFT_N(D_s), which equals the length-M circular
+// convolution of the zero-padded field with K_s = [d_0 .. d_{N-1}, 0 .. 0,
+// d_1 .. d_{N-1}] on its first N values (M >= 2N - 1, so the wrapped
+// terms never overlap).  With Khat_s = DFT_M(K_s) / M tabulated in row
+// order, a split step is two launches:
+//   spectral op    k_blu_rows<PAD>: plan (as the spectral pass), forward
+//                  row DFT -> x Khat_s x gain e^{-alpha s / 2} -> inverse
+//                  row DFT
+//   time op        k_blu_cols<PAD_OUT>: column IDFT -> the op on e < N,
+//                  zero beyond -> column DFT (PAD_ENTRY at entry)
+// and about 5 M values move per split step.
 #pragma once
 #include "bluestein_kernels.h"
 #include "fft_lds.h"
@@ -28,7 +42,7 @@
 
 namespace focss_hip {
 
-enum { BLU_ENTRY = 0, BLU_MID = 1, BLU_OUT = 2 };
+enum { BLU_ENTRY = 0, BLU_MID = 1, BLU_OUT = 2, BLU_PAD_ENTRY = 3, BLU_PAD_OUT = 4 };
 
 constexpr int kBluMinLog = 12;  // smaller Bluestein lengths run the k_gen_* kernels
 
@@ -60,9 +74,10 @@ struct BluShape {
 // spectral pass: forward row DFT N2 -> x Bhat (row order) -> inverse row
 // DFT N2 -> x W_M^{-n2 k1}.  after_mid: the second convolution of the
 // spectral op (runs when the op is active), else the first (runs while the
-// program has not ended).  A work item (s, j < N2/8) owns the values
-// j + r N2/8 of row s.
-template <typename R, int LOGM>
+// program has not ended).  PAD: the whole spectral op of a UNIFORM program,
+// planned here, with Khat_s and the op's scalar in place of Bhat.  A work
+// item (s, j < N2/8) owns the values j + r N2/8 of row s.
+template <typename R, int LOGM, bool PAD>
 __global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::S * BluShape<R, LOGM>::N2 / 8)) void k_blu_rows(
     BluArgs<R> g, int after_mid, Tables<R> tb) {
     using Sh = BluShape<R, LOGM>;
@@ -79,15 +94,40 @@ __global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::S * BluShape<R, LOG
     const int m = t / (N1 / S);
     const int k1_0 = (t % (N1 / S)) * S;
     const int w_id = g.p.waves[k];
-    if (after_mid) {
-        if (!g.active[w_id]) return;
-    } else if (g.p.states[(g.p.counter & 1) * g.p.n_waves + w_id].phase == PH_DONE) {
-        return;
+    const cx<R>* bh;
+    R kscale = 1;
+    if (PAD) {
+        const PassArgs& a = g.p;
+        WaveState st = a.states[(a.counter & 1) * a.n_waves + w_id];
+        const WaveStatic w = a.statics[w_id];
+        const int was_done = st.phase == PH_DONE;
+        const Reduction rd = a.red[(a.counter % 3) * a.n_waves + w_id];
+        const SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), rd.sum);
+        if (t == 0 && threadIdx.x == 0) {
+            a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
+            Reduction z;
+            z.peak_bits = 0;
+            z.sum = 0;
+            a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
+            if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
+        }
+        if (!op.active) return;
+        // the table of this step length (the host tabulated every length the
+        // program applies)
+        const GenWave<R>& gw = g.gw[w_id];
+        const int sel = fabs(op.s - gw.kval[1]) < fabs(op.s - gw.kval[0]);
+        bh = gw.khat[sel] + (long long)k1_0 * N2;
+        kscale = (R)(op.gain * exp(-0.5 * w.alpha * op.s));
+    } else {
+        if (after_mid) {
+            if (!g.active[w_id]) return;
+        } else if (g.p.states[(g.p.counter & 1) * g.p.n_waves + w_id].phase == PH_DONE) {
+            return;
+        }
+        bh = g.gw[w_id].bhat_rows + (long long)k1_0 * N2;
     }
-    const GenWave<R> gw = g.gw[w_id];
     const int modes = g.p.statics[w_id].modes;
     cx<R>* rows = g.work + ((long long)k * modes + m) * M + (long long)k1_0 * N2;
-    const cx<R>* bh = gw.bhat_rows + (long long)k1_0 * N2;
     const auto tw = TW::template load<TN>(tw_a, tw_b, tb.bank);
     cx<R> v[PER][8];
 #pragma unroll
@@ -109,7 +149,10 @@ __global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::S * BluShape<R, LOG
             const int s = id % S, j = id / S;
             last_stage_load<R, N2, S, -1>(buf, tw, v[b], s, j);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) v[b][r] = v[b][r] * bh[s * N2 + j + r * Q];
+            for (int r = 0; r < 8; ++r) {
+                const cx<R> f = bh[s * N2 + j + r * Q];
+                v[b][r] = v[b][r] * (PAD ? f * kscale : f);
+            }
         }
     }
     __syncthreads();
@@ -187,13 +230,15 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
     } else if (st.top == TOP_NONE) {
         return;
     }
+    constexpr bool PAD = KIND == BLU_PAD_ENTRY || KIND == BLU_PAD_OUT;
+    constexpr bool ENTRY = KIND == BLU_ENTRY || KIND == BLU_PAD_ENTRY;
     const GenWave<R> gw = g.gw[w_id];
     const int n = gw.n;
     cx<R>* rows = g.work + (long long)k * MODES * M;
     cx<R>* x = reinterpret_cast<cx<R>*>(a.field) + w.offset;
     const auto tw = TW::template load<TN>(tw_a, tw_b, tb.bank);
     cx<R> v[PER][MODES][8];
-    if (KIND == BLU_ENTRY) {
+    if (ENTRY) {
 #pragma unroll
         for (int b = 0; b < PER; ++b) {
             const int id = threadIdx.x + b * TN;
@@ -271,16 +316,16 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
                 for (int r = 0; r < 8; ++r) {
                     const int e = (j + r * Q) * N2 + n2_0 + c;
                     if (e < n) {
-                        const cx<R> ce = gw.chirp[e];
+                        const cx<R> ce = PAD ? mk<R>(1, 0) : gw.chirp[e];
                         cx<R> u[MODES];
 #pragma unroll
                         for (int mm = 0; mm < MODES; ++mm)
-                            u[mm] = KIND == BLU_ENTRY ? v[b][mm][r] : conj(ce * v[b][mm][r]);
+                            u[mm] = ENTRY || PAD ? v[b][mm][r] : conj(ce * v[b][mm][r]);
                         time_op<R, MODES>(u, w, st, e, pk, sm);
 #pragma unroll
                         for (int mm = 0; mm < MODES; ++mm) {
                             if (last) x[(long long)mm * n + e] = u[mm];
-                            v[b][mm][r] = u[mm] * ce;
+                            v[b][mm][r] = PAD ? u[mm] : u[mm] * ce;
                         }
                     } else {
 #pragma unroll
@@ -337,6 +382,44 @@ __global__ void k_blu_row_order(const cx<R>* __restrict__ in, cx<R>* __restrict_
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
         const int k1 = i / N2, k2 = i % N2;
         out[i] = in[k1 + N1 * k2];
+    }
+}
+
+// Building Khat_s for a UNIFORM program (FP64): D_s(k) = exp(i arg w_k^2),
+// arg = beta2 s / 2, w_k = kk dw with kk in (-N/2, N/2] (the spectral
+// factor of the passes without loss and gain)
+static __global__ void k_pad_disp(cx<double>* __restrict__ d, int n, double arg, double dw) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int kk = k <= n / 2 ? k : k - n;
+    const double om = (double)kk * dw;
+    double sn, cs;
+    sincos(arg * (om * om), &sn, &cs);
+    d[k] = mk<double>(cs, sn);
+}
+
+// K (M values) from the unscaled inverse DFT u = N d of D_s:
+// K_j = d_j (j < N), K_{M-j} = d_{N-j} (0 < j < N), 0 between
+static __global__ void k_pad_kernel(const cx<double>* __restrict__ u, cx<double>* __restrict__ kv, int n, int log_m) {
+    const int M = 1 << log_m;
+    const double inv_n = 1.0 / n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        cx<double> v = mk<double>(0, 0);
+        if (i < n)
+            v = u[i] * inv_n;
+        else if (i > M - n)
+            v = u[i - (M - n)] * inv_n;
+        kv[i] = v;
+    }
+}
+
+// Khat in the row order of the split, at the field's precision
+template <typename R>
+__global__ void k_pad_rows(const cx<double>* __restrict__ in, cx<R>* __restrict__ out, int log_m, int log_n1) {
+    const int M = 1 << log_m, N2 = M >> log_n1, N1 = 1 << log_n1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const cx<double> v = in[(i / N2) + N1 * (i % N2)];
+        out[i] = mk<R>((R)v.x, (R)v.y);
     }
 }
 

@@ -28,6 +28,10 @@ struct GenWave {
     const cx<R>* chirp;      // c_j, j < N
     const cx<R>* bhat;       // Bhat_k, k < M
     const cx<R>* bhat_rows;  // Bhat in the row order of the fused kernels (M >= 2^12)
+    // padded-convolution UNIFORM programs: Khat_s = DFT_M(K_s) / M in row
+    // order for the step lengths kval[0], kval[1] (signed, as SpectralOp::s)
+    const cx<R>* khat[2];
+    double kval[2];
     long long row0;          // its first row of the work buffer (modes rows of M)
     int n;                   // samples per mode
     int pad;

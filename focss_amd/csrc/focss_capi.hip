@@ -6,10 +6,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -116,6 +118,13 @@ struct focss_ctx {
     size_t blu_work_bytes = 0;
     void* gen_work = nullptr;  // link programs of generic length: rows of the Bluestein convolutions
     size_t gen_work_bytes = 0;
+    // padded-convolution kernels of UNIFORM generic-length programs:
+    // (length, precision, beta2, s, dw as bits) -> Khat_s in row order
+    std::map<std::tuple<int, int, unsigned long long, unsigned long long, unsigned long long>, void*> pad;
+    size_t pad_bytes = 0;
+    void* pad_tmp = nullptr;  // FP64 staging of one table (N + M values)
+    size_t pad_tmp_bytes = 0;
+    bool pad_uniform = true;  // FOCSS_BLUESTEIN_UNIFORM=1: UNIFORM programs through Bluestein too
     void* d_genwave = nullptr;  // [cap] GenWave
     int* d_active = nullptr;    // [cap]
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
@@ -565,8 +574,10 @@ int fft_impl(focss_ctx* c, int sign, int modes, int samples, void* data, void* w
 struct GenClass {
     int log_m, modes;
     int fused;         // log_m >= kBluMinLog: the four-launch k_blu_* kernels
+    int padded;        // fused UNIFORM programs: the two-launch padded convolution
     int first, count;  // into the waves array
     int max_n;         // longest waveform of the class
+    int unused = 0;
     long long work_off;  // its rows' first value in gen_work
 };
 
@@ -599,7 +610,7 @@ BluArgs<R> blu_args(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
 
 // the spectral op of counter a.counter: two Bluestein DFTs around the
 // dispersion multiply (whole-row DFT kernels below 2^kBluMinLog, else the
-// fused row / column kernels)
+// fused row / column kernels), or one padded convolution (one row launch)
 template <typename R>
 void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb) {
     if (gc.fused) {
@@ -607,6 +618,10 @@ void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const 
         BluArgs<R> g = blu_args<R>(c, gc, a);
         g.tiles = gc.modes * r.row_tiles[gc.log_m];
         const dim3 rows_grid(gc.count * g.tiles), rows_block(r.row_threads[gc.log_m]);
+        if (gc.padded) {
+            hipLaunchKernelGGL(r.rows_pad[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 0, tb);
+            return;
+        }
         hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 0, tb);
         BluArgs<R> gm = g;
         gm.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
@@ -636,7 +651,8 @@ void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a, const 
         const BluRegistry<R>& r = blu_reg<R>();
         BluArgs<R> g = blu_args<R>(c, gc, a);
         g.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
-        hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][entry ? BLU_ENTRY : BLU_OUT], dim3(gc.count * g.tiles),
+        const int kind = gc.padded ? (entry ? BLU_PAD_ENTRY : BLU_PAD_OUT) : (entry ? BLU_ENTRY : BLU_OUT);
+        hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][kind], dim3(gc.count * g.tiles),
                            dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, c->stream, g, tb);
         return;
     }
@@ -646,11 +662,84 @@ void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a, const 
     hipLaunchKernelGGL(k, dim3(gc.count * g.p.tiles_per_wave), dim3(256), 0, c->stream, g);
 }
 
-// Bluestein tables, GenWave records and work rows of the generic classes
+// The signed step lengths of the spectral ops of a UNIFORM program (at most
+// two: h / 2 at a program's or ASE span's ends, h between), from the device's
+// own planner run on the host; false when the program is not UNIFORM, is one
+// linear operator (kappa gamma = 0), or applies more lengths
+bool uniform_step_lengths(const WaveStatic& w, double kv[2]) {
+    if (w.grid != GRID_UNIFORM || w.linear_only) return false;
+    WaveState st = initial_state(w);
+    int nv = 0;
+    const long long limit = (long long)w.n_spans * ((long long)w.steps + 2) + 4;
+    for (long long it = 0; it < limit; ++it) {
+        const SpectralOp op = plan_spectral(w, st, 0.0, 0.0);  // UNIFORM plans ignore the power
+        if (!op.active) {
+            if (nv == 1) kv[1] = kv[0];
+            return nv > 0;
+        }
+        int j = 0;
+        while (j < nv && kv[j] != op.s) ++j;
+        if (j == nv) {
+            if (nv == 2) return false;
+            kv[nv++] = op.s;
+        }
+    }
+    return false;
+}
+
+constexpr size_t kPadCacheBytes = (size_t)8 << 30;  // tables kept across calls, beyond which they are rebuilt
+
+// Khat_s = DFT_M(K_s) / M in row order at precision R for a UNIFORM program
+// of length n (K_s from d_s = IDFT_n(D_s), all in FP64), built on the
+// context stream once per (length, precision, beta2, s, dw) and kept
+template <typename R>
+int pad_table(focss_ctx* c, int n, int log_m, double beta2, double s, double dw, const cx<R>** out) {
+    const int prec = sizeof(R) == 8 ? FOCSS_F64 : FOCSS_F32;
+    const auto key = std::make_tuple(n, prec, double_to_bits(beta2), double_to_bits(s), double_to_bits(dw));
+    auto it = c->pad.find(key);
+    if (it == c->pad.end()) {
+        const int M = 1 << log_m;
+        if (int rc = grow(c, &c->pad_tmp, &c->pad_tmp_bytes, ((size_t)n + M) * sizeof(cx<double>))) return rc;
+        cx<double>* u = (cx<double>*)c->pad_tmp;
+        cx<double>* kv = u + n;
+        void* t = nullptr;
+        HIP_TRY(dev_malloc(&t, (size_t)M * sizeof(cx<R>)));
+        hipLaunchKernelGGL(k_pad_disp, dim3((n + 255) / 256), dim3(256), 0, c->stream, u, n, 0.5 * beta2 * s, dw);
+        int rc = fft_enqueue<double>(c, +1, 1, n, u);  // n d
+        if (!rc) {
+            hipLaunchKernelGGL(k_pad_kernel, dim3(std::min(M / 256, 4096)), dim3(256), 0, c->stream,
+                               (const cx<double>*)u, kv, n, log_m);
+            rc = fft_enqueue<double>(c, -1, 1, M, kv);  // DFT_M(K) / M
+        }
+        if (!rc)
+            hipLaunchKernelGGL(k_pad_rows<R>, dim3(std::min(M / 256, 4096)), dim3(256), 0, c->stream,
+                               (const cx<double>*)kv, (cx<R>*)t, log_m, blu_log_n1(log_m));
+        const hipError_t e = hipGetLastError();
+        if (rc || e != hipSuccess) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(t);
+            return rc ? rc : fail(FOCSS_E_HIP, std::string("padded-convolution table: ") + hipGetErrorString(e));
+        }
+        c->pad_bytes += (size_t)M * sizeof(cx<R>);
+        it = c->pad.emplace(key, t).first;
+    }
+    *out = (const cx<R>*)it->second;
+    return 0;
+}
+
+// Bluestein or padded-convolution tables, GenWave records and work rows of
+// the generic classes
 template <typename R>
 int prepare_generic(focss_ctx* c, int n, const focss_link* links, const std::vector<int>& order,
-                    std::vector<GenClass>& gen) {
+                    std::vector<GenClass>& gen, const std::vector<WaveStatic>& st,
+                    const std::vector<std::array<double, 2>>& kvals) {
     if (gen.empty()) return 0;
+    if (c->pad_bytes > kPadCacheBytes) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (auto& kv : c->pad) HIP_TRY(hipFree(kv.second));
+        c->pad.clear();
+        c->pad_bytes = 0;
+    }
     std::vector<GenWave<R>> gw(n);
     long long values = 0;
     for (GenClass& gc : gen) {
@@ -659,8 +748,20 @@ int prepare_generic(focss_ctx* c, int n, const focss_link* links, const std::vec
         for (int k = 0; k < gc.count; ++k) {
             const int i = order[gc.first + k];
             GenWave<R>& w = gw[i];
-            int lm = 0;
-            if (int rc = blue_table<R>(c, links[i].samples, &w.chirp, &w.bhat, &lm, &w.bhat_rows)) return rc;
+            w.chirp = w.bhat = w.bhat_rows = nullptr;
+            w.khat[0] = w.khat[1] = nullptr;
+            w.kval[0] = w.kval[1] = 0;
+            if (gc.padded) {
+                for (int j = 0; j < 2; ++j) {
+                    w.kval[j] = kvals[i][j];
+                    if (int rc = pad_table<R>(c, links[i].samples, gc.log_m, st[i].beta2, kvals[i][j], st[i].dw,
+                                              &w.khat[j]))
+                        return rc;
+                }
+            } else {
+                int lm = 0;
+                if (int rc = blue_table<R>(c, links[i].samples, &w.chirp, &w.bhat, &lm, &w.bhat_rows)) return rc;
+            }
             w.row0 = (long long)k * gc.modes;
             w.n = links[i].samples;
             w.pad = 0;
@@ -697,22 +798,31 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     }
     if (int rc = ensure_capacity(c, n)) return rc;
     // group by length class; lengths without a kernel of their own (not a
-    // power of two in [2^6, 2^20]) by their Bluestein length, keyed 100 + log_m
-    std::map<std::pair<int, int>, std::vector<int>> groups;
+    // power of two in [2^6, 2^20]) by their Bluestein length, keyed 100 + log_m,
+    // and by whether they run the padded convolution
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    std::vector<std::array<double, 2>> kvals(n, std::array<double, 2>{0.0, 0.0});
     for (int i = 0; i < n; ++i) {
         const int l = ilog2_int(links[i].samples);
-        const int key = (l >= 6 && l <= 20) ? l : 100 + ceil_log2(2LL * links[i].samples - 1);
-        groups[{key, links[i].modes}].push_back(i);
+        int key = l, padded = 0;
+        if (l < 6 || l > 20) {
+            const int lm = ceil_log2(2LL * links[i].samples - 1);
+            key = 100 + lm;
+            padded = c->pad_uniform && lm >= kBluMinLog && uniform_step_lengths(st[i], kvals[i].data());
+        }
+        groups[std::make_tuple(key, links[i].modes, padded)].push_back(i);
     }
     std::vector<int> order;
     std::vector<LengthClass> resident, two_pass;
     std::vector<GenClass> gen;
     for (auto& kv : groups) {
-        if (kv.first.first >= 100) {
+        const int key = std::get<0>(kv.first), modes = std::get<1>(kv.first);
+        if (key >= 100) {
             GenClass gc;
-            gc.log_m = kv.first.first - 100;
-            gc.modes = kv.first.second;
+            gc.log_m = key - 100;
+            gc.modes = modes;
             gc.fused = gc.log_m >= kBluMinLog;
+            gc.padded = std::get<2>(kv.first);
             gc.first = (int)order.size();
             gc.count = (int)kv.second.size();
             gc.max_n = 0;
@@ -722,8 +832,8 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
             continue;
         }
         LengthClass lc;
-        lc.log_n = kv.first.first;
-        lc.modes = kv.first.second;
+        lc.log_n = key;
+        lc.modes = modes;
         lc.variant = usable_variant(true, precision, lc.log_n, lc.modes, c->tile_variant[0][precision][lc.log_n]);
         lc.spec_variant = usable_variant(false, precision, lc.log_n, lc.modes, c->tile_variant[1][precision][lc.log_n]);
         lc.first = (int)order.size();
@@ -746,7 +856,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     }
     if ((profile_pairs > 0 || graph_replays > 0) && !gen.empty())
         return fail(FOCSS_E_UNSUPPORTED, "pass timing takes power-of-two lengths in [2^6, 2^20] only");
-    if (int rc = prepare_generic<R>(c, n, links, order, gen)) return rc;
+    if (int rc = prepare_generic<R>(c, n, links, order, gen, st, kvals)) return rc;
     std::vector<WaveState> init(2 * (size_t)n);
     for (int i = 0; i < n; ++i) init[i] = initial_state(st[i]);
     std::vector<Reduction> red0(3 * (size_t)n);
@@ -1470,6 +1580,8 @@ int focss_ctx_create(int device, focss_ctx** out) {
     c->use_graph = !(ng && ng[0] == '1');
     const char* nl = getenv("FOCSS_NATURAL_LAYOUT");
     c->natural_layout = nl && nl[0] == '1';
+    const char* bu = getenv("FOCSS_BLUESTEIN_UNIFORM");
+    c->pad_uniform = !(bu && bu[0] == '1');
     for (int p = 0; p <= 20; ++p) {
         // measured defaults at 2^18 samples: time pass 2048 elements per
         // tile; spectral pass 1024, two tiles per workgroup for complex64
@@ -1550,7 +1662,9 @@ int focss_ctx_destroy(focss_ctx* c) {
         for (void* p : {kv.second.chirp, kv.second.bhat, kv.second.bhat_rows})
             if (p) (void)hipFree(p);
     c->blue.clear();
-    for (void* p : {c->blu_work, c->gen_work})
+    for (auto& kv : c->pad) (void)hipFree(kv.second);
+    c->pad.clear();
+    for (void* p : {c->blu_work, c->gen_work, c->pad_tmp})
         if (p) (void)hipFree(p);
     if (c->h_done) (void)hipHostFree(c->h_done);
     for (int i = 0; i < 2; ++i)

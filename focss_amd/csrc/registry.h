@@ -133,7 +133,8 @@ void reg_fft_two(FftRegistry<R>& r) {
 template <typename R>
 struct BluRegistry {
     void (*rows[21])(BluArgs<R>, int, Tables<R>) = {};
-    void (*cols[21][2][3])(BluArgs<R>, Tables<R>) = {};
+    void (*rows_pad[21])(BluArgs<R>, int, Tables<R>) = {};  // padded-convolution spectral op
+    void (*cols[21][2][5])(BluArgs<R>, Tables<R>) = {};
     int row_tiles[21] = {};     // row-kernel workgroups per mode (N1 / S)
     int col_tiles[21][2] = {};  // column-kernel workgroups per waveform (N2 / C)
     int row_threads[21] = {};   // workgroup sizes the kernels are built for
@@ -146,6 +147,8 @@ void reg_blu_modes(BluRegistry<R>& r) {
     r.cols[LOGM][MODES - 1][BLU_ENTRY] = &k_blu_cols<R, LOGM, MODES, BLU_ENTRY>;
     r.cols[LOGM][MODES - 1][BLU_MID] = &k_blu_cols<R, LOGM, MODES, BLU_MID>;
     r.cols[LOGM][MODES - 1][BLU_OUT] = &k_blu_cols<R, LOGM, MODES, BLU_OUT>;
+    r.cols[LOGM][MODES - 1][BLU_PAD_ENTRY] = &k_blu_cols<R, LOGM, MODES, BLU_PAD_ENTRY>;
+    r.cols[LOGM][MODES - 1][BLU_PAD_OUT] = &k_blu_cols<R, LOGM, MODES, BLU_PAD_OUT>;
     r.col_tiles[LOGM][MODES - 1] = Sh::N2 / Sh::template C<MODES>;
     r.col_threads[LOGM][MODES - 1] = pass_threads(Sh::template C<MODES> * Sh::N1 / 8);
 }
@@ -155,7 +158,8 @@ void reg_blu(BluRegistry<R>& r) {
     using Sh = BluShape<R, LOGM>;
     static_assert(Sh::N1 % Sh::S == 0 && Sh::N2 % Sh::template C<1> == 0 && Sh::N2 % Sh::template C<2> == 0,
                   "Bluestein tile shape");
-    r.rows[LOGM] = &k_blu_rows<R, LOGM>;
+    r.rows[LOGM] = &k_blu_rows<R, LOGM, false>;
+    r.rows_pad[LOGM] = &k_blu_rows<R, LOGM, true>;
     r.row_tiles[LOGM] = Sh::N1 / Sh::S;
     r.row_threads[LOGM] = pass_threads(Sh::S * Sh::N2 / 8);
     reg_blu_modes<R, LOGM, 1>(r);
