@@ -107,6 +107,13 @@ struct focss_ctx {
     // passes (they fill only one workgroup per waveform)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // capture-only streams: the pass-pair graph holds one chain of launches
+    // per lane, so that classes on different lanes run side by side
+    static constexpr int kLanes = 4;
+    int lanes = kLanes;  // FOCSS_GRAPH_LANES=1..4
+    hipStream_t lane[kLanes - 1] = {};
+    hipEvent_t ev_lane_fork = nullptr;
+    hipEvent_t ev_lane_join[kLanes - 1] = {};
     size_t cap_waves = 0;
     int tile_variant[2][2][21] = {};  // [pass: 0 time, 1 spectral][precision][log2 samples]
     GraphCache graph;
@@ -271,7 +278,7 @@ struct LengthClass {
 };
 
 template <typename R>
-void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
+void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb, hipStream_t s = nullptr) {
     const Registry<R>& r = reg<R>();
     int cols = r.time_cols[lc.log_n][lc.modes - 1][lc.variant];
     int n2 = 1 << (lc.log_n - pass_log_n1(lc.log_n));
@@ -281,12 +288,12 @@ void launch_time(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     a.layout_g = lc.layout_g;
     KernelFn<R> k = r.time_k[lc.log_n][lc.modes - 1][lc.variant];
     const int groups = lc.count * (a.tiles_per_wave / r.time_tpb[lc.log_n][lc.modes - 1][lc.variant]);
-    hipLaunchKernelGGL(k, dim3(groups), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]), 0, c->stream, a,
-                       tb);
+    hipLaunchKernelGGL(k, dim3(groups), dim3(r.time_threads[lc.log_n][lc.modes - 1][lc.variant]), 0,
+                       s ? s : c->stream, a, tb);
 }
 
 template <typename R>
-void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) {
+void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb, hipStream_t s = nullptr) {
     const Registry<R>& r = reg<R>();
     int rows = r.spec_rows[lc.log_n][lc.spec_variant];
     int n1 = 1 << pass_log_n1(lc.log_n);
@@ -297,7 +304,8 @@ void launch_spec(focss_ctx* c, const LengthClass& lc, PassArgs a, Tables<R> tb) 
     a.time_tiles = (1 << (lc.log_n - pass_log_n1(lc.log_n))) / r.time_cols[lc.log_n][lc.modes - 1][lc.variant];
     KernelFn<R> k = r.spec_k[lc.log_n][lc.spec_variant];
     const int groups = lc.count * (a.tiles_per_wave / r.spec_tpb[lc.log_n][lc.spec_variant]);
-    hipLaunchKernelGGL(k, dim3(groups), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0, c->stream, a, tb);
+    hipLaunchKernelGGL(k, dim3(groups), dim3(r.spec_threads[lc.log_n][lc.spec_variant]), 0, s ? s : c->stream, a,
+                       tb);
 }
 
 // natural order back after the programs of an interleaved class end
@@ -612,22 +620,24 @@ BluArgs<R> blu_args(focss_ctx* c, const GenClass& gc, const PassArgs& a) {
 // dispersion multiply (whole-row DFT kernels below 2^kBluMinLog, else the
 // fused row / column kernels), or one padded convolution (one row launch)
 template <typename R>
-void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb) {
+void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb,
+                     hipStream_t s = nullptr) {
     if (gc.fused) {
+        if (!s) s = c->stream;
         const BluRegistry<R>& r = blu_reg<R>();
         BluArgs<R> g = blu_args<R>(c, gc, a);
         g.tiles = gc.modes * r.row_tiles[gc.log_m];
         const dim3 rows_grid(gc.count * g.tiles), rows_block(r.row_threads[gc.log_m]);
         if (gc.padded) {
-            hipLaunchKernelGGL(r.rows_pad[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 0, tb);
+            hipLaunchKernelGGL(r.rows_pad[gc.log_m], rows_grid, rows_block, 0, s, g, 0, tb);
             return;
         }
-        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 0, tb);
+        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, s, g, 0, tb);
         BluArgs<R> gm = g;
         gm.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
         hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][BLU_MID], dim3(gc.count * gm.tiles),
-                           dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, c->stream, gm, tb);
-        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, c->stream, g, 1, tb);
+                           dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, s, gm, tb);
+        hipLaunchKernelGGL(r.rows[gc.log_m], rows_grid, rows_block, 0, s, g, 1, tb);
         return;
     }
     const GenArgs<R> g = gen_args<R>(c, gc, a);
@@ -646,14 +656,16 @@ void launch_gen_spec(focss_ctx* c, const GenClass& gc, const PassArgs& a, const 
 
 // the time op of counter a.counter (entry: the first one)
 template <typename R>
-void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb, bool entry) {
+void launch_gen_time(focss_ctx* c, const GenClass& gc, const PassArgs& a, const Tables<R>& tb, bool entry,
+                     hipStream_t s = nullptr) {
     if (gc.fused) {
+        if (!s) s = c->stream;
         const BluRegistry<R>& r = blu_reg<R>();
         BluArgs<R> g = blu_args<R>(c, gc, a);
         g.tiles = r.col_tiles[gc.log_m][gc.modes - 1];
         const int kind = gc.padded ? (entry ? BLU_PAD_ENTRY : BLU_PAD_OUT) : (entry ? BLU_ENTRY : BLU_OUT);
         hipLaunchKernelGGL(r.cols[gc.log_m][gc.modes - 1][kind], dim3(gc.count * g.tiles),
-                           dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, c->stream, g, tb);
+                           dim3(r.col_threads[gc.log_m][gc.modes - 1]), 0, s, g, tb);
         return;
     }
     GenArgs<R> g = gen_args<R>(c, gc, a);
@@ -950,15 +962,58 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
         put(&c->d_active, sizeof(void*));
         if (c->use_graph && (c->graph.key != key || !c->graph.exec)) {
             drop_graph(c);
+            // classes to lanes, the most work first onto the least loaded
+            // lane (values moved per split step); the whole-row generic
+            // classes, whose DFTs launch on the context stream, stay on lane 0
+            std::vector<int> tp_lane(two_pass.size(), 0), gen_lane(gen.size(), 0);
+            std::vector<std::pair<double, int>> work;  // (values, unit: two-pass i, or generic ~i)
+            double load[focss_ctx::kLanes] = {};
+            for (size_t i = 0; i < two_pass.size(); ++i)
+                work.push_back({4.0 * two_pass[i].count * two_pass[i].modes * (1 << two_pass[i].log_n), (int)i});
+            for (size_t i = 0; i < gen.size(); ++i) {
+                const double v = (gen[i].padded ? 5.0 : 10.0) * gen[i].count * gen[i].modes * (1 << gen[i].log_m);
+                if (gen[i].fused)
+                    work.push_back({v, ~(int)i});
+                else
+                    load[0] += v;
+            }
+            std::sort(work.begin(), work.end(), [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+                return x.first > y.first;
+            });
+            int lanes = 1;
+            for (const auto& w : work) {
+                int l = 0;
+                for (int j = 1; j < std::min(c->lanes, (int)work.size()); ++j)
+                    if (load[j] < load[l]) l = j;
+                load[l] += w.first;
+                lanes = std::max(lanes, l + 1);
+                if (w.second >= 0)
+                    tp_lane[w.second] = l;
+                else
+                    gen_lane[~w.second] = l;
+            }
+            auto lane_stream = [c](int l) { return l == 0 ? c->stream : c->lane[l - 1]; };
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            if (lanes > 1) {
+                HIP_TRY(hipEventRecord(c->ev_lane_fork, c->stream));
+                for (int l = 1; l < lanes; ++l) HIP_TRY(hipStreamWaitEvent(c->lane[l - 1], c->ev_lane_fork, 0));
+            }
             for (int k = 0; k < c->graph_pairs; ++k) {
                 PassArgs ak = a;
                 ak.counter = k;
-                for (const LengthClass& lc : two_pass) launch_spec<R>(c, lc, ak, tb);
-                for (const GenClass& gc : gen) launch_gen_spec<R>(c, gc, ak, tb);
+                for (size_t i = 0; i < two_pass.size(); ++i)
+                    launch_spec<R>(c, two_pass[i], ak, tb, lane_stream(tp_lane[i]));
+                for (size_t i = 0; i < gen.size(); ++i)
+                    launch_gen_spec<R>(c, gen[i], ak, tb, lane_stream(gen_lane[i]));
                 ak.counter = k + 1;
-                for (const LengthClass& lc : two_pass) launch_time<R>(c, lc, ak, tb);
-                for (const GenClass& gc : gen) launch_gen_time<R>(c, gc, ak, tb, false);
+                for (size_t i = 0; i < two_pass.size(); ++i)
+                    launch_time<R>(c, two_pass[i], ak, tb, lane_stream(tp_lane[i]));
+                for (size_t i = 0; i < gen.size(); ++i)
+                    launch_gen_time<R>(c, gen[i], ak, tb, false, lane_stream(gen_lane[i]));
+            }
+            for (int l = 1; l < lanes; ++l) {
+                HIP_TRY(hipEventRecord(c->ev_lane_join[l - 1], c->lane[l - 1]));
+                HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_lane_join[l - 1], 0));
             }
             HIP_TRY(hipStreamEndCapture(c->stream, &c->graph.graph));
             HIP_TRY(hipGraphInstantiate(&c->graph.exec, c->graph.graph, nullptr, nullptr, 0));
@@ -1580,6 +1635,8 @@ int focss_ctx_create(int device, focss_ctx** out) {
     c->use_graph = !(ng && ng[0] == '1');
     const char* nl = getenv("FOCSS_NATURAL_LAYOUT");
     c->natural_layout = nl && nl[0] == '1';
+    const char* gl = getenv("FOCSS_GRAPH_LANES");
+    if (gl && atoi(gl) >= 1) c->lanes = std::min(atoi(gl), (int)focss_ctx::kLanes);
     const char* bu = getenv("FOCSS_BLUESTEIN_UNIFORM");
     c->pad_uniform = !(bu && bu[0] == '1');
     for (int p = 0; p <= 20; ++p) {
@@ -1633,6 +1690,11 @@ int ctx_init(focss_ctx* c) {
     HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_lane_fork, hipEventDisableTiming));
+    for (int l = 0; l < focss_ctx::kLanes - 1; ++l) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->lane[l], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_lane_join[l], hipEventDisableTiming));
+    }
     return 0;
 }
 }  // namespace
@@ -1672,6 +1734,11 @@ int focss_ctx_destroy(focss_ctx* c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->ev_lane_fork) (void)hipEventDestroy(c->ev_lane_fork);
+    for (int l = 0; l < focss_ctx::kLanes - 1; ++l) {
+        if (c->ev_lane_join[l]) (void)hipEventDestroy(c->ev_lane_join[l]);
+        if (c->lane[l]) (void)hipStreamDestroy(c->lane[l]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;

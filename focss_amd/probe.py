@@ -28,8 +28,9 @@ FIRs and 9-tap intensity filters) on synthetic fields, one polarisation
 untimed then `reps` timed steps each, host-timed.
 --workload generic: plain DBP (UNIFORM, 16 steps per span, 2 spans) of
 `batch` waveforms of N = 2 n_sym samples, n_sym uniform in [2^11, 2^15]
-(seed 5: the Bluestein link kernels), a complex Gaussian field, `reps`
-calls of focss_link_run, each timed on the host.
+(seed 5: the padded-convolution link kernels), or with --lengths pow2 of
+N drawn from 2^12 .. 2^16, a complex Gaussian field, `reps` calls of
+focss_link_run, each timed on the host.
 Usage:
     python -m focss_amd.probe [--batch 64] [--pairs 12] [--precision f64|f32] [--variant V]
     python -m focss_amd.probe --workload side [--reps 5]
@@ -57,6 +58,8 @@ def main():
     ap.add_argument("--replays", type=int, default=4)
     ap.add_argument("--time-variant", type=int, default=-1)
     ap.add_argument("--spec-variant", type=int, default=-1)
+    ap.add_argument("--lengths", choices=["uniform", "pow2"], default="uniform",
+                    help="generic workload: N = 2 n_sym (uniform) or N in 2^12 .. 2^16 (pow2)")
     args = ap.parse_args()
     if args.workload == "side":
         return side(args)
@@ -165,7 +168,10 @@ def generic(args):
     from focss_amd.learned import ALPHA, BAUD, F_C, SPAN_LEN, fiber
     prec = F.F64 if args.precision == "f64" else F.F32
     rng = np.random.default_rng(5)
-    ns = 2 * rng.integers(1 << 11, (1 << 15) + 1, size=args.batch)
+    if args.lengths == "pow2":
+        ns = 1 << rng.integers(12, 17, size=args.batch)
+    else:
+        ns = 2 * rng.integers(1 << 11, (1 << 15) + 1, size=args.batch)
     gain = np.exp(ALPHA * SPAN_LEN)
     links = [F.make_link(fiber(F), F.Grid(F.UNIFORM, 16, 0.0, 0.0), F.BACKWARD, 2, int(n), 1 / (2 * BAUD),
                          center_frequency=F_C, pre_scale=1 / np.sqrt(gain)) for n in ns]
