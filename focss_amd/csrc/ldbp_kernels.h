@@ -621,6 +621,7 @@ __global__ __launch_bounds__(256) void k_ldbp_grad_reduce(const double* part, in
 // the nonlinear phase, where one thread owns one block.
 // ------------------------------------------------------------------
 typedef _Float16 ldbp_h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 ldbp_h2 __attribute__((ext_vector_type(2)));  // two FP16 lanes of one packed instruction
 typedef float ldbp_f16 __attribute__((ext_vector_type(16)));
 
 struct cx_h {
@@ -714,7 +715,11 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
             acc[it] = ldbp_f16{};
             const int tile = wid + 16 * it;
             if (tile < n_tiles) {
-                const int colj = col, hlj = hl;
+                // lane coordinates made opaque per step: the operand
+                // addresses are formed here rather than held across the
+                // steps, where they would be spilled
+                int colj = col, hlj = hl;
+                __asm__ volatile("" : "+v"(colj), "+v"(hlj));
                 const cx_h4* xm = xs + (tile / mode_tiles) * NQ;
                 const int blk = (tile % mode_tiles) * 32 + colj;
 #pragma unroll
@@ -758,8 +763,15 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
         // nonlinear phase, one thread per block of 16 samples
         const bool last = j == a.J - 1;
         float g[KP];
+        ldbp_h2 g2[KP];
 #pragma unroll
-        for (int mm = 0; mm < KP; ++mm) g[mm] = sload(a.kerr + j * KP + mm);
+        for (int mm = 0; mm < KP; ++mm) {
+            g[mm] = sload(a.kerr + j * KP + mm);
+            // the packed pair is wave-uniform: held in a scalar register
+            const ldbp_h2 gv = ldbp_h2{(_Float16)g[mm], (_Float16)g[mm]};
+            g2[mm] = __builtin_bit_cast(
+                ldbp_h2, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, gv)));
+        }
         for (int blk = threadIdx.x; blk < nb; blk += LI_NT) {
             // intensities of quad rq of block bl, summed over the modes
             auto inten4 = [&](int rq, int bl) {
@@ -774,47 +786,119 @@ __global__ __launch_bounds__(LI_NT) void k_ldbp_infer(LdbpArgs a, const ldbp_h8*
                 }
                 return v;
             };
-            float pe[16 + 2 * P];  // p(16 blk - P + u), summed over the modes
-#pragma unroll
-            for (int rq = 0; rq < 4; ++rq) {
-                const float4 v = inten4(rq, blk);
-                pe[P + 4 * rq] = v.x;
-                pe[P + 4 * rq + 1] = v.y;
-                pe[P + 4 * rq + 2] = v.z;
-                pe[P + 4 * rq + 3] = v.w;
-            }
-            if constexpr (P > 0) {
-                const int bp = (blk + nb - 1) & (nb - 1), bn = (blk + 1) & (nb - 1);
-                // rows 16 - P .. 15 of the block before, rows 0 .. P - 1 of the one after
-                float lo[16], hi[16];
+            // pe[u] = p(16 blk - P + u), summed over the modes
+            auto gather = [&](float (&pe)[16 + 2 * P]) {
 #pragma unroll
                 for (int rq = 0; rq < 4; ++rq) {
-                    if (4 * rq + 3 >= 16 - P) {
-                        const float4 v = inten4(rq, bp);
-                        lo[4 * rq] = v.x;
-                        lo[4 * rq + 1] = v.y;
-                        lo[4 * rq + 2] = v.z;
-                        lo[4 * rq + 3] = v.w;
+                    const float4 v = inten4(rq, blk);
+                    pe[P + 4 * rq] = v.x;
+                    pe[P + 4 * rq + 1] = v.y;
+                    pe[P + 4 * rq + 2] = v.z;
+                    pe[P + 4 * rq + 3] = v.w;
+                }
+                if constexpr (P > 0) {
+                    const int bp = (blk + nb - 1) & (nb - 1), bn = (blk + 1) & (nb - 1);
+                    // rows 16 - P .. 15 of the block before, rows 0 .. P - 1 of the one after
+                    float lo[16], hi[16];
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq) {
+                        if (4 * rq + 3 >= 16 - P) {
+                            const float4 v = inten4(rq, bp);
+                            lo[4 * rq] = v.x;
+                            lo[4 * rq + 1] = v.y;
+                            lo[4 * rq + 2] = v.z;
+                            lo[4 * rq + 3] = v.w;
+                        }
+                        if (4 * rq < P) {
+                            const float4 v = inten4(rq, bn);
+                            hi[4 * rq] = v.x;
+                            hi[4 * rq + 1] = v.y;
+                            hi[4 * rq + 2] = v.z;
+                            hi[4 * rq + 3] = v.w;
+                        }
                     }
-                    if (4 * rq < P) {
-                        const float4 v = inten4(rq, bn);
-                        hi[4 * rq] = v.x;
-                        hi[4 * rq + 1] = v.y;
-                        hi[4 * rq + 2] = v.z;
-                        hi[4 * rq + 3] = v.w;
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        pe[i] = lo[16 - P + i];
+                        pe[P + 16 + i] = hi[i];
                     }
+                }
+            };
+            // Steps before the last whose 16 phases all lie within 1/2 rad
+            // (wave-uniform; the learned models' per-step phases) run on
+            // packed FP16, two samples per instruction: the intensity filter
+            // over pairs of samples, sin and cos - 1 as polynomials (x^5,
+            // x^4), and the rotation as z + z (cos - 1 + i sin), so the
+            // FP16 rounding of the small correction stays below 1e-4 of |z|
+            // and the field is rounded once, as the FP32 path rounds it.
+            bool fast = !last;
+            ldbp_h2 cm2[8], sn2[8];
+            if (fast) {
+                ldbp_h2 p2[8 + P];  // (p_2k, p_2k+1) of pe
+                ldbp_h2 po[7 + P];  // (p_2k+1, p_2k+2)
+                {
+                    float pe[16 + 2 * P];
+                    gather(pe);
+#pragma unroll
+                    for (int k = 0; k < 8 + P; ++k) p2[k] = ldbp_h2{(_Float16)pe[2 * k], (_Float16)pe[2 * k + 1]};
                 }
 #pragma unroll
-                for (int i = 0; i < P; ++i) {
-                    pe[i] = lo[16 - P + i];
-                    pe[P + 16 + i] = hi[i];
+                for (int k = 0; k < 7 + P; ++k) po[k] = __builtin_shufflevector(p2[k], p2[k + 1], 1, 2);
+                ldbp_h2 mx2 = {0, 0};
+#pragma unroll
+                for (int r2 = 0; r2 < 8; ++r2) {
+                    ldbp_h2 phi = {0, 0};
+#pragma unroll
+                    for (int mm = 0; mm < KP; ++mm) {
+                        const int base = 2 * r2 - mm + 2 * P;  // samples 2 r2, 2 r2 + 1 take p_base, p_base+1
+                        const ldbp_h2 pv = (base & 1) ? po[(base - 1) >> 1] : p2[base >> 1];
+                        phi = g2[mm] * pv + phi;
+                    }
+                    sn2[r2] = phi;
+                    mx2 = __builtin_elementwise_max(mx2, __builtin_elementwise_abs(phi));
                 }
+                fast = __all((float)__builtin_elementwise_max(mx2.x, mx2.y) <= 0.5f);
             }
-            // eight phases at a time, then one wave-uniform choice of the
-            // rotation: every phase within 1/8 rad (the learned models'
-            // per-step phases) takes a polynomial (sin to x^5, cos to x^4:
-            // below 1e-8 off), within 1000 rad the hardware sine and cosine
-            // (v_sin_f32 / v_cos_f32, ~1e-6), else the reduced series
+            if (fast) {
+#pragma unroll
+                for (int r2 = 0; r2 < 8; ++r2) {
+                    const ldbp_h2 x = sn2[r2], x2 = x * x;
+                    const ldbp_h2 t = x2 * ldbp_h2{(_Float16)(1.0f / 120), (_Float16)(1.0f / 120)} +
+                                      ldbp_h2{(_Float16)(-1.0f / 6), (_Float16)(-1.0f / 6)};
+                    sn2[r2] = (x * x2) * t + x;
+                    cm2[r2] = x2 * (x2 * ldbp_h2{(_Float16)(1.0f / 24), (_Float16)(1.0f / 24)} +
+                                    ldbp_h2{(_Float16)-0.5f, (_Float16)-0.5f});
+                }
+#pragma unroll
+                for (int rq = 0; rq < 4; ++rq) {
+#pragma unroll
+                    for (int md = 0; md < M; ++md) {
+                        const int q = md * NQ + (rq << lnb) + blk;
+                        ldbp_h8 zq = __builtin_bit_cast(ldbp_h8, xs[q]);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int r = 4 * rq + e;
+                            const _Float16 cmr = (r & 1) ? cm2[r >> 1].y : cm2[r >> 1].x;
+                            const _Float16 snr = (r & 1) ? sn2[r >> 1].y : sn2[r >> 1].x;
+                            const ldbp_h2 z = ldbp_h2{zq[2 * e], zq[2 * e + 1]};
+                            ldbp_h2 d = z * ldbp_h2{cmr, cmr};
+                            d = z.yx * ldbp_h2{-snr, snr} + d;
+                            const ldbp_h2 zn = z + d;
+                            zq[2 * e] = zn.x;
+                            zq[2 * e + 1] = zn.y;
+                        }
+                        xs[q] = __builtin_bit_cast(cx_h4, zq);
+                    }
+                }
+                continue;
+            }
+            float pe[16 + 2 * P];
+            gather(pe);
+            // otherwise eight phases at a time in FP32, then one wave-uniform
+            // choice of the rotation: every phase within 1/8 rad takes a
+            // polynomial (sin to x^5, cos to x^4: below 1e-8 off), within
+            // 1000 rad the hardware sine and cosine (v_sin_f32 / v_cos_f32,
+            // ~1e-6), else the reduced series
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
                 float sn[8], cs[8], mx = 0.0f;

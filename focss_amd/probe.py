@@ -16,7 +16,8 @@ at each batch of `--batches` realisations.
 --workload side: the other hot kernels at their measured sizes, `reps`
 launches each --
   learned DBP matrix-core inference, one polarisation: 6 steps/span x 20
-    spans, 31 taps, 9 intensity taps, 512 waveforms of 2^14 samples;
+    spans, 31 taps (random, unit energy per step), 9 intensity taps,
+    512 waveforms of 2^14 samples;
   the same, two polarisations: 4 steps/span x 30 spans, 512 waveforms of
     2 x 8192 samples;
   the save_field record reorder of 11 channels x 2 polarisations x 2^20
@@ -208,7 +209,10 @@ def side(args):
             fir = np.zeros((steps, taps), dtype=np.complex128)
             fir[:, taps // 2] = 1
             fir += 0.03 * crandn(steps, taps)
-            g = 0.01 * rng.standard_normal((steps, kt))
+            # unit-energy steps keep the field's power, as a trained model's
+            # FIRs do; per-step phases of a few 10 mrad on |x|^2 ~ 0.5
+            fir /= np.linalg.norm(fir, axis=1, keepdims=True)
+            g = 0.005 * rng.standard_normal((steps, kt))
             m = F.LearnedDBP(ctx, steps, taps, n, batch, fir, g, modes=modes)
             x = (0.5 * crandn(batch * modes * n)).astype(np.complex64)
             bi, bo = ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
