@@ -220,3 +220,38 @@ def test_uniform_padded_matches_chirp_z(ctx, ctx_chirp, n, modes, steps, ase):
     assert s_pad == s_chirp == s32 == 3 * steps
     assert err <= 1e-12
     assert err32 <= 2e-4
+
+
+def test_graph_lanes_bit_identical(ctx):
+    """Length classes captured as parallel lanes of the pass-pair graph
+    (default) and as one chain (FOCSS_GRAPH_LANES=1) give bit-identical
+    fields and equal step counts: every waveform's ops run in the same order
+    on the same data either way.  The batch holds two-pass classes (2^14,
+    2^16), padded-convolution UNIFORM classes (two lengths) and a chirp-z
+    ADAPTIVE class."""
+    import focss_amd as F
+    os.environ["FOCSS_GRAPH_LANES"] = "1"
+    try:
+        one = F.Context(0)
+    finally:
+        del os.environ["FOCSS_GRAPH_LANES"]
+    cases = [(1 << 14, 1, "adaptive_ase"), (1 << 16, 1, "uniform"), (10247, 1, "uniform"), (3 << 12, 2, "uniform"),
+             (24690, 1, "adaptive_ase"), (1 << 14, 1, "uniform")]
+    fields, links = [], []
+    for n, modes, g in cases:
+        x, dt = _case(n, modes, g)
+        fields.append(x)
+        links.append(_link(F, n, modes, g, dt))
+    offs = np.cumsum([0] + [f.size for f in fields[:-1]]).astype(np.int64)
+    host = np.concatenate(fields)
+    outs, steps = [], []
+    for c in (ctx, one):
+        buf = c.alloc(host.nbytes)
+        buf.upload(host)
+        steps.append(list(c.link_run(links, buf, offs, precision=F.F64)))
+        outs.append(buf.download(np.empty_like(host)))
+        buf.free()
+    one.close()
+    print(f"lanes: steps {steps[0]} / one chain {steps[1]}", flush=True)
+    assert steps[0] == steps[1]
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
