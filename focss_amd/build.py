@@ -72,10 +72,19 @@ def build_host():
     return lib
 
 
+def build_probe():
+    """focss_amd/pattern_probe: the memory-only floor of the two passes'
+    access patterns (a measurement program, not part of the library)."""
+    exe = os.path.join(HERE, "pattern_probe")
+    _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] + ["-o", exe, os.path.join(HERE, "pattern_probe.hip")])
+    return exe
+
+
 def main():
     only_host = "--host" in sys.argv
     if not only_host:
         build_hip(incremental="--incremental" in sys.argv)
+        build_probe()
     build_host()
 
 
