@@ -224,18 +224,37 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
 // j < L/8); it owns elements j + r L/8, r < 8.
 
 // Inter-pass twiddles W_N^{n2 (j + r Q)}, r < 8, formed in FP64 from two bank
-// lookups and products of depth <= 4, then rounded once to R.
+// lookups and products of depth <= 4, then rounded once to R: t0 = W^{n2 j}
+// and q1 = W^{n2 Q} from the tables (inter_bases), then t0 q1^r (inter_form).
 template <typename R>
-__device__ __forceinline__ void inter_twiddles(const Tables<R>& tb, unsigned n2, unsigned j, unsigned Q, int log_n,
-                                               cx<R>* t) {
+__device__ __forceinline__ void inter_bases(const Tables<R>& tb, unsigned n2, unsigned j, unsigned Q, int log_n,
+                                            cx<double>& t0, cx<double>& q1) {
     const unsigned mask = (1u << log_n) - 1u;
     const unsigned sh = 20 - log_n;
     auto big = [&](unsigned e) {
         unsigned E = (e << sh) & ((1u << 20) - 1u);
         return tb.bank64[tw_offset(10) + (E >> 10)] * tb.fine64[E & 1023u];
     };
-    cx<double> t0 = big((n2 * j) & mask);
-    cx<double> q1 = big((n2 * Q) & mask);
+    t0 = big((n2 * j) & mask);
+    q1 = big((n2 * Q) & mask);
+}
+
+// the four table values of inter_bases, not yet multiplied
+template <typename R>
+__device__ __forceinline__ void inter_loads(const Tables<R>& tb, unsigned n2, unsigned j, unsigned Q, int log_n,
+                                            cx<double>* t0, cx<double>* q1) {
+    const unsigned mask = (1u << log_n) - 1u;
+    const unsigned sh = 20 - log_n;
+    const unsigned E0 = ((((n2 * j) & mask) << sh) & ((1u << 20) - 1u));
+    const unsigned E1 = ((((n2 * Q) & mask) << sh) & ((1u << 20) - 1u));
+    t0[0] = tb.bank64[tw_offset(10) + (E0 >> 10)];
+    t0[1] = tb.fine64[E0 & 1023u];
+    q1[0] = tb.bank64[tw_offset(10) + (E1 >> 10)];
+    q1[1] = tb.fine64[E1 & 1023u];
+}
+
+template <typename R>
+__device__ __forceinline__ void inter_form(cx<double> t0, cx<double> q1, cx<R>* t) {
     cx<double> q2 = q1 * q1;
     cx<double> q4 = q2 * q2;
     cx<double> d[8];
@@ -249,6 +268,14 @@ __device__ __forceinline__ void inter_twiddles(const Tables<R>& tb, unsigned n2,
     d[7] = d[3] * q4;
 #pragma unroll
     for (int r = 0; r < 8; ++r) t[r] = mk<R>((R)d[r].x, (R)d[r].y);
+}
+
+template <typename R>
+__device__ __forceinline__ void inter_twiddles(const Tables<R>& tb, unsigned n2, unsigned j, unsigned Q, int log_n,
+                                               cx<R>* t) {
+    cx<double> t0, q1;
+    inter_bases<R>(tb, n2, j, Q, log_n, t0, q1);
+    inter_form<R>(t0, q1, t);
 }
 
 // time pass: C columns x M modes, each column of length N1 with stride N2.
@@ -388,6 +415,18 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
             if (w.grid == GRID_LOGARITHMIC)
                 a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + tile] = sm;
         }
+        // the store's twiddle table reads, issued before the forward DFT so
+        // that their latency passes during its LDS stages
+        cx<double> it0[PER][2], iq1[PER][2];
+        if (forward && kRecompute) {
+#pragma unroll
+            for (int b = 0; b < PER; ++b) {
+                const int id = threadIdx.x + b * TN;
+                if (ITEMS % TN == 0 || id < ITEMS)
+                    inter_loads<R>(tb, (unsigned)(n2_0 + id % C), (unsigned)(id / C), (unsigned)Q, a.log_n, it0[b],
+                                   iq1[b]);
+            }
+        }
         if (forward) {
 #pragma unroll
             for (int b = 0; b < PER; ++b) {
@@ -408,7 +447,7 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
                 const int c = id % C, j = id / C;
                 const int n2 = n2_0 + c;
                 const long long base = at(j, n2);
-                if (forward && kRecompute) inter_twiddles<R>(tb, (unsigned)n2, (unsigned)j, (unsigned)Q, a.log_n, itw[b]);
+                if (forward && kRecompute) inter_form<R>(it0[b][0] * it0[b][1], iq1[b][0] * iq1[b][1], itw[b]);
 #pragma unroll
                 for (int m = 0; m < M; ++m) {
                     if (forward) {
