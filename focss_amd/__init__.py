@@ -115,6 +115,17 @@ def _check(rc):
         raise FocssError(f"focss error {rc}: {_lib.focss_last_error().decode()}")
 
 
+def _batch_extent(links, offs, precision, what):
+    """Bytes of a batch buffer that link programs at element offsets `offs`
+    touch; offsets must be one per program and non-negative."""
+    if len(offs) != len(links):
+        raise FocssError(f"{what}: {len(links)} programs but {len(offs)} offsets")
+    if len(links) and int(offs.min()) < 0:
+        raise FocssError(f"{what}: negative offset")
+    elem = 16 if precision == F64 else 8
+    return max((int(o) + l.modes * l.samples for o, l in zip(offs, links)), default=0) * elem
+
+
 def _need(buf, nbytes, what):
     """The C-ABI takes no buffer sizes: refuse a device buffer smaller than
     the call will touch."""
@@ -207,6 +218,7 @@ class Context:
         n = len(links)
         arr = (Link * n)(*links)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        _need(batch, _batch_extent(links, offs, precision, "link_run"), "link_run")
         steps = np.zeros(n, dtype=np.int32)
         _check(_lib.focss_link_run(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
                                    steps.ctypes.data))
@@ -220,6 +232,7 @@ class Context:
         _need(schedule, n * capacity * 8, "link_run_schedule")
         arr = (Link * n)(*links)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        _need(batch, _batch_extent(links, offs, precision, "link_run_schedule"), "link_run_schedule")
         steps = np.zeros(n, dtype=np.int32)
         _check(_lib.focss_link_run_schedule(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
                                             steps.ctypes.data, mode, schedule.ptr, capacity))
@@ -231,6 +244,7 @@ class Context:
         n = len(links)
         arr = (Link * n)(*links)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        _need(batch, _batch_extent(links, offs, precision, "link_profile"), "link_profile")
         out = np.zeros(3, dtype=np.float64)
         _check(_lib.focss_link_profile(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
                                        n_pairs, out.ctypes.data))
@@ -242,12 +256,15 @@ class Context:
         n = len(links)
         arr = (Link * n)(*links)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        _need(batch, _batch_extent(links, offs, precision, "link_profile_graph"), "link_profile_graph")
         out = C.c_double(0)
         _check(_lib.focss_link_profile_graph(self.handle, precision, n, arr, batch.ptr, offs.ctypes.data,
                                              n_replays, C.byref(out)))
         return out.value
 
     def device_copy(self, dst, src, nbytes):
+        _need(dst, nbytes, "device_copy destination")
+        _need(src, nbytes, "device_copy source")
         _check(_lib.focss_device_copy(self.handle, dst.ptr, src.ptr, nbytes))
 
     def stream(self):

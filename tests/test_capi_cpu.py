@@ -142,3 +142,22 @@ def test_buffer_size_guard_refuses_small_buffers():
         F._need(Fake(), 101, "x")
     with pytest.raises(F.FocssError, match="needed"):
         F._need(None, 1, "x")
+
+
+def test_batch_extent_of_link_programs():
+    """link_run's guard: the bytes a batch of programs touches at its element
+    offsets, per precision; mismatched or negative offsets are refused."""
+    import numpy as np
+    import focss_amd as F
+    fib = F.Fiber(alpha=0.0, beta2=-2.17e-26, gamma=1.3e-3, length=8e4, wavelength=1.55e-6, kappa=1.0)
+    grid = F.Grid(F.UNIFORM, 4, 0.0, 0.0)
+    a = F.make_link(fib, grid, F.FORWARD, 1, 1000, 1e-11)
+    b = F.make_link(fib, grid, F.FORWARD, 1, 4096, 1e-11, modes=2)
+    offs = np.array([0, 1000], dtype=np.int64)
+    assert F._batch_extent([a, b], offs, F.F64, "x") == (1000 + 2 * 4096) * 16
+    assert F._batch_extent([a, b], offs, F.F32, "x") == (1000 + 2 * 4096) * 8
+    assert F._batch_extent([b, a], np.array([1000, 0], dtype=np.int64), F.F64, "x") == (1000 + 2 * 4096) * 16
+    with pytest.raises(F.FocssError, match="offsets"):
+        F._batch_extent([a, b], offs[:1], F.F64, "x")
+    with pytest.raises(F.FocssError, match="negative"):
+        F._batch_extent([a, b], np.array([0, -1], dtype=np.int64), F.F64, "x")
