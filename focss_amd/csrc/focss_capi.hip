@@ -814,13 +814,29 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     // and by whether they run the padded convolution
     std::map<std::tuple<int, int, int>, std::vector<int>> groups;
     std::vector<std::array<double, 2>> kvals(n, std::array<double, 2>{0.0, 0.0});
+    // the step lengths of a UNIFORM program depend on these fields only
+    // (the scales change the gains, not the lengths): one planner run each
+    using StepKey = std::tuple<int, int, int, int, unsigned long long, unsigned long long>;
+    std::map<StepKey, std::pair<bool, std::array<double, 2>>> step_memo;
     for (int i = 0; i < n; ++i) {
         const int l = ilog2_int(links[i].samples);
         int key = l, padded = 0;
         if (l < 6 || l > 20) {
             const int lm = ceil_log2(2LL * links[i].samples - 1);
             key = 100 + lm;
-            padded = c->pad_uniform && lm >= kBluMinLog && uniform_step_lengths(st[i], kvals[i].data());
+            if (c->pad_uniform && lm >= kBluMinLog) {
+                const WaveStatic& w = st[i];
+                const StepKey sk{w.grid * 2 + w.linear_only, w.steps, w.n_spans, w.noise_sigma != 0,
+                                 double_to_bits(w.length), double_to_bits(w.dir)};
+                auto it = step_memo.find(sk);
+                if (it == step_memo.end()) {
+                    std::array<double, 2> kv{0.0, 0.0};
+                    const bool ok = uniform_step_lengths(w, kv.data());
+                    it = step_memo.emplace(sk, std::make_pair(ok, kv)).first;
+                }
+                padded = it->second.first;
+                kvals[i] = it->second.second;
+            }
         }
         groups[std::make_tuple(key, links[i].modes, padded)].push_back(i);
     }
