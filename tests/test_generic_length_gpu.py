@@ -194,7 +194,7 @@ def ctx_chirp():
 
 
 @pytest.mark.parametrize("n,modes,steps,ase", [(10247, 1, 4, True), (5 << 11, 2, 1, False), (24690, 1, 3, False),
-                                               ((1 << 16) - 3, 1, 2, True)])
+                                               ((1 << 16) - 3, 1, 2, True), (1500, 2, 3, True), (1025, 1, 2, False)])
 def test_uniform_padded_matches_chirp_z(ctx, ctx_chirp, n, modes, steps, ase):
     """UNIFORM programs of generic length run each spectral op as one
     zero-padded circular convolution of length M with a tabulated kernel;
