@@ -126,6 +126,15 @@ def _batch_extent(links, offs, precision, what):
     return max((int(o) + l.modes * l.samples for o, l in zip(offs, links)), default=0) * elem
 
 
+def _host_array(a, dtype, count, what):
+    """Contiguous host array of which the C-ABI reads (and may write) exactly
+    `count` values, taking no size for it: refuse one of another size."""
+    h = np.ascontiguousarray(a, dtype=dtype)
+    if h.size != count:
+        raise FocssError(f"{what}: {count} values needed, {h.size} given")
+    return h
+
+
 def _need(buf, nbytes, what):
     """The C-ABI takes no buffer sizes: refuse a device buffer smaller than
     the call will touch."""
@@ -207,7 +216,7 @@ class Context:
 
     def link_run_host(self, link, field, precision=F64):
         """Run one link program on a host complex128 array (copied in/out)."""
-        y = np.ascontiguousarray(field, dtype=np.complex128).copy()
+        y = _host_array(field, np.complex128, link.modes * link.samples, "link_run_host").copy()
         steps = C.c_int(0)
         _check(_lib.focss_link_run_host(self.handle, precision, C.byref(link), y.ctypes.data,
                                         C.byref(steps)))
@@ -370,7 +379,7 @@ class Context:
         write_records(path, rec, channels, modes)
 
     def sino_cubic(self, weights, radius, inp, out, modes, samples, precision=F64):
-        w = np.ascontiguousarray(weights, dtype=np.complex128)
+        w = _host_array(weights, np.complex128, (2 * radius + 1) ** 2, "sino_cubic weights")
         _check(_lib.focss_sino_cubic(self.handle, precision, radius, w.ctypes.data, modes, samples,
                                      inp.ptr, out.ptr))
 
@@ -385,8 +394,8 @@ class Context:
         return w, ic[0], wl[0]
 
     def wdm_ppe(self, c_radius, Cm, d_radius, Dm, a6, spm, xpm, samples, precision=F64):
-        cm = np.ascontiguousarray(Cm, dtype=np.complex128)
-        dm = np.ascontiguousarray(Dm, dtype=np.complex128)
+        cm = _host_array(Cm, np.complex128, (2 * c_radius + 1) ** 2 if c_radius >= 0 else np.size(Cm), "wdm_ppe C")
+        dm = _host_array(Dm, np.complex128, (2 * d_radius + 1) ** 2 if d_radius >= 0 else np.size(Dm), "wdm_ppe D")
         _check(_lib.focss_wdm_ppe(self.handle, precision, c_radius, cm.ctypes.data, d_radius, dm.ctypes.data,
                                   samples, a6.ptr, spm.ptr, xpm.ptr))
 

@@ -102,12 +102,19 @@ __global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::S * BluShape<R, LOG
         const WaveStatic w = a.statics[w_id];
         const int was_done = st.phase == PH_DONE;
         const Reduction rd = a.red[(a.counter % 3) * a.n_waves + w_id];
-        const SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), rd.sum);
+        // the column kernel's tile sums in tile order (UNIFORM programs use
+        // the sum for energy_comp only)
+        const double sum = !w.energy_comp ? rd.sum : comp_sum(st, rd, [&] {
+            return tile_sum(a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride,
+                            N2 / (w.modes == 1 ? Sh::template C<1> : Sh::template C<2>));
+        });
+        const SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
         if (t == 0 && threadIdx.x == 0) {
             a.states[((a.counter + 1) & 1) * a.n_waves + w_id] = st;
             Reduction z;
             z.peak_bits = 0;
             z.sum = 0;
+            z.fix = 0;
             a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
             if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
         }
@@ -117,7 +124,7 @@ __global__ __launch_bounds__(pass_threads(BluShape<R, LOGM>::S * BluShape<R, LOG
         const GenWave<R>& gw = g.gw[w_id];
         const int sel = fabs(op.s - gw.kval[1]) < fabs(op.s - gw.kval[0]);
         bh = gw.khat[sel] + (long long)k1_0 * N2;
-        kscale = (R)(op.gain * exp(-0.5 * w.alpha * op.s));
+        kscale = (R)(op.gain * op.loss);
     } else {
         if (after_mid) {
             if (!g.active[w_id]) return;
@@ -214,6 +221,11 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
             const double* p = a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride;
             sum = 0;
             for (int i = 0; i < g.tiles; ++i) sum += p[i];
+        } else if (w.energy_comp) {
+            sum = comp_sum(st, rd, [&] {
+                return tile_sum(a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride,
+                                g.tiles);
+            });
         }
         op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
         if (t == 0 && threadIdx.x == 0) {
@@ -222,6 +234,7 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
             Reduction z;
             z.peak_bits = 0;
             z.sum = 0;
+            z.fix = 0;
             a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
             g.active[w_id] = op.active;
             if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
@@ -282,7 +295,7 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
     bool last = false;
     if (KIND == BLU_MID) {
         const double arg = 0.5 * w.beta2 * op.s;
-        const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)n;
+        const double mag = op.gain * op.loss / (double)n;
 #pragma unroll
         for (int b = 0; b < PER; ++b) {
             const int id = threadIdx.x + b * TN;
@@ -341,7 +354,8 @@ k_blu_cols(BluArgs<R> g, Tables<R> tb) {
             Reduction* rd = a.red + (a.counter % 3) * a.n_waves + w_id;
             atomicMax(&rd->peak_bits, double_to_bits(pk));
             atomicAdd(&rd->sum, sm);
-            if (w.grid == GRID_LOGARITHMIC)
+            if (w.energy_comp && expected_energy(st) > 0) add_fixed_sum(rd, sm, expected_energy(st));
+            if (w.grid == GRID_LOGARITHMIC || (w.energy_comp && !(expected_energy(st) > 0)))
                 a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + t] = sm;
         }
         if (last) return;

@@ -110,6 +110,11 @@ __global__ __launch_bounds__(256) void k_gen_mid(GenArgs<R> g) {
         const int chunks = (gw.n + kGenTimeChunk - 1) / kGenTimeChunk;
         sum = 0;
         for (int t = 0; t < chunks; ++t) sum += p[t];
+    } else if (w.energy_comp) {
+        sum = comp_sum(st, rd, [&] {
+            return tile_sum(a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride,
+                            (gw.n + kGenTimeChunk - 1) / kGenTimeChunk);
+        });
     }
     SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
     if (blockIdx.x % (g.chunks_per_row * g.modes) == 0 && threadIdx.x == 0) {
@@ -118,13 +123,14 @@ __global__ __launch_bounds__(256) void k_gen_mid(GenArgs<R> g) {
         Reduction z;
         z.peak_bits = 0;
         z.sum = 0;
+        z.fix = 0;
         a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
         g.active[w_id] = op.active;
         if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
     }
     if (!op.active) return;
     const double arg = 0.5 * w.beta2 * op.s;
-    const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)gw.n;
+    const double mag = op.gain * op.loss / (double)gw.n;
     cx<R>* row = g.work + (gw.row0 + m) * (1LL << g.log_m);
     const int M = 1 << g.log_m;
     for (int e = e0 + threadIdx.x; e < e0 + kGenChunk && e < M; e += blockDim.x) {
@@ -182,7 +188,8 @@ __global__ __launch_bounds__(256) void k_gen_time(GenArgs<R> g) {
         Reduction* r = a.red + (a.counter % 3) * a.n_waves + w_id;
         atomicMax(&r->peak_bits, double_to_bits(pk));
         atomicAdd(&r->sum, sm);
-        if (w.grid == GRID_LOGARITHMIC)
+        if (w.energy_comp && expected_energy(st) > 0) add_fixed_sum(r, sm, expected_energy(st));
+        if (w.grid == GRID_LOGARITHMIC || (w.energy_comp && !(expected_energy(st) > 0)))
             a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + chunk] = sm;
     }
 }

@@ -57,6 +57,17 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Sum of a time op's n per-tile power sums in a fixed order (lane-strided,
+// then the xor butterfly, after which every lane holds the same bits): plans
+// formed from it are run-to-run deterministic whatever order the tiles
+// finished in.  Every lane of the wave calls it.
+__device__ inline double tile_sum(const double* p, int n) {
+    double s = 0;
+    for (int t = threadIdx.x & 63; t < n; t += 64) s += p[t];
+    return wave_sum(s);
+}
+
+
 // Workgroup max and sum; every thread gets the result.  red needs
 // 2 * (NT / 64) doubles of shared memory.
 template <int NT>

@@ -132,6 +132,7 @@ struct focss_ctx {
     void* pad_tmp = nullptr;  // FP64 staging of one table (N + M values)
     size_t pad_tmp_bytes = 0;
     bool pad_uniform = true;  // FOCSS_BLUESTEIN_UNIFORM=1: UNIFORM programs through Bluestein too
+    bool energy_comp = true;  // FOCSS_FP32_ENERGY=0: complex64 programs without the compensated energy
     void* d_genwave = nullptr;  // [cap] GenWave
     int* d_active = nullptr;    // [cap]
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
@@ -380,6 +381,7 @@ int fill_static(const focss_link& L, long long offset, WaveStatic& w) {
     w.sched_mode = 0;
     w.sched_cap = 0;
     w.sched = nullptr;
+    w.energy_comp = 0;
     return 0;
 }
 
@@ -802,6 +804,7 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
     for (int i = 0; i < n; ++i) {
         int rc = fill_static(links[i], offsets[i], st[i]);
         if (rc) return rc;
+        st[i].energy_comp = sizeof(R) == 4 && c->energy_comp;
         if (schedule.mode) {
             st[i].sched_mode = schedule.mode;
             st[i].sched_cap = schedule.cap;
@@ -1655,6 +1658,8 @@ int focss_ctx_create(int device, focss_ctx** out) {
     if (gl && atoi(gl) >= 1) c->lanes = std::min(atoi(gl), (int)focss_ctx::kLanes);
     const char* bu = getenv("FOCSS_BLUESTEIN_UNIFORM");
     c->pad_uniform = !(bu && bu[0] == '1');
+    const char* fe = getenv("FOCSS_FP32_ENERGY");
+    c->energy_comp = !(fe && fe[0] == '0');
     for (int p = 0; p <= 20; ++p) {
         // measured defaults at 2^18 samples: time pass 2048 elements per
         // tile; spectral pass 1024, two tiles per workgroup for complex64

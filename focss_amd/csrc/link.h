@@ -22,11 +22,13 @@
 #define FOCSS_HD __host__ __device__
 #define FOCSS_EXP ::exp
 #define FOCSS_LOG ::log
+#define FOCSS_SQRT ::sqrt
 #else
 #include <cmath>
 #define FOCSS_HD
 #define FOCSS_EXP std::exp
 #define FOCSS_LOG std::log
+#define FOCSS_SQRT std::sqrt
 #endif
 
 namespace focss_hip {
@@ -55,6 +57,9 @@ struct WaveStatic {
     // instead of estimating them; sched_cap entries
     int sched_mode, sched_cap;
     double* sched;
+    // 1: hold the field's energy to an FP64 running expectation (complex64
+    // programs; see plan_spectral)
+    int energy_comp;
 };
 
 struct WaveState {
@@ -69,12 +74,14 @@ struct WaveState {
     double nl_coef;      // kappa gamma step for a TOP_NL op
     double t_scale;      // scale applied by a transition / entry op
     double log_inv_gain; // LOGARITHMIC: (e^{-alpha L} - 1) / n_log
+    double e_ref;        // energy_comp: expected energy after the next time op
 };
 
 struct SpectralOp {
     int active;
     double s;     // signed linear distance of this multiply
     double gain;  // scalar gain (span scales), excluding loss and 1/N
+    double loss;  // e^{-alpha s / 2}, the amplitude loss of the multiply
 };
 
 // Power reductions produced by each time-domain op: max and sum over samples
@@ -83,7 +90,28 @@ struct SpectralOp {
 struct Reduction {
     unsigned long long peak_bits;  // non-negative double as bits (atomicMax)
     double sum;
+    // energy_comp: the sum in fixed point, kFixOne = the expected energy
+    // (integer adds are exact: the total does not depend on the order the
+    // tiles finish in); kFixBad or more = a tile sum out of range
+    unsigned long long fix;
 };
+
+constexpr double kFixOne = 1125899906842624.0;  // 2^50
+constexpr unsigned long long kFixBad = 1ull << 62;
+
+#ifdef __HIPCC__
+// energy_comp: one tile's power sum into rd->fix, relative to the expected
+// energy ex > 0, rounded to the nearest unit.  A tile sum above twice the
+// expected total (or not finite) sets kFixBad instead; the adds of at most
+// 1024 tiles below that stay under 2^61 and never reach its bit.
+__device__ inline void add_fixed_sum(Reduction* rd, double sm, double ex) {
+    const double q = sm * (kFixOne / ex);
+    if (q >= 0 && q < 2 * kFixOne)
+        atomicAdd(&rd->fix, (unsigned long long)(q + 0.5));
+    else
+        atomicOr(&rd->fix, kFixBad);
+}
+#endif
 
 FOCSS_HD inline double bits_to_double(unsigned long long b) {
     union {
@@ -170,8 +198,7 @@ FOCSS_HD inline void record_step(const WaveStatic& w, const WaveState& st) {
 // Plan the next spectral multiply and the time-domain op after it.
 // red = reductions of the previous time-domain op.  Pure function of its
 // inputs, so every workgroup of a pass computes the same plan.
-FOCSS_HD inline SpectralOp plan_spectral(const WaveStatic& w, WaveState& st, double peak,
-                                         double sum) {
+FOCSS_HD inline SpectralOp plan_step(const WaveStatic& w, WaveState& st, double peak, double sum) {
     SpectralOp op;
     op.active = 0;
     op.s = 0;
@@ -261,6 +288,50 @@ FOCSS_HD inline SpectralOp plan_spectral(const WaveStatic& w, WaveState& st, dou
     return op;
 }
 
+// plan_step, and for energy_comp programs the compensated energy: the
+// multiply, the inverse DFT and the Kerr op change the energy
+// sum_t P(t) by exactly gain^2 e^{-alpha s} (dispersion and Kerr phases have
+// modulus 1), and a span transition without ASE by t_scale^2, so the energy
+// after those ops is known in FP64 from the one measured after the entry or
+// the last transition with ASE (ASE makes that one random).  Complex64
+// rounding loses a steady ~2e-7 of it per split step, which also lowers
+// every later Kerr phase; the multiply after each such op takes
+// sqrt(expected / measured) into its scalar, so the drift does not
+// accumulate over the steps and spans.  sum must be run-to-run
+// deterministic (formed in tile order) for the plans to be.
+//
+// expected_energy: what the time op that st plans leaves the field with
+// when its arithmetic is exact; 0 when that is not known (entry, ASE).
+FOCSS_HD inline double expected_energy(const WaveState& st) {
+    if (st.top == TOP_NL) return st.e_ref;
+    if (st.top == TOP_TRANSITION && st.noise_stream < 0) return st.e_ref * st.t_scale * st.t_scale;
+    return 0;
+}
+
+FOCSS_HD inline SpectralOp plan_spectral(const WaveStatic& w, WaveState& st, double peak,
+                                         double sum) {
+    const double expect = w.energy_comp ? expected_energy(st) : 0;  // of the op just run
+    SpectralOp op = plan_step(w, st, peak, sum);
+    op.loss = op.active ? FOCSS_EXP(-0.5 * w.alpha * op.s) : 1.0;
+    if (!w.energy_comp || !op.active) return op;
+    const bool held = expect > 0 && sum > 0;
+    const double base = held ? expect : sum;
+    st.e_ref = base * (op.gain * op.loss) * (op.gain * op.loss);
+    if (held) op.gain *= FOCSS_SQRT(base / sum);
+    return op;
+}
+
+// energy_comp: the measured energy from the fixed-point sum when the
+// expected one is known (a tile sum out of range: the float sum, whose
+// order of adds varies), else from the tile sums in tile order (ordered(),
+// which the time op writes then only: once a span at most)
+template <typename F>
+FOCSS_HD inline double comp_sum(const WaveState& st, const Reduction& rd, F ordered) {
+    const double ex = expected_energy(st);
+    if (!(ex > 0)) return ordered();
+    return rd.fix < kFixBad ? (double)rd.fix * (ex / kFixOne) : rd.sum;
+}
+
 // Initial state: the first time-domain op applies span 0's scale and
 // measures the field for the first step estimate.
 FOCSS_HD inline WaveState initial_state(const WaveStatic& w) {
@@ -279,6 +350,7 @@ FOCSS_HD inline WaveState initial_state(const WaveStatic& w) {
     st.nl_coef = 0;
     st.t_scale = w.pre_scale;
     st.log_inv_gain = 0;
+    st.e_ref = 0;
     return st;
 }
 

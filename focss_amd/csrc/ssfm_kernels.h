@@ -181,7 +181,7 @@ __global__ __launch_bounds__(NT) void k_resident(PassArgs a, Tables<R> tb) {
         if (threadIdx.x == 0) record_step(w, st);
         if (!op.active) break;
         const double arg = 0.5 * w.beta2 * op.s;
-        const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
+        const double mag = op.gain * op.loss / (double)N;
         for (int k = threadIdx.x; k < N; k += NT) {
             cx<R> f = spectral_factor<R>(w, op, k, arg, mag);
 #pragma unroll
@@ -409,10 +409,11 @@ __global__ __launch_bounds__(pass_threads(C * N1 / 8)) void k_time_pass(PassArgs
             Reduction* rd = a.red + (a.counter % 3) * a.n_waves + w_id;
             atomicMax(&rd->peak_bits, double_to_bits(pk));
             atomicAdd(&rd->sum, sm);
+            if (w.energy_comp && expected_energy(st) > 0) add_fixed_sum(rd, sm, expected_energy(st));
             // the logarithmic grid's step count depends on the sum; keep the
             // tile sums so that it is formed in tile order, whatever order
             // the workgroups finish in
-            if (w.grid == GRID_LOGARITHMIC)
+            if (w.grid == GRID_LOGARITHMIC || (w.energy_comp && !(expected_energy(st) > 0)))
                 a.partials[((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride + tile] = sm;
         }
         // the store's twiddle table reads, issued before the forward DFT so
@@ -543,12 +544,7 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
     const bool il_out = a.layout_g == 2;
     const WaveStatic w = a.statics[w_id];
     const Reduction rd = a.red[(a.counter % 3) * a.n_waves + w_id];
-    double sum = rd.sum;
-    if (w.grid == GRID_LOGARITHMIC && st.phase == PH_SPAN_START) {
-        const double* p = a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride;
-        sum = 0;
-        for (int t = 0; t < a.time_tiles; ++t) sum += p[t];
-    }
+    const double* parts = a.partials + ((long long)(a.counter % 3) * a.n_waves + w_id) * a.partial_stride;
     cx<R>* g0 = reinterpret_cast<cx<R>*>(a.field) + w.offset;
     // offset of element j + r Q of row s of the tile: s N2 + j + r Q in
     // natural order, or Interleave(s, j) + 2 r Q (Q a multiple of W)
@@ -574,6 +570,13 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
     // finished program's workgroups load nothing)
     const auto tw_staged = TW::template fetch<TN>(tb.bank);
     if (!was_done) load(tile0, va);
+    double sum = rd.sum;
+    if (w.grid == GRID_LOGARITHMIC && st.phase == PH_SPAN_START) {
+        sum = 0;
+        for (int t = 0; t < a.time_tiles; ++t) sum += parts[t];
+    } else if (w.energy_comp) {
+        sum = comp_sum(st, rd, [&] { return tile_sum(parts, a.time_tiles); });
+    }
     SpectralOp op = plan_spectral(w, st, bits_to_double(rd.peak_bits), sum);
     if (tile0 == 0 && threadIdx.x == 0) {
         record_step(w, st);
@@ -581,6 +584,7 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
         Reduction z;
         z.peak_bits = 0;
         z.sum = 0;
+        z.fix = 0;
         a.red[((a.counter + 1) % 3) * a.n_waves + w_id] = z;
         if (!was_done && st.phase == PH_DONE) atomicAdd(a.done_count, 1);
     }
@@ -603,7 +607,7 @@ __global__ __launch_bounds__(pass_threads(S * N2 / 8)) void k_spec_pass(PassArgs
         __syncthreads();
         middle_stages<R, N2, S, TN, -1, false>(buf, tw_t);
         const double arg = 0.5 * w.beta2 * op.s;
-        const double mag = op.gain * exp(-0.5 * w.alpha * op.s) / (double)N;
+        const double mag = op.gain * op.loss / (double)N;
 #pragma unroll
         for (int b = 0; b < PER; ++b) {
             const int id = threadIdx.x + b * TN;

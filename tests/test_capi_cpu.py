@@ -161,3 +161,16 @@ def test_batch_extent_of_link_programs():
         F._batch_extent([a, b], offs[:1], F.F64, "x")
     with pytest.raises(F.FocssError, match="negative"):
         F._batch_extent([a, b], np.array([0, -1], dtype=np.int64), F.F64, "x")
+
+
+def test_host_arrays_of_the_size_read():
+    """Host arrays the C-ABI reads (and writes) by a count it derives itself
+    (link_run_host: modes x samples; the equalizer weights: (2R+1)^2): one of
+    another size is refused before any call reads past its end."""
+    import numpy as np
+    import focss_amd as F
+    assert F._host_array(np.zeros(6), np.complex128, 6, "x").dtype == np.complex128
+    with pytest.raises(F.FocssError, match="10247 values needed, 10246 given"):
+        F._host_array(np.zeros(10246, dtype=np.complex128), np.complex128, 10247, "link_run_host")
+    with pytest.raises(F.FocssError, match="25 values needed"):
+        F._host_array(np.zeros((4, 4)), np.complex128, 25, "sino_cubic weights")
