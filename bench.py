@@ -196,6 +196,21 @@ def run_c5(F, ctx, batch, log):
     return out
 
 
+def context_without_energy_comp(F):
+    """A context whose complex64 programs run without the compensated energy
+    (FOCSS_FP32_ENERGY=0, read when the context is created): the per-span
+    error beside the default's shows what the compensation does."""
+    old = os.environ.get("FOCSS_FP32_ENERGY")
+    os.environ["FOCSS_FP32_ENERGY"] = "0"
+    try:
+        return F.Context(0)
+    finally:
+        if old is None:
+            del os.environ["FOCSS_FP32_ENERGY"]
+        else:
+            os.environ["FOCSS_FP32_ENERGY"] = old
+
+
 def f32_error_per_span(F, ctx, field, dt, grid, replay=False):
     """Relative L2 error of the FP32 path against the FP64 path after each
     span, one realisation carried span by span through both precisions with
@@ -370,6 +385,12 @@ def main():
                                                                      replay=True),
                         "uniform_100_steps_per_span": f32_error_per_span(F, ctx, field, dt,
                                                                          F.Grid(F.UNIFORM, 100, 0.0, 0.0))}
+            with context_without_energy_comp(F) as off:
+                span_err["without_energy_comp"] = {
+                    "adaptive_fp64_schedule": f32_error_per_span(F, off, field, dt, F.Grid(F.ADAPTIVE, 0, PHI_MAX, H_MAX),
+                                                                 replay=True),
+                    "uniform_100_steps_per_span": f32_error_per_span(F, off, field, dt,
+                                                                     F.Grid(F.UNIFORM, 100, 0.0, 0.0))}
     head = res.get("f64") or res.get("f32")
     head_key = "f64" if "f64" in res else "f32"
     traffic = load_traffic(head_key)
@@ -421,8 +442,8 @@ def main():
             # the ADAPTIVE schedule follows each precision's own peak power, so
             # the two runs take different steps; the replayed FP64 schedule and
             # the UNIFORM run isolate the arithmetic
-            line["f32"]["rel_l2_vs_f64_per_span"] = {k: [float(f"{e:.3g}") for e in v]
-                                                     for k, v in span_err.items()}
+            fmt = lambda v: {k: fmt(x) for k, x in v.items()} if isinstance(v, dict) else [float(f"{e:.3g}") for e in v]
+            line["f32"]["rel_l2_vs_f64_per_span"] = fmt(span_err)
     if c5:
         line["c5"] = c5
     if not args.no_learned:
