@@ -133,6 +133,7 @@ struct focss_ctx {
     size_t pad_tmp_bytes = 0;
     bool pad_uniform = true;  // FOCSS_BLUESTEIN_UNIFORM=1: UNIFORM programs through Bluestein too
     bool energy_comp = true;  // FOCSS_FP32_ENERGY=0: complex64 programs without the compensated energy
+    bool split_classes = true;  // FOCSS_SPLIT_CLASSES=0: a large two-pass class stays one chain of the graph
     void* d_genwave = nullptr;  // [cap] GenWave
     int* d_active = nullptr;    // [cap]
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
@@ -188,6 +189,7 @@ void host_tables(std::vector<cx<R>>& bank, std::vector<cx<R>>& fine) {
     }
 }
 
+constexpr int kMinSplitWaves = 16;  // two-pass classes of this many waveforms run as two chains of the graph
 constexpr int kMaxTimeTiles = 1024;  // time-pass tiles per waveform (N2 <= 1024)
 // lengths that are not powers of two (Bluestein DFTs of length 2^20 at most);
 // their time op's chunks (kGenTimeChunk samples) also fit kMaxTimeTiles
@@ -884,6 +886,26 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
             return fail(FOCSS_E_UNSUPPORTED, "no kernel for samples=2^" + std::to_string(lc.log_n) +
                                                  " modes=" + std::to_string(lc.modes));
         }
+    }
+    // in the captured graph a large two-pass class runs as two chains of half
+    // its waveforms on separate lanes, so that one half's time pass overlaps
+    // the other's spectral pass across the kernel boundaries (each waveform's
+    // arithmetic is the same: the results do not change)
+    if (c->use_graph && c->lanes > 1 && c->split_classes && profile_pairs == 0) {
+        std::vector<LengthClass> split;
+        for (const LengthClass& lc : two_pass) {
+            if (lc.count >= kMinSplitWaves) {
+                LengthClass h = lc;
+                h.count = lc.count / 2;
+                split.push_back(h);
+                h.first = lc.first + h.count;
+                h.count = lc.count - lc.count / 2;
+                split.push_back(h);
+            } else {
+                split.push_back(lc);
+            }
+        }
+        two_pass.swap(split);
     }
     if ((profile_pairs > 0 || graph_replays > 0) && !gen.empty())
         return fail(FOCSS_E_UNSUPPORTED, "pass timing takes power-of-two lengths in [2^6, 2^20] only");
@@ -1660,6 +1682,8 @@ int focss_ctx_create(int device, focss_ctx** out) {
     c->pad_uniform = !(bu && bu[0] == '1');
     const char* fe = getenv("FOCSS_FP32_ENERGY");
     c->energy_comp = !(fe && fe[0] == '0');
+    const char* sc = getenv("FOCSS_SPLIT_CLASSES");
+    c->split_classes = !(sc && sc[0] == '0');
     for (int p = 0; p <= 20; ++p) {
         // measured defaults at 2^18 samples: time pass 2048 elements per
         // tile; spectral pass 1024, two tiles per workgroup for complex64
