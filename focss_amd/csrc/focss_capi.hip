@@ -133,7 +133,7 @@ struct focss_ctx {
     size_t pad_tmp_bytes = 0;
     bool pad_uniform = true;  // FOCSS_BLUESTEIN_UNIFORM=1: UNIFORM programs through Bluestein too
     bool energy_comp = true;  // FOCSS_FP32_ENERGY=0: complex64 programs without the compensated energy
-    bool split_classes = true;  // FOCSS_SPLIT_CLASSES=0: a large two-pass class stays one chain of the graph
+    int split_parts = 2;  // FOCSS_SPLIT_CLASSES=k: chains of a large two-pass class in the graph (1 = one chain)
     void* d_genwave = nullptr;  // [cap] GenWave
     int* d_active = nullptr;    // [cap]
     int graph_pairs = 48;  // multiple of 6 (state parity 2, reduction slots 3)
@@ -887,22 +887,20 @@ int run_links(focss_ctx* c, int precision, int n, const focss_link* links, void*
                                                  " modes=" + std::to_string(lc.modes));
         }
     }
-    // in the captured graph a large two-pass class runs as two chains of half
-    // its waveforms on separate lanes, so that one half's time pass overlaps
-    // the other's spectral pass across the kernel boundaries (each waveform's
-    // arithmetic is the same: the results do not change)
-    if (c->use_graph && c->lanes > 1 && c->split_classes && profile_pairs == 0) {
+    // in the captured graph a large two-pass class runs as split_parts chains
+    // of its waveforms on separate lanes (two by default), so that one part's
+    // time pass overlaps another's spectral pass across the kernel boundaries
+    // (each waveform's arithmetic is the same: the results do not change)
+    const int parts = std::min(c->split_parts, c->lanes);
+    if (c->use_graph && parts > 1 && profile_pairs == 0) {
         std::vector<LengthClass> split;
         for (const LengthClass& lc : two_pass) {
-            if (lc.count >= kMinSplitWaves) {
+            const int k = lc.count >= kMinSplitWaves ? parts : 1;
+            for (int p = 0; p < k; ++p) {
                 LengthClass h = lc;
-                h.count = lc.count / 2;
+                h.first = lc.first + (int)((long long)lc.count * p / k);
+                h.count = (int)((long long)lc.count * (p + 1) / k) - (int)((long long)lc.count * p / k);
                 split.push_back(h);
-                h.first = lc.first + h.count;
-                h.count = lc.count - lc.count / 2;
-                split.push_back(h);
-            } else {
-                split.push_back(lc);
             }
         }
         two_pass.swap(split);
@@ -1683,7 +1681,7 @@ int focss_ctx_create(int device, focss_ctx** out) {
     const char* fe = getenv("FOCSS_FP32_ENERGY");
     c->energy_comp = !(fe && fe[0] == '0');
     const char* sc = getenv("FOCSS_SPLIT_CLASSES");
-    c->split_classes = !(sc && sc[0] == '0');
+    if (sc && sc[0]) c->split_parts = std::max(1, std::min(atoi(sc), (int)focss_ctx::kLanes));
     for (int p = 0; p <= 20; ++p) {
         // measured defaults at 2^18 samples: time pass 2048 elements per
         // tile; spectral pass 1024, two tiles per workgroup for complex64
